@@ -1,0 +1,145 @@
+/*
+ * stwo_hip.h — C-ABI of the MI355X (gfx950) backend for the Stwo Circle-STARK
+ * prover.  Each entry point replaces one method of the reference's backend trait
+ * family (crates/stwo/src/prover/backend/mod.rs:18-66); the comment above it
+ * cites the trait method and the SimdBackend implementation it stands in for.
+ *
+ * Conventions
+ *  - Every function returns SH_OK (0) or a negative SH_ERR_* code; the message
+ *    of the last failure is available from sh_last_error().  The reference
+ *    panics (assert!) where these return SH_ERR_INVALID.
+ *  - Device buffers are plain device pointers (from sh_alloc or any HIP
+ *    allocation); "host arrays of device pointers" are ordinary host arrays
+ *    whose elements are device addresses.
+ *  - M31 words are canonical, in [0, 2^31 - 1).  A QM31 column is 4 device
+ *    arrays, one per coordinate (SecureColumnByCoords,
+ *    crates/stwo/src/prover/secure_column.rs:13-63).  QM31 scalars are 4 words
+ *    [a.0, a.1, b.0, b.1]; QM31 circle points are 8 words (x then y).
+ *  - Evaluations are in bit-reversed order of the circle domain; coefficients
+ *    are in the FFT basis order of CpuBackend (natural order).
+ *  - Hashes are 32-byte Blake2s-256 digests, stored contiguously.
+ *  - Work is enqueued on the context's stream; downloads and sh_sync block.
+ *  - Contexts are independent; use one per host thread.
+ */
+#ifndef STWO_HIP_H
+#define STWO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SH_OK 0
+#define SH_ERR_INVALID (-1) /* bad sizes/arguments (a reference assert!) */
+#define SH_ERR_OOM (-2)
+#define SH_ERR_HIP (-3)
+#define SH_ERR_NOT_FOUND (-4)
+
+typedef struct sh_ctx sh_ctx;
+typedef struct sh_twiddles sh_twiddles;
+
+/* ---------------------------------------------------------------- context */
+/* The reference backend is a zero-sized marker type
+ * (crates/stwo/src/prover/backend/simd/mod.rs:38-45); a context carries the
+ * device state the GPU needs instead (stream, staging ring, scratch). */
+int sh_ctx_create(int device, sh_ctx **out);
+int sh_ctx_destroy(sh_ctx *ctx);
+const char *sh_last_error(const sh_ctx *ctx);
+int sh_sync(sh_ctx *ctx);
+/* Span timing by class, mirroring the reference's tracing classes
+ * (crates/stwo/src/tracing/mod.rs:17-104): hipEvents around every launch. */
+int sh_profile_enable(sh_ctx *ctx, int on);
+/* Writes a JSON object {"class": {"n": launches, "ms": total}, ...} and resets. */
+int sh_profile_report(sh_ctx *ctx, char *buf, uint64_t cap);
+/* Stream-ordered wall clock of the context stream (hipEvent pair). */
+int sh_timer_start(sh_ctx *ctx);
+int sh_timer_stop(sh_ctx *ctx, float *ms);
+
+/* ---------------------------------------------------------------- columns */
+/* Column::zeros / uninitialized / to_cpu / from_iter
+ * (crates/stwo/src/prover/backend/mod.rs:45-66). */
+int sh_alloc(sh_ctx *ctx, uint64_t bytes, void **d_out);
+int sh_free(sh_ctx *ctx, void *d);
+int sh_upload(sh_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
+int sh_download(sh_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
+int sh_memset(sh_ctx *ctx, void *d_dst, int value, uint64_t bytes);
+/* ColumnOps::bit_reverse_column (crates/stwo/src/prover/backend/mod.rs:37-43;
+ * SIMD crates/stwo/src/prover/backend/simd/bit_reverse.rs:25-107). */
+int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
+
+/* ---------------------------------------------------------------- PolyOps */
+/* PolyOps::precompute_twiddles (crates/stwo/src/prover/poly/circle/ops.rs:143;
+ * SIMD crates/stwo/src/prover/backend/simd/circle.rs:431-462).  Root coset is
+ * Coset(root_initial_index, root_log_size); buffers hold 2^root_log_size words
+ * in CpuBackend's layout (crates/stwo/src/prover/backend/cpu/circle.rs:274-285). */
+int sh_twiddles_create(sh_ctx *ctx, uint32_t root_initial_index, uint32_t root_log_size,
+                       sh_twiddles **out);
+int sh_twiddles_destroy(sh_ctx *ctx, sh_twiddles *tw);
+int sh_twiddles_download(sh_ctx *ctx, const sh_twiddles *tw, uint32_t *h_tw, uint32_t *h_itw);
+/* PolyOps::interpolate (crates/stwo/src/prover/poly/circle/ops.rs:25-28;
+ * SIMD crates/stwo/src/prover/backend/simd/circle.rs:135-163).  In place:
+ * bit-reversed evaluations on CanonicCoset(log_size) -> coefficients. */
+int sh_interpolate(sh_ctx *ctx, const sh_twiddles *tw, uint32_t *d_values, uint32_t log_size);
+/* PolyOps::interpolate_columns (crates/stwo/src/prover/poly/circle/ops.rs:30-40):
+ * many columns of uneven sizes in one call (batched launches per size class). */
+int sh_interpolate_columns(sh_ctx *ctx, const sh_twiddles *tw, uint32_t *const *d_cols,
+                           const uint32_t *log_sizes, uint64_t n);
+/* PolyOps::evaluate / evaluate_into (crates/stwo/src/prover/poly/circle/ops.rs:94-107;
+ * SIMD crates/stwo/src/prover/backend/simd/circle.rs:389-425): coefficients of
+ * length 2^coeff_log -> bit-reversed evaluations on CanonicCoset(domain_log). */
+int sh_evaluate(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *d_coeffs, uint32_t coeff_log,
+                uint32_t *d_out, uint32_t domain_log);
+/* PolyOps::evaluate_polynomials (crates/stwo/src/prover/poly/circle/ops.rs:109-140). */
+int sh_evaluate_polynomials(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_coeffs,
+                            const uint32_t *coeff_logs, uint64_t n, uint32_t log_blowup,
+                            uint32_t *const *d_outs);
+
+/* ---------------------------------------------------------------- Merkle */
+/* MerkleOpsLifted::build_leaves (crates/stwo/src/prover/vcs_lifted/ops.rs:14-15;
+ * SIMD crates/stwo/src/prover/backend/simd/blake2s_lifted.rs:46-229).  Columns
+ * must be sorted ascending by size (as the trait requires); d_out receives
+ * 2^lifting_log_size hashes (one hash when n == 0). */
+int sh_merkle_build_leaves(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *log_sizes,
+                           uint64_t n, uint32_t lifting_log_size, uint8_t *d_out);
+/* MerkleOpsLifted::build_next_layer (crates/stwo/src/prover/vcs_lifted/ops.rs:19;
+ * SIMD crates/stwo/src/prover/backend/simd/blake2s_lifted.rs:232-263). */
+int sh_merkle_build_next_layer(sh_ctx *ctx, const uint8_t *d_prev, uint32_t prev_log_size,
+                               uint8_t *d_out);
+/* MerkleProverLifted::commit (crates/stwo/src/prover/vcs_lifted/prover.rs:35-80):
+ * stable-sorts columns ascending, builds the leaves and every layer.
+ * d_layers[k] (k = 0..lifting_log_size) receives the 2^k hashes of layer k
+ * (layer 0 is the root); root (host) receives the root. */
+int sh_merkle_commit_lifted(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *log_sizes,
+                            uint64_t n, uint32_t lifting_log_size, uint8_t *const *d_layers,
+                            uint8_t root[32]);
+/* MerkleOps::commit_on_layer (crates/stwo/src/prover/vcs/ops.rs:24-29;
+ * SIMD crates/stwo/src/prover/backend/simd/blake2s.rs:60-143): mixed-degree
+ * node hash Blake2s(prev[2i] || prev[2i+1] || LE32(col[i]) ...); d_prev may be NULL. */
+int sh_merkle_commit_on_layer(sh_ctx *ctx, uint32_t log_size, const uint8_t *d_prev,
+                              const uint32_t *const *d_cols, uint64_t n, uint8_t *d_out);
+/* MerkleProver::commit (crates/stwo/src/prover/vcs/prover.rs:39-59): columns
+ * stable-sorted descending, commit_on_layer for log_size = max..0.
+ * d_layers[k] receives layer k (2^k hashes). */
+int sh_merkle_commit_mixed(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *log_sizes,
+                           uint64_t n, uint8_t *const *d_layers, uint8_t root[32]);
+
+/* ---------------------------------------------------------------- trace commit */
+/* TreeBuilder::extend_evals + CommitmentTreeProver::new
+ * (crates/stwo/src/prover/pcs/mod.rs:325-334, 360-386): interpolate every
+ * column of bit-reversed evaluations d_evals[i] into coefficients d_coeffs[i]
+ * (may alias d_evals[i]), evaluate each on CanonicCoset(log_size + log_blowup)
+ * into d_ldes[i], and commit the lifted Merkle tree of the LDEs with lifting
+ * size max(log_size) + log_blowup (d_layers[k] receives layer k, 2^k hashes).
+ * Batched: one launch chain per size class for the FFTs, grouped leaf hashing
+ * with hash-state sharing, fused inner layers.  root may be NULL (no sync). */
+int sh_commit_trace(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_evals,
+                    uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
+                    uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
+                    uint8_t root[32]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,71 @@
+"""ctypes binding of libstwo_hip.so (the C-ABI declared in include/stwo_hip.h).
+
+The library must have been built (``python -m proving_amd.build`` or
+``__graft_entry__.build()``); loading fails loudly otherwise - there is no
+CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstwo_hip.so")
+
+c_int, c_u32, c_u64, c_vp, c_char_p = (ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                      ctypes.c_void_p, ctypes.c_char_p)
+c_f32p = ctypes.POINTER(ctypes.c_float)
+c_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "sh_ctx_create": (c_int, [c_int, ctypes.POINTER(c_vp)]),
+    "sh_ctx_destroy": (c_int, [c_vp]),
+    "sh_last_error": (c_char_p, [c_vp]),
+    "sh_sync": (c_int, [c_vp]),
+    "sh_profile_enable": (c_int, [c_vp, c_int]),
+    "sh_profile_report": (c_int, [c_vp, c_char_p, c_u64]),
+    "sh_timer_start": (c_int, [c_vp]),
+    "sh_timer_stop": (c_int, [c_vp, c_f32p]),
+    "sh_alloc": (c_int, [c_vp, c_u64, ctypes.POINTER(c_vp)]),
+    "sh_free": (c_int, [c_vp, c_vp]),
+    "sh_upload": (c_int, [c_vp, c_vp, c_vp, c_u64]),
+    "sh_download": (c_int, [c_vp, c_vp, c_vp, c_u64]),
+    "sh_memset": (c_int, [c_vp, c_vp, c_int, c_u64]),
+    "sh_bit_reverse_column": (c_int, [c_vp, c_vp, c_u32]),
+    "sh_twiddles_create": (c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
+    "sh_twiddles_destroy": (c_int, [c_vp, c_vp]),
+    "sh_twiddles_download": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "sh_interpolate": (c_int, [c_vp, c_vp, c_vp, c_u32]),
+    "sh_interpolate_columns": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64]),
+    "sh_evaluate": (c_int, [c_vp, c_vp, c_vp, c_u32, c_vp, c_u32]),
+    "sh_evaluate_polynomials": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
+    "sh_merkle_build_leaves": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
+    "sh_merkle_build_next_layer": (c_int, [c_vp, c_vp, c_u32, c_vp]),
+    "sh_merkle_commit_lifted": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp]),
+    "sh_merkle_commit_on_layer": (c_int, [c_vp, c_u32, c_vp, c_vp, c_u64, c_vp]),
+    "sh_merkle_commit_mixed": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
+    "sh_commit_trace": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libstwo_hip.so and declares every C-ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m proving_amd.build` "
+            "(the HIP backend has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,298 @@
+"""Host-side mirror of the Stwo backend trait family over the HIP C-ABI.
+
+``HipBackend`` exposes the reference's trait methods under their reference
+names (``PolyOps``, ``MerkleOpsLifted``, ``MerkleOps``, ``FriOps``,
+``QuotientOps``, ``AccumulationOps``, ``GrindOps``; see
+crates/stwo/src/prover/backend/mod.rs:18-66) on device-resident columns.
+Argument meaning and error behaviour follow the reference: where it would
+panic (``assert!``) the C-ABI returns SH_ERR_INVALID and this module raises
+``HipError``.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import HipError, load
+
+P = (1 << 31) - 1
+
+
+def _ptrs(bufs):
+    arr = (ctypes.c_void_p * max(1, len(bufs)))(*[b.ptr for b in bufs])
+    return arr
+
+
+def _u32s(vals):
+    return (ctypes.c_uint32 * max(1, len(vals)))(*[int(v) for v in vals])
+
+
+class Context:
+    """A device, its stream and staging state (one per host thread)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.sh_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise HipError(f"sh_ctx_create({device}) failed with code {rc}")
+        self.h = h
+        self._closed = False
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            msg = self.lib.sh_last_error(self.h).decode(errors="replace")
+            raise HipError(f"{what}: code {rc}: {msg}")
+
+    def close(self):
+        if not self._closed:
+            self.lib.sh_ctx_destroy(self.h)
+            self._closed = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        self.check(self.lib.sh_sync(self.h), "sh_sync")
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def profile(self, on=True):
+        self.check(self.lib.sh_profile_enable(self.h, 1 if on else 0), "profile")
+
+    def profile_report(self):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        self.check(self.lib.sh_profile_report(self.h, buf, len(buf)), "profile_report")
+        return json.loads(buf.value.decode())
+
+    def timer_start(self):
+        self.check(self.lib.sh_timer_start(self.h), "timer_start")
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        self.check(self.lib.sh_timer_stop(self.h, ctypes.byref(ms)), "timer_stop")
+        return ms.value
+
+
+class DeviceBuffer:
+    """An owned device allocation."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        ctx.check(ctx.lib.sh_alloc(ctx.h, self.nbytes, ctypes.byref(p)), "sh_alloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr is not None and not self.ctx._closed:
+            self.ctx.lib.sh_free(self.ctx.h, ctypes.c_void_p(self.ptr))
+        self.ptr = None
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.sh_upload(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes),
+                       "sh_upload")
+
+    def download(self, dtype=np.uint32, count=None):
+        item = np.dtype(dtype).itemsize
+        n = self.nbytes // item if count is None else count
+        out = np.empty(n, dtype)
+        self.ctx.check(self.ctx.lib.sh_download(self.ctx.h, out.ctypes.data, self.ptr, n * item),
+                       "sh_download")
+        return out
+
+
+class BaseColumn(DeviceBuffer):
+    """Col<HipBackend, BaseField>: 2^log_size canonical M31 words on the device."""
+
+    def __init__(self, ctx, log_size):
+        super().__init__(ctx, 4 << log_size)
+        self.log_size = log_size
+
+    def __len__(self):
+        return 1 << self.log_size
+
+    @classmethod
+    def from_cpu(cls, ctx, values):
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        n = len(v)
+        assert n and n & (n - 1) == 0, "column length must be a power of two"
+        c = cls(ctx, n.bit_length() - 1)
+        c.upload(v)
+        return c
+
+    def to_cpu(self):
+        return self.download(np.uint32, len(self))
+
+    def bit_reverse(self):
+        self.ctx.check(self.ctx.lib.sh_bit_reverse_column(self.ctx.h, self.ptr, self.log_size),
+                       "bit_reverse_column")
+
+
+class SecureColumnByCoords:
+    """QM31 column stored as 4 coordinate columns (crates/stwo/src/prover/secure_column.rs:13-63)."""
+
+    def __init__(self, ctx, log_size, columns=None):
+        self.ctx = ctx
+        self.log_size = log_size
+        self.columns = columns or [BaseColumn(ctx, log_size) for _ in range(4)]
+
+    @classmethod
+    def from_cpu(cls, ctx, values):
+        v = np.ascontiguousarray(values, dtype=np.uint32)
+        assert v.shape[0] == 4
+        return cls(ctx, v.shape[1].bit_length() - 1, [BaseColumn.from_cpu(ctx, v[c]) for c in range(4)])
+
+    def to_cpu(self):
+        return np.stack([c.to_cpu() for c in self.columns])
+
+    def __len__(self):
+        return 1 << self.log_size
+
+    def free(self):
+        for c in self.columns:
+            c.free()
+
+
+class HashLayer(DeviceBuffer):
+    """Col<HipBackend, Blake2sHash>: 2^log_size 32-byte hashes."""
+
+    def __init__(self, ctx, log_size):
+        super().__init__(ctx, 32 << log_size)
+        self.log_size = log_size
+
+    def to_cpu(self):
+        return self.download(np.uint8, 32 << self.log_size).reshape(-1, 32)
+
+
+class TwiddleTree:
+    """TwiddleTree<HipBackend> (crates/stwo/src/prover/poly/twiddles.rs:10-15)."""
+
+    def __init__(self, ctx, root_initial_index, root_log_size):
+        self.ctx = ctx
+        self.root_initial_index = root_initial_index
+        self.root_log_size = root_log_size
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.sh_twiddles_create(ctx.h, root_initial_index, root_log_size, ctypes.byref(h)),
+                  "precompute_twiddles")
+        self.h = h
+
+    def to_cpu(self):
+        n = 1 << self.root_log_size
+        tw = np.empty(n, np.uint32)
+        itw = np.empty(n, np.uint32)
+        self.ctx.check(self.ctx.lib.sh_twiddles_download(self.ctx.h, self.h, tw.ctypes.data,
+                                                         itw.ctypes.data), "twiddles_download")
+        return tw, itw
+
+    def free(self):
+        if self.h is not None and not self.ctx._closed:
+            self.ctx.lib.sh_twiddles_destroy(self.ctx.h, self.h)
+        self.h = None
+
+
+def canonic_half_initial(log_size):
+    """CanonicCoset(log_size).half_coset initial index (half_odds(log_size - 1))."""
+    return 1 << (31 - (log_size + 1))
+
+
+class HipBackend:
+    """The MI355X backend: one method per reference trait method."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = ctx.h
+
+    def _chk(self, rc, what):
+        self.ctx.check(rc, what)
+
+    # ---- PolyOps (crates/stwo/src/prover/poly/circle/ops.rs:18-165) -------------------------
+    def precompute_twiddles(self, root_initial_index, root_log_size):
+        return TwiddleTree(self.ctx, root_initial_index, root_log_size)
+
+    def precompute_canonic_twiddles(self, domain_log):
+        """Twiddles of CanonicCoset(domain_log).half_coset."""
+        return self.precompute_twiddles(canonic_half_initial(domain_log), domain_log - 1)
+
+    def interpolate(self, col, twiddles):
+        """In place: bit-reversed evaluations on CanonicCoset(col.log_size) -> coefficients."""
+        self._chk(self.lib.sh_interpolate(self.h, twiddles.h, col.ptr, col.log_size), "interpolate")
+        return col
+
+    def interpolate_columns(self, cols, twiddles):
+        self._chk(self.lib.sh_interpolate_columns(self.h, twiddles.h, _ptrs(cols),
+                                                  _u32s([c.log_size for c in cols]), len(cols)),
+                  "interpolate_columns")
+        return cols
+
+    def evaluate(self, coeffs, domain_log, twiddles, out=None):
+        out = out or BaseColumn(self.ctx, domain_log)
+        self._chk(self.lib.sh_evaluate(self.h, twiddles.h, coeffs.ptr, coeffs.log_size, out.ptr,
+                                       domain_log), "evaluate")
+        return out
+
+    def evaluate_polynomials(self, polys, log_blowup, twiddles, outs=None):
+        outs = outs or [BaseColumn(self.ctx, p.log_size + log_blowup) for p in polys]
+        self._chk(self.lib.sh_evaluate_polynomials(self.h, twiddles.h, _ptrs(polys),
+                                                   _u32s([p.log_size for p in polys]), len(polys),
+                                                   log_blowup, _ptrs(outs)), "evaluate_polynomials")
+        return outs
+
+    # ---- MerkleOpsLifted (crates/stwo/src/prover/vcs_lifted/ops.rs:10-20) ----------------------
+    def build_leaves(self, cols_sorted_asc, lifting_log_size):
+        out = HashLayer(self.ctx, lifting_log_size if cols_sorted_asc else 0)
+        self._chk(self.lib.sh_merkle_build_leaves(self.h, _ptrs(cols_sorted_asc),
+                                                  _u32s([c.log_size for c in cols_sorted_asc]),
+                                                  len(cols_sorted_asc), lifting_log_size, out.ptr),
+                  "build_leaves")
+        return out
+
+    def build_next_layer(self, prev):
+        out = HashLayer(self.ctx, prev.log_size - 1)
+        self._chk(self.lib.sh_merkle_build_next_layer(self.h, prev.ptr, prev.log_size, out.ptr),
+                  "build_next_layer")
+        return out
+
+    def commit_lifted(self, cols, lifting_log_size):
+        """MerkleProverLifted::commit: returns (layers root-first, root bytes)."""
+        layers = [HashLayer(self.ctx, k) for k in range(lifting_log_size + 1)]
+        root = (ctypes.c_uint8 * 32)()
+        self._chk(self.lib.sh_merkle_commit_lifted(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
+                                                   len(cols), lifting_log_size, _ptrs(layers), root),
+                  "commit_lifted")
+        return layers, bytes(root)
+
+    # ---- MerkleOps (crates/stwo/src/prover/vcs/ops.rs:8-29) ------------------------------------
+    def commit_on_layer(self, log_size, prev, cols):
+        out = HashLayer(self.ctx, log_size)
+        self._chk(self.lib.sh_merkle_commit_on_layer(self.h, log_size, prev.ptr if prev else None,
+                                                     _ptrs(cols), len(cols), out.ptr), "commit_on_layer")
+        return out
+
+    def commit_mixed(self, cols):
+        max_log = max(c.log_size for c in cols) if cols else 0
+        layers = [HashLayer(self.ctx, k) for k in range(max_log + 1)]
+        root = (ctypes.c_uint8 * 32)()
+        self._chk(self.lib.sh_merkle_commit_mixed(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
+                                                  len(cols), _ptrs(layers), root), "commit_mixed")
+        return layers, bytes(root)
+
+    # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------
+    def commit_trace(self, evals, log_blowup, twiddles, coeffs=None, ldes=None, layers=None,
+                     want_root=True):
+        coeffs = coeffs or [BaseColumn(self.ctx, e.log_size) for e in evals]
+        ldes = ldes or [BaseColumn(self.ctx, e.log_size + log_blowup) for e in evals]
+        L = max(e.log_size for e in evals) + log_blowup
+        layers = layers or [HashLayer(self.ctx, k) for k in range(L + 1)]
+        root = (ctypes.c_uint8 * 32)() if want_root else None
+        self._chk(self.lib.sh_commit_trace(self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs),
+                                           _u32s([e.log_size for e in evals]), len(evals), log_blowup,
+                                           _ptrs(ldes), _ptrs(layers), root), "commit_trace")
+        return coeffs, ldes, layers, (bytes(root) if want_root else None)

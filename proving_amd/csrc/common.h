@@ -1,0 +1,112 @@
+// Internal definitions shared by the translation units of libstwo_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/stwo_hip.h"
+#include "m31.h"
+
+// One context per host thread: a device, its stream, a staging ring for the
+// small per-call tables (column pointer lists) and reusable scratch buffers.
+struct sh_prof_rec {
+  const char* cls;
+  hipEvent_t start, stop;
+};
+
+struct sh_scratch {
+  void* ptr = nullptr;
+  uint64_t bytes = 0;
+};
+
+struct sh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // pinned host ring + device mirror for per-call tables
+  uint8_t* h_ring = nullptr;
+  uint8_t* d_ring = nullptr;
+  uint64_t ring_size = 0, ring_off = 0;
+  hipEvent_t ring_fence = nullptr;
+  bool ring_fence_valid = false;
+  // profiling (hipEvent pairs around each kernel launch, tagged by span class)
+  bool profile = false;
+  std::vector<sh_prof_rec> prof;
+  std::vector<hipEvent_t> event_pool;
+  // named scratch buffers
+  sh_scratch scratch[8];
+  hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+};
+
+struct sh_twiddles {
+  uint32_t root_initial_index;
+  uint32_t root_log_size;  // K: buffers hold 2^K words
+  uint32_t* tw;            // device
+  uint32_t* itw;           // device
+};
+
+enum {
+  SCR_MERKLE_STATE_A = 0,
+  SCR_MERKLE_STATE_B = 1,
+  SCR_MERKLE_TMP = 2,
+  SCR_FFT_TMP = 3,
+  SCR_MISC = 4,
+  SCR_QUOT = 5,
+  SCR_GRIND = 6,
+};
+
+int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...);
+#define SH_CHECK_HIP(ctx, expr)                                                   \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess)                                                        \
+      return sh_fail(ctx, SH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+#define SH_REQUIRE(ctx, cond, ...)                       \
+  do {                                                   \
+    if (!(cond)) return sh_fail(ctx, SH_ERR_INVALID, __VA_ARGS__); \
+  } while (0)
+#define SH_TRY(expr)            \
+  do {                          \
+    int _r = (expr);            \
+    if (_r != SH_OK) return _r; \
+  } while (0)
+
+// Copies a small host table into the device ring (stream-ordered) and returns
+// its device address.
+int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev);
+// Ensures scratch buffer `slot` holds at least `bytes` and returns it.
+int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out);
+// Profiling spans around launches.
+void sh_prof_begin(sh_ctx* ctx, const char* cls, sh_prof_rec* rec);
+void sh_prof_end(sh_ctx* ctx, sh_prof_rec* rec);
+int sh_launch_check(sh_ctx* ctx, const char* what);
+
+// Profiling span: RAII pair of events on the context stream.
+struct SpanGuard {
+  sh_ctx* ctx;
+  sh_prof_rec rec;
+  SpanGuard(sh_ctx* c, const char* cls) : ctx(c) { sh_prof_begin(ctx, cls, &rec); }
+  ~SpanGuard() { sh_prof_end(ctx, &rec); }
+};
+
+// Host helpers for circle-group indices (crates/stwo/src/core/circle.rs).
+namespace sh {
+// CirclePointIndex::to_point: M31_CIRCLE_GEN * index by double-and-add.
+mpt index_to_point_host(uint32_t index);
+// Table of M31_CIRCLE_GEN * 2^b for b = 0..30, for device-side index_to_point.
+const mpt* gen_pow2_table_host();
+// True if coset (init, log) is `root` doubled (root_log - log) times (Coset::is_doubling_of).
+bool is_doubling_of(uint32_t init, uint32_t log, uint32_t root_init, uint32_t root_log);
+}  // namespace sh
+
+// Device-side index -> point from the 31-entry power table.
+__device__ __forceinline__ sh::mpt sh_index_to_point(uint32_t idx, const sh::mpt* gen_pow2) {
+  sh::mpt r = {1, 0};
+#pragma unroll 1
+  for (int b = 0; b < 31; b++)
+    if ((idx >> b) & 1u) r = sh::padd(r, gen_pow2[b]);
+  return r;
+}

@@ -1,0 +1,301 @@
+// Context, device memory, staging ring, profiling spans and column utilities.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+
+#include "common.h"
+
+static const uint64_t kRingBytes = 4ull << 20;
+
+int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...) {
+  if (ctx) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->last_error = buf;
+  }
+  return code;
+}
+
+int sh_launch_check(sh_ctx* ctx, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return sh_fail(ctx, SH_ERR_HIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return SH_OK;
+}
+
+extern "C" int sh_ctx_create(int device, sh_ctx** out) {
+  if (!out) return SH_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return SH_ERR_HIP;
+  if (device < 0 || device >= count) return SH_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return SH_ERR_HIP;
+  sh_ctx* ctx = new sh_ctx();
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->h_ring, kRingBytes, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void**)&ctx->d_ring, kRingBytes) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ring_fence, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&ctx->timer_start) != hipSuccess || hipEventCreate(&ctx->timer_stop) != hipSuccess) {
+    sh_ctx_destroy(ctx);
+    return SH_ERR_HIP;
+  }
+  ctx->ring_size = kRingBytes;
+  *out = ctx;
+  return SH_OK;
+}
+
+extern "C" int sh_ctx_destroy(sh_ctx* ctx) {
+  if (!ctx) return SH_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->prof) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  for (auto e : ctx->event_pool) hipEventDestroy(e);
+  for (auto& s : ctx->scratch)
+    if (s.ptr) hipFree(s.ptr);
+  if (ctx->d_ring) hipFree(ctx->d_ring);
+  if (ctx->h_ring) hipHostFree(ctx->h_ring);
+  if (ctx->ring_fence) hipEventDestroy(ctx->ring_fence);
+  if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
+  if (ctx->timer_stop) hipEventDestroy(ctx->timer_stop);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SH_OK;
+}
+
+extern "C" const char* sh_last_error(const sh_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+extern "C" int sh_sync(sh_ctx* ctx) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SH_OK;
+}
+
+// The ring is reused front to back; before wrapping, wait until the copies
+// and kernels that read the previous round of tables have completed.
+int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
+  uint64_t need = (bytes + 255) & ~255ull;
+  SH_REQUIRE(ctx, need <= ctx->ring_size, "per-call table of %llu bytes exceeds the staging ring",
+             (unsigned long long)bytes);
+  if (ctx->ring_off + need > ctx->ring_size) {
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->ring_off = 0;
+  }
+  uint8_t* h = ctx->h_ring + ctx->ring_off;
+  uint8_t* d = ctx->d_ring + ctx->ring_off;
+  memcpy(h, host, bytes);
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ctx->ring_off += need;
+  *dev = d;
+  return SH_OK;
+}
+
+int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out) {
+  sh_scratch& s = ctx->scratch[slot];
+  if (s.bytes < bytes) {
+    if (s.ptr) {
+      SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      SH_CHECK_HIP(ctx, hipFree(s.ptr));
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    if (hipMalloc(&s.ptr, bytes) != hipSuccess) {
+      s.ptr = nullptr;
+      return sh_fail(ctx, SH_ERR_OOM, "scratch allocation of %llu bytes failed", (unsigned long long)bytes);
+    }
+    s.bytes = bytes;
+  }
+  *out = s.ptr;
+  return SH_OK;
+}
+
+static hipEvent_t pool_event(sh_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+void sh_prof_begin(sh_ctx* ctx, const char* cls, sh_prof_rec* rec) {
+  rec->cls = cls;
+  rec->start = rec->stop = nullptr;
+  if (!ctx->profile) return;
+  rec->start = pool_event(ctx);
+  hipEventRecord(rec->start, ctx->stream);
+}
+
+void sh_prof_end(sh_ctx* ctx, sh_prof_rec* rec) {
+  if (!ctx->profile || !rec->start) return;
+  rec->stop = pool_event(ctx);
+  hipEventRecord(rec->stop, ctx->stream);
+  ctx->prof.push_back(*rec);
+}
+
+extern "C" int sh_profile_enable(sh_ctx* ctx, int on) {
+  if (!ctx) return SH_ERR_INVALID;
+  ctx->profile = on != 0;
+  return SH_OK;
+}
+
+extern "C" int sh_profile_report(sh_ctx* ctx, char* buf, uint64_t cap) {
+  if (!ctx || !buf || cap == 0) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::map<std::string, std::pair<uint64_t, double>> agg;
+  for (auto& r : ctx->prof) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, r.start, r.stop);
+    auto& a = agg[r.cls];
+    a.first += 1;
+    a.second += ms;
+    ctx->event_pool.push_back(r.start);
+    ctx->event_pool.push_back(r.stop);
+  }
+  ctx->prof.clear();
+  std::string s = "{";
+  bool first = true;
+  for (auto& kv : agg) {
+    char item[256];
+    snprintf(item, sizeof(item), "%s\"%s\": {\"n\": %llu, \"ms\": %.6f}", first ? "" : ", ",
+             kv.first.c_str(), (unsigned long long)kv.second.first, kv.second.second);
+    s += item;
+    first = false;
+  }
+  s += "}";
+  if (s.size() + 1 > cap) return sh_fail(ctx, SH_ERR_INVALID, "profile report needs %zu bytes", s.size() + 1);
+  memcpy(buf, s.c_str(), s.size() + 1);
+  return SH_OK;
+}
+
+extern "C" int sh_timer_start(sh_ctx* ctx) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipEventRecord(ctx->timer_start, ctx->stream));
+  return SH_OK;
+}
+
+extern "C" int sh_timer_stop(sh_ctx* ctx, float* ms) {
+  if (!ctx || !ms) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipEventRecord(ctx->timer_stop, ctx->stream));
+  SH_CHECK_HIP(ctx, hipEventSynchronize(ctx->timer_stop));
+  SH_CHECK_HIP(ctx, hipEventElapsedTime(ms, ctx->timer_start, ctx->timer_stop));
+  return SH_OK;
+}
+
+extern "C" int sh_alloc(sh_ctx* ctx, uint64_t bytes, void** d_out) {
+  if (!ctx || !d_out) return SH_ERR_INVALID;
+  *d_out = nullptr;
+  if (bytes == 0) bytes = 4;
+  if (hipMalloc(d_out, bytes) != hipSuccess) {
+    *d_out = nullptr;
+    return sh_fail(ctx, SH_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+  }
+  return SH_OK;
+}
+
+extern "C" int sh_free(sh_ctx* ctx, void* d) {
+  if (!ctx) return SH_ERR_INVALID;
+  if (!d) return SH_OK;
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SH_CHECK_HIP(ctx, hipFree(d));
+  return SH_OK;
+}
+
+extern "C" int sh_upload(sh_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SH_OK;
+}
+
+extern "C" int sh_download(sh_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SH_OK;
+}
+
+// Fill kernel (not hipMemsetAsync, so the call stays safe inside a captured graph).
+__global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+extern "C" int sh_memset(sh_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
+  if (!ctx) return SH_ERR_INVALID;
+  if (bytes == 0) return SH_OK;
+  uint64_t blocks = (bytes + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_fill_u8<<<(unsigned)blocks, 256, 0, ctx->stream>>>((uint8_t*)d_dst, (uint8_t)value, bytes);
+  return sh_launch_check(ctx, "fill");
+}
+
+// bit_reverse_column: swaps i <-> bitrev(i) for i < bitrev(i).
+__global__ void k_bit_reverse(uint32_t* col, uint32_t log_n) {
+  uint64_t n = 1ull << log_n;
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t j = sh::bitrev((uint32_t)i, log_n);
+  if (j > i) {
+    uint32_t a = col[i], b = col[j];
+    col[i] = b;
+    col[j] = a;
+  }
+}
+
+extern "C" int sh_bit_reverse_column(sh_ctx* ctx, uint32_t* d_col, uint32_t log_size) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, log_size <= 31, "log_size %u too large", log_size);
+  uint64_t n = 1ull << log_size;
+  SpanGuard span(ctx, "bit_reverse");
+  k_bit_reverse<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(d_col, log_size);
+  return sh_launch_check(ctx, "bit_reverse");
+}
+
+namespace sh {
+mpt index_to_point_host(uint32_t index) {
+  mpt res = {1, 0}, cur = {2, 1268011823u};  // M31_CIRCLE_GEN (crates/stwo/src/core/circle.rs:181-182)
+  uint32_t s = index & 0x7fffffffu;
+  while (s) {
+    if (s & 1) res = padd(res, cur);
+    cur = padd(cur, cur);
+    s >>= 1;
+  }
+  return res;
+}
+
+const mpt* gen_pow2_table_host() {
+  static mpt table[31];
+  static bool init = false;
+  if (!init) {
+    mpt cur = {2, 1268011823u};
+    for (int b = 0; b < 31; b++) {
+      table[b] = cur;
+      cur = padd(cur, cur);
+    }
+    init = true;
+  }
+  return table;
+}
+
+bool is_doubling_of(uint32_t init, uint32_t log, uint32_t root_init, uint32_t root_log) {
+  if (log > root_log) return false;
+  uint32_t d = root_log - log;
+  uint64_t ri = root_init & 0x7fffffffu;
+  for (uint32_t k = 0; k < d; k++) ri = (ri * 2) & 0x7fffffffull;
+  return (uint32_t)ri == (init & 0x7fffffffu);
+}
+}  // namespace sh

@@ -1,0 +1,356 @@
+// Circle FFT: interpolate (iFFT) and evaluate (FFT / low-degree extension).
+//
+// Algorithm (CpuBackend, crates/stwo/src/prover/backend/cpu/circle.rs:26-79,
+// 178-238): on a circle domain of log size m, evaluations in bit-reversed
+// order go through the circle layer (layer 0, pairs (2h, 2h+1), twiddle from
+// the y-coordinates) and line layers 1..m-1 (pairs at distance 2^i, twiddle
+// h = pos >> (i+1) of the line layer table), then a scale by 2^-m.  Evaluate
+// runs the layers in reverse with butterfly instead of ibutterfly.
+//
+// Evaluating coefficients of length 2^n on a domain of log m > n: layers
+// i >= n of the zero-padded transform only copy (v1 = 0), so the output is
+// the coefficient vector replicated 2^(m-n) times followed by layers n-1..0 of
+// the size-2^m transform.  This is the SIMD backend's subdomain evaluation
+// (crates/stwo/src/prover/backend/simd/fft/rfft.rs:121-260) reading each
+// coefficient once per subdomain.
+//
+// GPU structure: a transform of log m is split into passes, each covering a
+// contiguous range of layers [lo, hi) held in LDS.  The first pass works on
+// contiguous blocks of up to 2^13 words (layers 0..12); later passes work on
+// groups of 2^(hi-lo) words at stride 2^lo, 32 consecutive groups per
+// workgroup so that every global access is a 128-byte coalesced row.  One
+// launch covers every column of one size class (blockIdx.y = column).
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kBlockLog = 13;  // contiguous pass: up to 2^13 words (32 KiB LDS)
+constexpr uint32_t kStrideLayers = 8;
+constexpr uint32_t kTileLog = 5;  // 32 consecutive groups per strided pass
+
+struct FftPass {
+  uint32_t lo, hi, tl;
+};
+
+// Layers [0, top) split into passes, ascending.
+std::vector<FftPass> plan_passes(uint32_t top) {
+  std::vector<FftPass> v;
+  uint32_t bh = top < kBlockLog ? top : kBlockLog;
+  v.push_back({0, bh, 0});
+  uint32_t l = bh;
+  while (l < top) {
+    uint32_t g = top - l < kStrideLayers ? top - l : kStrideLayers;
+    uint32_t tl = l < kTileLog ? l : kTileLog;
+    v.push_back({l, l + g, tl});
+    l += g;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t h) {
+  // crates/stwo/src/prover/backend/cpu/circle.rs:306-323: [x, y] -> [y, -y, -x, x]
+  uint32_t x = l1[(h >> 2) * 2], y = l1[(h >> 2) * 2 + 1];
+  switch (h & 3) {
+    case 0: return y;
+    case 1: return sh::mneg(y);
+    case 2: return sh::mneg(x);
+    default: return x;
+  }
+}
+
+// One pass of layers [lo, hi) over groups at stride 2^lo, 2^tl consecutive
+// groups per workgroup.  Layer i's table starts at tw + 2^K - 2^(Pl - i)
+// (Pl = log size of the domain whose tree layers are used; the transform of
+// a subdomain uses prefixes of its parent's layers).
+template <bool INVERSE>
+__global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* src_cols,
+                                                       uint32_t* const* dst_cols, uint32_t src_mask,
+                                                       const uint32_t* __restrict__ tw, uint32_t K,
+                                                       uint32_t Pl, uint32_t lo, uint32_t hi,
+                                                       uint32_t tl, uint32_t scale) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t G = hi - lo;
+  const uint32_t col = blockIdx.y;
+  const uint32_t* __restrict__ src = src_cols[col];
+  uint32_t* __restrict__ dst = dst_cols[col];
+  const uint32_t lo_tiles_log = lo - tl;
+  const uint32_t b = blockIdx.x;
+  const uint32_t g_lo0 = (b & ((1u << lo_tiles_log) - 1)) << tl;
+  const uint32_t g_hi = b >> lo_tiles_log;
+  const uint32_t base = (g_hi << hi) | g_lo0;
+  const uint32_t n_words = 1u << (G + tl);
+  const uint32_t tmask = (1u << tl) - 1;
+
+  for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
+    uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+    lds[w] = src[pos & src_mask];
+  }
+  __syncthreads();
+
+  const uint32_t n_bf = n_words >> 1;
+  const uint32_t tw_end = 1u << K;
+  for (uint32_t s = 0; s < G; s++) {
+    const uint32_t li = INVERSE ? s : G - 1 - s;
+    const uint32_t i = lo + li;
+    const uint32_t* tab = tw + tw_end - (1u << (Pl - (i >= 1 ? i : 1)));
+    for (uint32_t q = threadIdx.x; q < n_bf; q += kThreads) {
+      uint32_t t = q & tmask, jb = q >> tl;
+      uint32_t j0 = ((jb >> li) << (li + 1)) | (jb & ((1u << li) - 1));
+      uint32_t j1 = j0 | (1u << li);
+      uint32_t pos0 = base + (j0 << lo) + t;
+      uint32_t h = pos0 >> (i + 1);
+      uint32_t twv = i >= 1 ? tab[h] : circle_twiddle(tab, h);
+      uint32_t a0 = (j0 << tl) | t, a1 = (j1 << tl) | t;
+      uint32_t v0 = lds[a0], v1 = lds[a1];
+      if (INVERSE) {  // crates/stwo/src/core/fft.rs:14-21
+        lds[a0] = sh::madd(v0, v1);
+        lds[a1] = sh::mmul(sh::msub(v0, v1), twv);
+      } else {  // crates/stwo/src/core/fft.rs:5-12
+        uint32_t m = sh::mmul(v1, twv);
+        lds[a0] = sh::madd(v0, m);
+        lds[a1] = sh::msub(v0, m);
+      }
+    }
+    __syncthreads();
+  }
+
+  for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
+    uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+    uint32_t v = lds[w];
+    dst[pos] = scale ? sh::mmul(v, scale) : v;
+  }
+}
+
+// Domains of log size 1 and 2 (crates/stwo/src/prover/backend/cpu/circle.rs:33-58,
+// 203-223); c0..c2 are host-computed constants, one lane per column.
+__global__ void k_fft_tiny(const uint32_t* const* src_cols, uint32_t* const* dst_cols,
+                           uint32_t src_mask, uint32_t n_cols, uint32_t m, int inverse, uint32_t c0,
+                           uint32_t c1, uint32_t c2) {
+  uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n_cols) return;
+  const uint32_t* src = src_cols[col];
+  uint32_t* dst = dst_cols[col];
+  if (m == 1) {
+    uint32_t v0 = src[0 & src_mask], v1 = src[1 & src_mask];
+    if (inverse) {  // c0 = y_inv, c1 = n_inv
+      uint32_t a = sh::madd(v0, v1), b = sh::mmul(sh::msub(v0, v1), c0);
+      dst[0] = sh::mmul(a, c1);
+      dst[1] = sh::mmul(b, c1);
+    } else {  // c0 = y
+      uint32_t t = sh::mmul(v1, c0);
+      dst[0] = sh::madd(v0, t);
+      dst[1] = sh::msub(v0, t);
+    }
+    return;
+  }
+  uint32_t v[4];
+  for (int k = 0; k < 4; k++) v[k] = src[k & src_mask];
+  if (inverse) {  // c0 = x_inv, c1 = y_inv, c2 = n_inv
+    auto ib = [](uint32_t& a, uint32_t& b, uint32_t t) {
+      uint32_t s = sh::madd(a, b);
+      b = sh::mmul(sh::msub(a, b), t);
+      a = s;
+    };
+    ib(v[0], v[1], c1);
+    ib(v[2], v[3], sh::mneg(c1));
+    ib(v[0], v[2], c0);
+    ib(v[1], v[3], c0);
+    for (int k = 0; k < 4; k++) dst[k] = sh::mmul(v[k], c2);
+  } else {  // c0 = x, c1 = y
+    auto bf = [](uint32_t& a, uint32_t& b, uint32_t t) {
+      uint32_t p = sh::mmul(b, t);
+      b = sh::msub(a, p);
+      a = sh::madd(a, p);
+    };
+    bf(v[0], v[2], c0);
+    bf(v[1], v[3], c0);
+    bf(v[0], v[1], c1);
+    bf(v[2], v[3], sh::mneg(c1));
+    for (int k = 0; k < 4; k++) dst[k] = v[k];
+  }
+}
+
+// Replicates a constant polynomial (coeff_log = 0) over the domain.
+__global__ void k_replicate(const uint32_t* const* src_cols, uint32_t* const* dst_cols, uint32_t m) {
+  uint32_t col = blockIdx.y;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (1u << m)) dst_cols[col][i] = src_cols[col][0];
+}
+
+int check_domain(sh_ctx* ctx, const sh_twiddles* tw, uint32_t m) {
+  SH_REQUIRE(ctx, m >= 1 && m <= 30, "circle domain log size %u out of range [1, 30]", m);
+  // CanonicCoset(m).circle_domain().half_coset = half_odds(m - 1)
+  SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(m + 1), m - 1, tw->root_initial_index, tw->root_log_size),
+             "domain of log size %u is not a doubling of the twiddle root coset (log %u)", m,
+             tw->root_log_size);
+  return SH_OK;
+}
+
+}  // namespace
+
+// Interpolation of n_cols columns of log size m, all passes; src/dst are
+// device tables of column pointers (src may equal dst: in place).
+// Pl and half_init describe the domain: canonic domains pass Pl = m.
+int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_src,
+                             uint32_t* const* d_dst, uint32_t n_cols, uint32_t m, uint32_t Pl,
+                             uint32_t half_init) {
+  if (n_cols == 0) return SH_OK;
+  if (m <= 2) {
+    SpanGuard span(ctx, "fft_tiny");
+    sh::mpt p = sh::index_to_point_host(half_init);
+    uint32_t c0, c1, c2 = 0;
+    if (m == 1) {
+      uint32_t yn_inv = sh::minv(sh::mmul(p.y, 2));
+      c0 = sh::mmul(yn_inv, 2);
+      c1 = sh::mmul(yn_inv, p.y);
+    } else {
+      uint32_t xyn_inv = sh::minv(sh::mmul(sh::mmul(p.x, p.y), 4));
+      c0 = sh::mmul(sh::mmul(xyn_inv, p.y), 4);
+      c1 = sh::mmul(sh::mmul(xyn_inv, p.x), 4);
+      c2 = sh::mmul(sh::mmul(xyn_inv, p.x), p.y);
+    }
+    k_fft_tiny<<<(n_cols + 63) / 64, 64, 0, ctx->stream>>>(d_src, d_dst, 0xffffffffu, n_cols, m, 1,
+                                                           c0, c1, c2);
+    return sh_launch_check(ctx, "fft_tiny");
+  }
+  uint32_t scale = sh::minv(1u << m);
+  auto passes = plan_passes(m);
+  for (size_t k = 0; k < passes.size(); k++) {
+    const FftPass& p = passes[k];
+    uint32_t G = p.hi - p.lo;
+    dim3 grid(1u << (m - G - p.tl), n_cols);
+    size_t lds = 4u << (G + p.tl);
+    const uint32_t* const* s = k == 0 ? d_src : (const uint32_t* const*)d_dst;
+    SpanGuard span(ctx, "fft_ifft_pass");
+    k_fft_pass<true><<<grid, kThreads, lds, ctx->stream>>>(
+        s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi, p.tl,
+        k + 1 == passes.size() ? scale : 0u);
+    SH_TRY(sh_launch_check(ctx, "ifft_pass"));
+  }
+  return SH_OK;
+}
+
+// Evaluation of n_cols polynomials with 2^n coefficients on domains of log m.
+int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_src,
+                          uint32_t* const* d_dst, uint32_t n_cols, uint32_t n, uint32_t m,
+                          uint32_t Pl, uint32_t half_init) {
+  if (n_cols == 0) return SH_OK;
+  uint32_t src_mask = (1u << n) - 1;
+  if (m <= 2) {
+    SpanGuard span(ctx, "fft_tiny");
+    sh::mpt p = sh::index_to_point_host(half_init);
+    uint32_t c0 = m == 1 ? p.y : p.x, c1 = p.y;
+    k_fft_tiny<<<(n_cols + 63) / 64, 64, 0, ctx->stream>>>(d_src, d_dst, src_mask, n_cols, m, 0, c0,
+                                                           c1, 0);
+    return sh_launch_check(ctx, "fft_tiny");
+  }
+  if (n == 0) {
+    dim3 grid(((1u << m) + 255) / 256, n_cols);
+    SpanGuard span(ctx, "fft_replicate");
+    k_replicate<<<grid, 256, 0, ctx->stream>>>(d_src, d_dst, m);
+    return sh_launch_check(ctx, "replicate");
+  }
+  auto passes = plan_passes(n);
+  for (int k = (int)passes.size() - 1; k >= 0; k--) {
+    const FftPass& p = passes[k];
+    uint32_t G = p.hi - p.lo;
+    dim3 grid(1u << (m - G - p.tl), n_cols);
+    size_t lds = 4u << (G + p.tl);
+    bool first = k == (int)passes.size() - 1;
+    const uint32_t* const* s = first ? d_src : (const uint32_t* const*)d_dst;
+    SpanGuard span(ctx, "fft_rfft_pass");
+    k_fft_pass<false><<<grid, kThreads, lds, ctx->stream>>>(
+        s, d_dst, first ? src_mask : 0xffffffffu, tw->tw, tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u);
+    SH_TRY(sh_launch_check(ctx, "rfft_pass"));
+  }
+  return SH_OK;
+}
+
+// Groups columns by size class and runs one batched launch chain per class.
+static int interpolate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
+                               uint32_t* const* h_dst, const uint32_t* log_sizes, uint64_t n) {
+  std::vector<uint32_t> classes;
+  for (uint64_t i = 0; i < n; i++) {
+    SH_TRY(check_domain(ctx, tw, log_sizes[i]));
+    bool seen = false;
+    for (auto c : classes) seen |= c == log_sizes[i];
+    if (!seen) classes.push_back(log_sizes[i]);
+  }
+  for (uint32_t m : classes) {
+    std::vector<const void*> ptrs;
+    for (uint64_t i = 0; i < n; i++)
+      if (log_sizes[i] == m) ptrs.push_back(h_src[i]);
+    for (uint64_t i = 0; i < n; i++)
+      if (log_sizes[i] == m) ptrs.push_back(h_dst[i]);
+    void* d_tab;
+    SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_tab));
+    uint32_t cnt = (uint32_t)(ptrs.size() / 2);
+    const uint32_t* const* d_src = (const uint32_t* const*)d_tab;
+    uint32_t* const* d_dst = (uint32_t* const*)((const void**)d_tab + cnt);
+    SH_TRY(sh_fft_interpolate_batch(ctx, tw, d_src, d_dst, cnt, m, m, sh::subgroup_gen(m + 1)));
+  }
+  return SH_OK;
+}
+
+int sh_interpolate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
+                           uint32_t* const* h_dst, const uint32_t* log_sizes, uint64_t n) {
+  return interpolate_grouped(ctx, tw, h_src, h_dst, log_sizes, n);
+}
+
+int sh_evaluate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
+                        const uint32_t* coeff_logs, const uint32_t* domain_logs,
+                        uint32_t* const* h_dst, uint64_t n) {
+  std::vector<std::pair<uint32_t, uint32_t>> classes;
+  for (uint64_t i = 0; i < n; i++) {
+    SH_TRY(check_domain(ctx, tw, domain_logs[i]));
+    SH_REQUIRE(ctx, coeff_logs[i] <= domain_logs[i], "Can only evaluate on larger domains (%u > %u)",
+               coeff_logs[i], domain_logs[i]);
+    auto key = std::make_pair(coeff_logs[i], domain_logs[i]);
+    bool seen = false;
+    for (auto& c : classes) seen |= c == key;
+    if (!seen) classes.push_back(key);
+  }
+  for (auto& c : classes) {
+    std::vector<const void*> ptrs;
+    for (uint64_t i = 0; i < n; i++)
+      if (coeff_logs[i] == c.first && domain_logs[i] == c.second) ptrs.push_back(h_src[i]);
+    for (uint64_t i = 0; i < n; i++)
+      if (coeff_logs[i] == c.first && domain_logs[i] == c.second) ptrs.push_back(h_dst[i]);
+    void* d_tab;
+    SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_tab));
+    uint32_t cnt = (uint32_t)(ptrs.size() / 2);
+    const uint32_t* const* d_src = (const uint32_t* const*)d_tab;
+    uint32_t* const* d_dst = (uint32_t* const*)((const void**)d_tab + cnt);
+    SH_TRY(sh_fft_evaluate_batch(ctx, tw, d_src, d_dst, cnt, c.first, c.second, c.second,
+                                 sh::subgroup_gen(c.second + 1)));
+  }
+  return SH_OK;
+}
+
+extern "C" int sh_interpolate(sh_ctx* ctx, const sh_twiddles* tw, uint32_t* d_values, uint32_t log_size) {
+  if (!ctx || !tw || !d_values) return SH_ERR_INVALID;
+  const uint32_t* s = d_values;
+  return interpolate_grouped(ctx, tw, &s, &d_values, &log_size, 1);
+}
+
+extern "C" int sh_interpolate_columns(sh_ctx* ctx, const sh_twiddles* tw, uint32_t* const* d_cols,
+                                      const uint32_t* log_sizes, uint64_t n) {
+  if (!ctx || !tw || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
+  return interpolate_grouped(ctx, tw, (const uint32_t* const*)d_cols, d_cols, log_sizes, n);
+}
+
+extern "C" int sh_evaluate(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* d_coeffs, uint32_t coeff_log,
+                           uint32_t* d_out, uint32_t domain_log) {
+  if (!ctx || !tw || !d_coeffs || !d_out) return SH_ERR_INVALID;
+  return sh_evaluate_grouped(ctx, tw, &d_coeffs, &coeff_log, &domain_log, &d_out, 1);
+}
+
+extern "C" int sh_evaluate_polynomials(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_coeffs,
+                                       const uint32_t* coeff_logs, uint64_t n, uint32_t log_blowup,
+                                       uint32_t* const* d_outs) {
+  if (!ctx || !tw || (n && (!d_coeffs || !coeff_logs || !d_outs))) return SH_ERR_INVALID;
+  std::vector<uint32_t> dl(n);
+  for (uint64_t i = 0; i < n; i++) dl[i] = coeff_logs[i] + log_blowup;
+  return sh_evaluate_grouped(ctx, tw, d_coeffs, coeff_logs, dl.data(), d_outs, n);
+}

@@ -1,0 +1,420 @@
+// Blake2s Merkle trees: lifted leaves (MerkleOpsLifted::build_leaves), inner
+// layers (build_next_layer) and the mixed-degree tree (MerkleOps::commit_on_layer).
+//
+// Lifted leaves (crates/stwo/src/prover/backend/cpu/merkle_lifted.rs:46-90):
+// with columns sorted ascending by size, leaf i of a tree of height L is
+// Blake2s over LE32(col_c[((i >> (L - n_c + 1)) << 1) + (i & 1)]) for every
+// column c in order.  Rows of a smaller column group share their hash state
+// with every row they lift to, so each size group is absorbed once at its own
+// size (the SIMD backend does the same, crates/stwo/src/prover/backend/simd/
+// blake2s_lifted.rs:95-145): one launch per size group, one lane per row, the
+// partial state (chaining value + pending words of the unfinished block) kept
+// in HBM between groups, structure-of-arrays for coalescing.
+//
+// Streaming rule (the `blake2` crate): a full block is compressed only when
+// more data follows; the last block, full or not, is compressed with the final
+// flag.  Pending words therefore range over 1..16 between groups.
+#include "../../include/stwo_hip.h"
+#include "blake2s.h"
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+constexpr uint32_t kMsgStride = 17;  // odd stride: conflict-free per-lane message staging in LDS
+
+struct LeafGroupArgs {
+  const uint32_t* const* cols;  // device table of this group's columns
+  uint32_t cnt;                 // columns in the group
+  uint32_t log_n;               // rows = 2^log_n
+  const uint32_t* state_in;     // SoA state of the previous group (nullptr for the first)
+  uint32_t in_log;              // previous group's row log
+  uint32_t pend_in;             // pending words carried in (uniform)
+  uint32_t t_in;                // bytes compressed before this group (uniform)
+  uint32_t* state_out;          // SoA state for the next group (nullptr when final)
+  uint32_t* hash_out;           // final hashes (when state_out == nullptr), 8 words per row
+};
+
+__device__ __forceinline__ void load_msg(const uint32_t* lm, uint32_t m[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; j++) m[j] = lm[j];
+}
+
+__global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
+  __shared__ uint32_t msg_lds[kThreads * kMsgStride];
+  const uint32_t rows = 1u << a.log_n;
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  const bool active = r < rows;
+  uint32_t* lm = msg_lds + threadIdx.x * kMsgStride;
+  uint32_t h[8];
+  uint32_t pend = a.pend_in, t = a.t_in;
+  if (a.state_in) {
+    const uint32_t in_rows = 1u << a.in_log;
+    const uint32_t ratio = a.log_n - a.in_log;
+    const uint32_t parent = ((r >> (ratio + 1)) << 1) + (r & 1);
+    if (active) {
+#pragma unroll
+      for (int w = 0; w < 8; w++) h[w] = a.state_in[w * in_rows + parent];
+      for (uint32_t j = 0; j < pend; j++) lm[j] = a.state_in[(8 + j) * in_rows + parent];
+    }
+  } else {
+    sh::b2_init(h);
+  }
+  for (uint32_t k = 0; k < a.cnt; k++) {
+    if (pend == 16) {
+      t += 64;
+      if (active) {
+        uint32_t m[16];
+        load_msg(lm, m);
+        sh::b2_compress(h, m, t, false);
+      }
+      pend = 0;
+    }
+    const uint32_t* col = a.cols[k];
+    if (active) lm[pend] = col[r];
+    pend++;
+  }
+  if (a.state_out && pend == 16) {
+    // More data follows this group, so a full pending block can be compressed
+    // here, at the smaller size, instead of once per lifted row later.
+    t += 64;
+    if (active) {
+      uint32_t m[16];
+      load_msg(lm, m);
+      sh::b2_compress(h, m, t, false);
+    }
+    pend = 0;
+  }
+  if (!active) return;
+  if (a.state_out) {
+#pragma unroll
+    for (int w = 0; w < 8; w++) a.state_out[w * rows + r] = h[w];
+    for (uint32_t j = 0; j < pend; j++) a.state_out[(8 + j) * rows + r] = lm[j];
+  } else {
+    t += 4 * pend;
+    for (uint32_t j = pend; j < 16; j++) lm[j] = 0;
+    uint32_t m[16];
+    load_msg(lm, m);
+    sh::b2_compress(h, m, t, true);
+    uint4* o = (uint4*)(a.hash_out + 8 * r);
+    o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+  }
+}
+
+// Leaves of a tree taller than the largest column: leaf i copies the hash of
+// its lifted row ((i >> (ratio + 1)) << 1) + (i & 1).
+__global__ void k_lift_hashes(const uint32_t* src, uint32_t ratio, uint32_t* dst, uint32_t log_out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1u << log_out)) return;
+  uint32_t p = ((i >> (ratio + 1)) << 1) + (i & 1);
+  const uint4* s = (const uint4*)(src + 8 * p);
+  uint4* d = (uint4*)(dst + 8 * i);
+  d[0] = s[0];
+  d[1] = s[1];
+}
+
+// Hash of the empty stream (the single leaf of an empty tree).
+__global__ void k_empty_hash(uint32_t* out) {
+  uint32_t h[8], m[16];
+  sh::b2_init(h);
+#pragma unroll
+  for (int j = 0; j < 16; j++) m[j] = 0;
+  sh::b2_compress(h, m, 0, true);
+#pragma unroll
+  for (int w = 0; w < 8; w++) out[w] = h[w];
+}
+
+// Up to 8 inner layers per launch: level 0 hashes pairs of the input layer
+// from HBM, each further level hashes pairs held in LDS.  outs[l] receives
+// level l (2^(in_log - 1 - l) nodes).
+struct LayerOuts {
+  uint32_t* p[8];
+};
+
+__global__ __launch_bounds__(kThreads) void k_next_layers(const uint32_t* prev, uint32_t in_log,
+                                                          uint32_t levels, LayerOuts outs) {
+  __shared__ uint32_t node[kThreads * 8];
+  const uint32_t n0 = 1u << (in_log - 1);
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t h[8];
+  if (i < n0) {
+    const uint4* s = (const uint4*)(prev + 16 * i);
+    uint4 q0 = s[0], q1 = s[1], q2 = s[2], q3 = s[3];
+    uint32_t m[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                      q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+    sh::b2_init(h);
+    sh::b2_compress(h, m, 64, true);
+    uint4* o = (uint4*)(outs.p[0] + 8 * i);
+    o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+#pragma unroll
+    for (int w = 0; w < 8; w++) node[threadIdx.x * 8 + w] = h[w];
+  }
+  uint32_t width = kThreads;  // nodes of the current level held by this workgroup
+  for (uint32_t l = 1; l < levels; l++) {
+    __syncthreads();
+    width >>= 1;
+    uint32_t mine = 0;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gi = blockIdx.x * width + tid;  // node index in level l
+    const uint32_t nl = n0 >> l;
+    if (tid < width && gi < nl) {
+      uint32_t m[16];
+#pragma unroll
+      for (int w = 0; w < 16; w++) m[w] = node[tid * 16 + w];
+      sh::b2_init(h);
+      sh::b2_compress(h, m, 64, true);
+      mine = 1;
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+      for (int w = 0; w < 8; w++) node[tid * 8 + w] = h[w];
+      uint4* o = (uint4*)(outs.p[l] + 8 * gi);
+      o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+      o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+  }
+}
+
+// Mixed-degree node (crates/stwo/src/core/vcs/blake2_merkle.rs:13-29):
+// Blake2s(prev[2i] || prev[2i+1] || LE32(col_0[i]) || ... ).
+__global__ __launch_bounds__(kThreads) void k_commit_on_layer(uint32_t log_size, const uint32_t* prev,
+                                                              const uint32_t* const* cols, uint32_t cnt,
+                                                              uint32_t* out) {
+  __shared__ uint32_t msg_lds[kThreads * kMsgStride];
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  const bool active = r < (1u << log_size);
+  uint32_t* lm = msg_lds + threadIdx.x * kMsgStride;
+  uint32_t h[8];
+  sh::b2_init(h);
+  uint32_t pend = 0, t = 0;
+  if (prev) {
+    if (active)
+      for (int j = 0; j < 16; j++) lm[j] = prev[16 * r + j];
+    pend = 16;
+  }
+  for (uint32_t k = 0; k < cnt; k++) {
+    if (pend == 16) {
+      t += 64;
+      if (active) {
+        uint32_t m[16];
+        load_msg(lm, m);
+        sh::b2_compress(h, m, t, false);
+      }
+      pend = 0;
+    }
+    const uint32_t* col = cols[k];
+    if (active) lm[pend] = col[r];
+    pend++;
+  }
+  if (!active) return;
+  t += 4 * pend;
+  for (uint32_t j = pend; j < 16; j++) lm[j] = 0;
+  uint32_t m[16];
+  load_msg(lm, m);
+  sh::b2_compress(h, m, t, true);
+  uint4* o = (uint4*)(out + 8 * r);
+  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+}  // namespace
+
+// Leaves for columns already sorted ascending (host table h_cols).
+int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
+                         uint32_t L, uint32_t* d_out) {
+  if (n == 0) {
+    SpanGuard span(ctx, "merkle_empty");
+    k_empty_hash<<<1, 1, 0, ctx->stream>>>(d_out);
+    return sh_launch_check(ctx, "empty_hash");
+  }
+  for (uint64_t i = 0; i < n; i++) {
+    SH_REQUIRE(ctx, log_sizes[i] >= 1, "A column must be of length >= 2.");
+    SH_REQUIRE(ctx, log_sizes[i] <= 30, "column log size %u too large", log_sizes[i]);
+    if (i) SH_REQUIRE(ctx, log_sizes[i] >= log_sizes[i - 1], "columns must be sorted ascending by size");
+  }
+  SH_REQUIRE(ctx, L >= log_sizes[n - 1], "lifting_log_size %u < %u", L, log_sizes[n - 1]);
+  void* d_tab;
+  SH_TRY(sh_stage(ctx, h_cols, n * sizeof(void*), &d_tab));
+  const uint32_t* const* d_cols = (const uint32_t* const*)d_tab;
+  // group boundaries
+  std::vector<std::pair<uint64_t, uint64_t>> groups;
+  for (uint64_t s = 0; s < n;) {
+    uint64_t e = s;
+    while (e < n && log_sizes[e] == log_sizes[s]) e++;
+    groups.push_back({s, e});
+    s = e;
+  }
+  const uint32_t* state_in = nullptr;
+  uint32_t in_log = 0;
+  uint32_t pend = 0, t = 0;  // uniform stream position: pending words, bytes compressed
+  int slot = SCR_MERKLE_STATE_A;
+  uint32_t n_last = log_sizes[n - 1];
+  for (size_t g = 0; g < groups.size(); g++) {
+    LeafGroupArgs a;
+    a.cols = d_cols + groups[g].first;
+    a.cnt = (uint32_t)(groups[g].second - groups[g].first);
+    a.log_n = log_sizes[groups[g].first];
+    a.state_in = state_in;
+    a.in_log = in_log;
+    a.pend_in = pend;
+    a.t_in = t;
+    bool last = g + 1 == groups.size();
+    // replay the kernel's uniform stream bookkeeping
+    for (uint32_t k = 0; k < a.cnt; k++) {
+      if (pend == 16) {
+        t += 64;
+        pend = 0;
+      }
+      pend++;
+    }
+    if (!last && pend == 16) {
+      t += 64;
+      pend = 0;
+    }
+    uint64_t rows = 1ull << a.log_n;
+    if (!last) {
+      uint32_t pend_after = pend;
+      void* st;
+      SH_TRY(sh_scratch_get(ctx, slot, 4ull * (8 + pend_after) * rows, &st));
+      a.state_out = (uint32_t*)st;
+      a.hash_out = nullptr;
+    } else {
+      a.state_out = nullptr;
+      if (L == n_last) {
+        a.hash_out = d_out;
+      } else {
+        void* tmp;
+        SH_TRY(sh_scratch_get(ctx, SCR_MERKLE_TMP, 32ull * rows, &tmp));
+        a.hash_out = (uint32_t*)tmp;
+      }
+    }
+    SpanGuard span(ctx, "merkle_leaves_group");
+    k_leaves_group<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(a);
+    SH_TRY(sh_launch_check(ctx, "leaves_group"));
+    state_in = a.state_out;
+    in_log = a.log_n;
+    slot = slot == SCR_MERKLE_STATE_A ? SCR_MERKLE_STATE_B : SCR_MERKLE_STATE_A;
+  }
+  if (L > n_last) {
+    void* tmp;
+    SH_TRY(sh_scratch_get(ctx, SCR_MERKLE_TMP, 32ull << n_last, &tmp));
+    uint64_t out_n = 1ull << L;
+    SpanGuard span(ctx, "merkle_lift_hashes");
+    k_lift_hashes<<<(unsigned)((out_n + 255) / 256), 256, 0, ctx->stream>>>((const uint32_t*)tmp, L - n_last,
+                                                                            d_out, L);
+    SH_TRY(sh_launch_check(ctx, "lift_hashes"));
+  }
+  return SH_OK;
+}
+
+// Layers below the leaves: layer k-1 from layer k for k = L..1; layers[k] has 2^k hashes.
+int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers) {
+  uint32_t k = L;
+  while (k > 0) {
+    uint32_t levels = k < 8 ? k : 8;
+    LayerOuts outs;
+    for (uint32_t l = 0; l < levels; l++) outs.p[l] = layers[k - 1 - l];
+    uint64_t n0 = 1ull << (k - 1);
+    SpanGuard span(ctx, "merkle_next_layers");
+    k_next_layers<<<(unsigned)((n0 + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(layers[k], k,
+                                                                                            levels, outs);
+    SH_TRY(sh_launch_check(ctx, "next_layers"));
+    k -= levels;
+  }
+  return SH_OK;
+}
+
+extern "C" int sh_merkle_build_leaves(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
+                                      uint64_t n, uint32_t lifting_log_size, uint8_t* d_out) {
+  if (!ctx || !d_out || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
+  return sh_build_leaves_impl(ctx, d_cols, log_sizes, n, lifting_log_size, (uint32_t*)d_out);
+}
+
+extern "C" int sh_merkle_build_next_layer(sh_ctx* ctx, const uint8_t* d_prev, uint32_t prev_log_size,
+                                          uint8_t* d_out) {
+  if (!ctx || !d_prev || !d_out) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, prev_log_size >= 1 && prev_log_size <= 31, "bad layer log size %u", prev_log_size);
+  SpanGuard span(ctx, "merkle_next_layers");
+  LayerOuts outs;
+  outs.p[0] = (uint32_t*)d_out;
+  uint64_t n0 = 1ull << (prev_log_size - 1);
+  k_next_layers<<<(unsigned)((n0 + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(
+      (const uint32_t*)d_prev, prev_log_size, 1, outs);
+  return sh_launch_check(ctx, "next_layer");
+}
+
+extern "C" int sh_merkle_commit_lifted(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
+                                       uint64_t n, uint32_t lifting_log_size, uint8_t* const* d_layers,
+                                       uint8_t root[32]) {
+  if (!ctx || !d_layers || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
+  if (n == 0) SH_REQUIRE(ctx, lifting_log_size == 0, "lifting_log_size must be 0 when no columns are committed");
+  // stable sort ascending by size (crates/stwo/src/prover/vcs_lifted/prover.rs:68)
+  std::vector<uint64_t> order(n);
+  for (uint64_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t x, uint64_t y) { return log_sizes[x] < log_sizes[y]; });
+  std::vector<const uint32_t*> cols(n);
+  std::vector<uint32_t> logs(n);
+  for (uint64_t i = 0; i < n; i++) {
+    cols[i] = d_cols[order[i]];
+    logs[i] = log_sizes[order[i]];
+  }
+  SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), n, lifting_log_size,
+                              (uint32_t*)d_layers[lifting_log_size]));
+  SH_TRY(sh_build_layers_impl(ctx, lifting_log_size, (uint32_t* const*)d_layers));
+  if (root) {
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return SH_OK;
+}
+
+int sh_commit_on_layer_impl(sh_ctx* ctx, uint32_t log_size, const uint32_t* d_prev,
+                            const uint32_t* const* h_cols, uint64_t n, uint32_t* d_out) {
+  SH_REQUIRE(ctx, log_size <= 30, "layer log size %u too large", log_size);
+  const uint32_t* const* d_cols = nullptr;
+  if (n) {
+    void* d_tab;
+    SH_TRY(sh_stage(ctx, h_cols, n * sizeof(void*), &d_tab));
+    d_cols = (const uint32_t* const*)d_tab;
+  }
+  uint64_t rows = 1ull << log_size;
+  SpanGuard span(ctx, "merkle_commit_on_layer");
+  k_commit_on_layer<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(
+      log_size, d_prev, d_cols, (uint32_t)n, d_out);
+  return sh_launch_check(ctx, "commit_on_layer");
+}
+
+extern "C" int sh_merkle_commit_on_layer(sh_ctx* ctx, uint32_t log_size, const uint8_t* d_prev,
+                                         const uint32_t* const* d_cols, uint64_t n, uint8_t* d_out) {
+  if (!ctx || !d_out || (n && !d_cols)) return SH_ERR_INVALID;
+  return sh_commit_on_layer_impl(ctx, log_size, (const uint32_t*)d_prev, d_cols, n, (uint32_t*)d_out);
+}
+
+extern "C" int sh_merkle_commit_mixed(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
+                                      uint64_t n, uint8_t* const* d_layers, uint8_t root[32]) {
+  if (!ctx || !d_layers || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
+  // stable sort descending by size (crates/stwo/src/prover/vcs/prover.rs:45)
+  std::vector<uint64_t> order(n);
+  for (uint64_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t x, uint64_t y) { return log_sizes[x] > log_sizes[y]; });
+  uint32_t max_log = n ? log_sizes[order[0]] : 0;
+  uint64_t pos = 0;
+  const uint32_t* prev = nullptr;
+  for (int ls = (int)max_log; ls >= 0; ls--) {
+    std::vector<const uint32_t*> layer_cols;
+    while (pos < n && log_sizes[order[pos]] == (uint32_t)ls) layer_cols.push_back(d_cols[order[pos++]]);
+    SH_TRY(sh_commit_on_layer_impl(ctx, (uint32_t)ls, prev, layer_cols.data(), layer_cols.size(),
+                                   (uint32_t*)d_layers[ls]));
+    prev = (const uint32_t*)d_layers[ls];
+  }
+  if (root) {
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return SH_OK;
+}

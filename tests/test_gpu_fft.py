@@ -1,0 +1,113 @@
+"""GPU parity: twiddle tree, interpolate, evaluate (LDE) against the CPU oracle, bit-exact.
+
+Mirrors the reference's differential tests (crates/stwo/src/prover/backend/simd/fft/
+rfft.rs:911-1011, ifft.rs:571-725, simd/circle.rs:606-736) with the oracle standing in for
+CpuBackend.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from proving_amd.backend import BaseColumn, canonic_half_initial
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+def rand_col(rng, log):
+    return rng.integers(0, P, 1 << log, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("root_log", [0, 1, 2, 3, 5, 10, 14, 20])
+def test_twiddles_match_cpu(backend, root_log):
+    init = canonic_half_initial(root_log + 1)
+    t = backend.precompute_twiddles(init, root_log)
+    tw, itw = t.to_cpu()
+    etw, eitw = O.precompute_twiddles(init, root_log)
+    np.testing.assert_array_equal(tw, etw)
+    np.testing.assert_array_equal(itw, eitw)
+    t.free()
+
+
+def test_twiddles_non_canonic_root(backend):
+    t = backend.precompute_twiddles(12345, 9)
+    tw, itw = t.to_cpu()
+    etw, eitw = O.precompute_twiddles(12345, 9)
+    np.testing.assert_array_equal(tw, etw)
+    np.testing.assert_array_equal(itw, eitw)
+    t.free()
+
+
+@pytest.mark.parametrize("log", [1, 2, 3, 4, 5, 8, 12, 13, 14, 16, 18])
+def test_interpolate_matches_cpu(ctx, backend, log):
+    rng = np.random.default_rng(log)
+    tree = backend.precompute_canonic_twiddles(log + 1)
+    _, itw = O.canonic_twiddles(log + 1)
+    ev = rand_col(rng, log)
+    col = BaseColumn.from_cpu(ctx, ev)
+    backend.interpolate(col, tree)
+    np.testing.assert_array_equal(col.to_cpu(), O.interpolate(itw, ev, log))
+    col.free()
+    tree.free()
+
+
+@pytest.mark.parametrize("n,m", [(1, 1), (2, 2), (1, 2), (3, 3), (3, 4), (0, 4), (5, 7), (12, 13),
+                                 (13, 14), (14, 15), (16, 17), (17, 18), (10, 14), (18, 19)])
+def test_evaluate_matches_cpu(ctx, backend, n, m):
+    rng = np.random.default_rng(100 * n + m)
+    tree = backend.precompute_canonic_twiddles(max(m, 1))
+    tw, _ = O.canonic_twiddles(max(m, 1))
+    coeffs = rand_col(rng, n)
+    c = BaseColumn.from_cpu(ctx, coeffs)
+    out = backend.evaluate(c, m, tree)
+    np.testing.assert_array_equal(out.to_cpu(), O.evaluate(tw, coeffs, m))
+    for b in (c, out):
+        b.free()
+    tree.free()
+
+
+def test_round_trip_closed_form(ctx, backend):
+    """Interpolating values 0..2^n (natural order) gives a polynomial with f(domain.at(i)) = i:
+    evaluate(interpolate(x)) == x, and the coefficients match the oracle."""
+    log = 15
+    tree = backend.precompute_canonic_twiddles(log + 1)
+    nat = np.arange(1 << log, dtype=np.uint32)
+    br = np.array([int(bin(j)[2:].zfill(log)[::-1], 2) for j in range(1 << log)], np.uint32)
+    col = BaseColumn.from_cpu(ctx, nat[br])
+    backend.interpolate(col, tree)
+    out = backend.evaluate(col, log, tree)
+    np.testing.assert_array_equal(out.to_cpu(), nat[br])
+
+
+def test_batched_uneven_sizes(ctx, backend):
+    rng = np.random.default_rng(7)
+    logs = [3, 12, 5, 12, 16, 3, 9, 14]
+    tree = backend.precompute_canonic_twiddles(max(logs) + 2)
+    tw, itw = O.canonic_twiddles(max(logs) + 2)
+    evs = [rand_col(rng, lg) for lg in logs]
+    cols = [BaseColumn.from_cpu(ctx, e) for e in evs]
+    backend.interpolate_columns(cols, tree)
+    coeffs = [O.interpolate(itw, e, lg) for e, lg in zip(evs, logs)]
+    for c, e in zip(cols, coeffs):
+        np.testing.assert_array_equal(c.to_cpu(), e)
+    for blowup in (1, 2):
+        outs = backend.evaluate_polynomials(cols, blowup, tree)
+        for o, e, lg in zip(outs, coeffs, logs):
+            np.testing.assert_array_equal(o.to_cpu(), O.evaluate(tw, e, lg + blowup))
+
+
+def test_bit_reverse(ctx, backend):
+    rng = np.random.default_rng(3)
+    v = rand_col(rng, 11)
+    col = BaseColumn.from_cpu(ctx, v)
+    col.bit_reverse()
+    br = np.array([int(bin(j)[2:].zfill(11)[::-1], 2) for j in range(1 << 11)])
+    np.testing.assert_array_equal(col.to_cpu(), v[br])
+
+
+def test_domain_not_in_tower_rejected(ctx, backend):
+    from proving_amd import HipError
+    tree = backend.precompute_canonic_twiddles(8)
+    col = BaseColumn(ctx, 10)
+    with pytest.raises(HipError, match="not a doubling"):
+        backend.interpolate(col, tree)
