@@ -139,6 +139,81 @@ int sh_commit_trace(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d
                     uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
                     uint8_t root[32]);
 
+/* ---------------------------------------------------------------- OOD sampling */
+/* PolyOps::eval_at_point (crates/stwo/src/prover/poly/circle/ops.rs:44-48;
+ * CpuBackend crates/stwo/src/prover/backend/cpu/circle.rs:81-98): evaluates the
+ * polynomial with 2^log_size coefficients at a QM31 circle point (8 words). */
+int sh_eval_at_point(sh_ctx *ctx, const uint32_t *d_coeffs, uint32_t log_size, const uint32_t point[8],
+                     uint32_t out[4]);
+/* PolyOps::barycentric_weights_into (crates/stwo/src/prover/poly/circle/ops.rs:50-62;
+ * SIMD crates/stwo/src/prover/backend/simd/circle.rs:165-300): weights for
+ * CanonicCoset(log_size) and the point, written to 4 coordinate arrays. */
+int sh_barycentric_weights(sh_ctx *ctx, uint32_t log_size, const uint32_t point[8], uint32_t *const d_w[4]);
+/* PolyOps::barycentric_eval_at_point (crates/stwo/src/prover/poly/circle/ops.rs:64-70;
+ * SIMD crates/stwo/src/prover/backend/simd/circle.rs:302-353). */
+int sh_barycentric_eval_at_point(sh_ctx *ctx, const uint32_t *d_evals, uint32_t log_size,
+                                 const uint32_t *const d_w[4], uint32_t out[4]);
+
+/* ---------------------------------------------------------------- FriOps */
+/* FriOps::fold_line (crates/stwo/src/prover/fri.rs:41-45; SIMD
+ * crates/stwo/src/prover/backend/simd/fri.rs:32-109): src is a QM31 evaluation on
+ * LineDomain(Coset(line_initial_index, log_size)) (must be in the twiddle tower);
+ * alphas holds fold_step QM31 values (alpha, alpha^2, ...); dst receives
+ * 2^(log_size - fold_step) values.  fold_step in [1, 4]. */
+int sh_fold_line(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const src[4],
+                 uint32_t line_initial_index, uint32_t log_size, const uint32_t *alphas,
+                 uint32_t fold_step, uint32_t *const dst[4]);
+/* FriOps::fold_circle_into_line (crates/stwo/src/prover/fri.rs:62-66; SIMD
+ * crates/stwo/src/prover/backend/simd/fri.rs:111-166): src on CanonicCoset(log_size)
+ * -> dst (2^(log_size-1) values) on LineDomain(half_odds(log_size - 1)). */
+int sh_fold_circle_into_line(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const src[4],
+                             uint32_t log_size, const uint32_t alpha[4], uint32_t *const dst[4]);
+/* FriOps::decompose (crates/stwo/src/prover/fri.rs:68-75; CpuBackend
+ * crates/stwo/src/prover/backend/cpu/fri.rs:38-62, 118-146). */
+int sh_fri_decompose(sh_ctx *ctx, const uint32_t *const src[4], uint32_t log_size, uint32_t *const dst[4],
+                     uint32_t lambda[4]);
+
+/* ---------------------------------------------------------------- QuotientOps */
+/* QuotientOps::accumulate_numerators (crates/stwo/src/prover/pcs/quotient_ops.rs:29-34;
+ * SIMD crates/stwo/src/prover/backend/simd/quotients.rs:35-78): n_cols columns of
+ * log col_log; sample batch b has point batch_points[8b..8b+8] and terms
+ * [batch_term_offsets[b], batch_term_offsets[b+1]) with column index, sampled value
+ * (4 words) and random coefficient (4 words) each, as ColumnSampleBatch holds them
+ * (crates/stwo/src/core/pcs/quotients.rs:40-76).  Batch b's partial numerator goes to
+ * d_outs[4b..4b+4] (2^(col_log - log_blowup) rows) and its summed a-coefficients
+ * to first_linear[4b..4b+4] (host). */
+int sh_accumulate_numerators(sh_ctx *ctx, const uint32_t *const *d_cols, uint64_t n_cols, uint32_t col_log,
+                             uint32_t log_blowup, const uint32_t *batch_points,
+                             const uint32_t *batch_term_offsets, const uint32_t *term_col_idx,
+                             const uint32_t *term_values, const uint32_t *term_random_coeffs,
+                             uint64_t n_batches, uint32_t *const *d_outs, uint32_t *first_linear);
+/* QuotientOps::compute_quotients_and_combine (crates/stwo/src/prover/pcs/quotient_ops.rs:47-53;
+ * SIMD crates/stwo/src/prover/backend/simd/quotients.rs:82-200): accumulation k has
+ * partial numerator d_partials[4k..4k+4] of log partial_logs[k], sample point
+ * points[8k..8k+8] and first linear term first_linear[4k..4k+4]; d_out receives the
+ * combined quotient evaluation on CanonicCoset(lifting_log_size) (4 arrays). */
+int sh_compute_quotients_and_combine(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_partials,
+                                     const uint32_t *partial_logs, const uint32_t *points,
+                                     const uint32_t *first_linear, uint64_t n_acc,
+                                     uint32_t lifting_log_size, uint32_t log_blowup, uint32_t *const *d_out);
+
+/* ---------------------------------------------------------------- AccumulationOps */
+/* AccumulationOps::lift_and_accumulate (crates/stwo/src/prover/air/accumulation.rs:198-200;
+ * SIMD crates/stwo/src/prover/backend/simd/accumulation.rs:12-78): QM31 columns
+ * d_cols[4k..4k+4] of strictly ascending log sizes -> d_out (size of the largest). */
+int sh_lift_and_accumulate(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *log_sizes, uint64_t n,
+                           uint32_t *const *d_out);
+/* AccumulationOps::accumulate (crates/stwo/src/prover/air/accumulation.rs:184-186):
+ * column += other. */
+int sh_accumulate(sh_ctx *ctx, uint32_t *const *d_col, const uint32_t *const *d_other, uint32_t log_size);
+
+/* ---------------------------------------------------------------- GrindOps */
+/* GrindOps::grind (crates/stwo/src/core/proof_of_work.rs:10-14; SIMD
+ * crates/stwo/src/prover/backend/simd/grind.rs:20-122) for the Blake2s channel:
+ * the smallest nonce whose proof-of-work hash over the channel digest has at
+ * least pow_bits trailing zero bits. */
+int sh_grind(sh_ctx *ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t *nonce);
+
 #ifdef __cplusplus
 }
 #endif

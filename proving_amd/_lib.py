@@ -44,6 +44,19 @@ SIGNATURES = {
     "sh_merkle_commit_on_layer": (c_int, [c_vp, c_u32, c_vp, c_vp, c_u64, c_vp]),
     "sh_merkle_commit_mixed": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "sh_commit_trace": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp]),
+    "sh_eval_at_point": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "sh_barycentric_weights": (c_int, [c_vp, c_u32, c_vp, c_vp]),
+    "sh_barycentric_eval_at_point": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "sh_fold_line": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u32, c_vp, c_u32, c_vp]),
+    "sh_fold_circle_into_line": (c_int, [c_vp, c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "sh_fri_decompose": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "sh_accumulate_numerators": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         c_u64, c_vp, c_vp]),
+    "sh_compute_quotients_and_combine": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_u32,
+                                                 c_vp]),
+    "sh_lift_and_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
+    "sh_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u32]),
+    "sh_grind": (c_int, [c_vp, c_vp, c_u32, c_u64p]),
 }
 
 _lib = None

@@ -296,3 +296,124 @@ class HipBackend:
                                            _u32s([e.log_size for e in evals]), len(evals), log_blowup,
                                            _ptrs(ldes), _ptrs(layers), root), "commit_trace")
         return coeffs, ldes, layers, (bytes(root) if want_root else None)
+
+
+def _u32arr(vals):
+    a = np.ascontiguousarray(np.asarray(vals, dtype=np.uint32).reshape(-1))
+    return a
+
+
+class _Ops:
+    """Trait methods beyond PolyOps / Merkle (bound onto HipBackend below)."""
+
+    # ---- PolyOps: OOD sampling (crates/stwo/src/prover/poly/circle/ops.rs:44-80) ---------------
+    def eval_at_point(self, coeffs, point):
+        out = (ctypes.c_uint32 * 4)()
+        pt = _u32s(list(point))
+        self._chk(self.lib.sh_eval_at_point(self.h, coeffs.ptr, coeffs.log_size, pt, out), "eval_at_point")
+        return np.array(out, np.uint32)
+
+    def barycentric_weights(self, log_size, point):
+        w = SecureColumnByCoords(self.ctx, log_size)
+        self._chk(self.lib.sh_barycentric_weights(self.h, log_size, _u32s(list(point)), _ptrs(w.columns)),
+                  "barycentric_weights")
+        return w
+
+    def barycentric_eval_at_point(self, evals, weights):
+        out = (ctypes.c_uint32 * 4)()
+        self._chk(self.lib.sh_barycentric_eval_at_point(self.h, evals.ptr, evals.log_size,
+                                                        _ptrs(weights.columns), out), "barycentric_eval")
+        return np.array(out, np.uint32)
+
+    # ---- FriOps (crates/stwo/src/prover/fri.rs:27-76) -----------------------------------------
+    def fold_line(self, src, line_initial_index, alphas, twiddles, dst=None):
+        f = len(alphas)
+        dst = dst or SecureColumnByCoords(self.ctx, src.log_size - f)
+        a = _u32s([w for al in alphas for w in al])
+        self._chk(self.lib.sh_fold_line(self.h, twiddles.h, _ptrs(src.columns), line_initial_index,
+                                        src.log_size, a, f, _ptrs(dst.columns)), "fold_line")
+        return dst
+
+    def fold_circle_into_line(self, src, alpha, twiddles, dst=None):
+        dst = dst or SecureColumnByCoords(self.ctx, src.log_size - 1)
+        self._chk(self.lib.sh_fold_circle_into_line(self.h, twiddles.h, _ptrs(src.columns), src.log_size,
+                                                    _u32s(list(alpha)), _ptrs(dst.columns)),
+                  "fold_circle_into_line")
+        return dst
+
+    def decompose(self, src):
+        dst = SecureColumnByCoords(self.ctx, src.log_size)
+        lam = (ctypes.c_uint32 * 4)()
+        self._chk(self.lib.sh_fri_decompose(self.h, _ptrs(src.columns), src.log_size, _ptrs(dst.columns), lam),
+                  "decompose")
+        return dst, np.array(lam, np.uint32)
+
+    # ---- QuotientOps (crates/stwo/src/prover/pcs/quotient_ops.rs:20-53) -----------------------
+    def accumulate_numerators(self, cols, sample_batches, log_blowup):
+        """sample_batches: list of (point[8], [(col_idx, value[4], random_coeff[4]), ...]).
+        Returns a list of (point, partial SecureColumnByCoords, first_linear[4])."""
+        col_log = cols[0].log_size
+        pts, offs, idx, vals, rc = [], [0], [], [], []
+        for point, terms in sample_batches:
+            pts.extend(int(v) for v in point)
+            for ci, v, r in terms:
+                idx.append(ci)
+                vals.extend(int(x) for x in v)
+                rc.extend(int(x) for x in r)
+            offs.append(len(idx))
+        outs = [SecureColumnByCoords(self.ctx, col_log - log_blowup) for _ in sample_batches]
+        fl = (ctypes.c_uint32 * (4 * max(1, len(sample_batches))))()
+        self._chk(self.lib.sh_accumulate_numerators(
+            self.h, _ptrs(cols), len(cols), col_log, log_blowup, _u32s(pts), _u32s(offs), _u32s(idx),
+            _u32s(vals), _u32s(rc), len(sample_batches), _ptrs([c for o in outs for c in o.columns]), fl),
+            "accumulate_numerators")
+        fl = np.array(fl, np.uint32)
+        return [(np.asarray(p[0], np.uint32), o, fl[4 * i:4 * i + 4])
+                for i, (p, o) in enumerate(zip(sample_batches, outs))]
+
+    def compute_quotients_and_combine(self, accumulations, lifting_log_size, log_blowup, twiddles, out=None):
+        """accumulations: list of (point[8], partial SecureColumnByCoords, first_linear[4])."""
+        out = out or SecureColumnByCoords(self.ctx, lifting_log_size)
+        parts = [c for _, p, _ in accumulations for c in p.columns]
+        logs = [p.log_size for _, p, _ in accumulations]
+        pts = [int(v) for pt, _, _ in accumulations for v in pt]
+        fls = [int(v) for _, _, f in accumulations for v in f]
+        self._chk(self.lib.sh_compute_quotients_and_combine(
+            self.h, twiddles.h, _ptrs(parts), _u32s(logs), _u32s(pts), _u32s(fls), len(accumulations),
+            lifting_log_size, log_blowup, _ptrs(out.columns)), "compute_quotients_and_combine")
+        return out
+
+    # ---- AccumulationOps (crates/stwo/src/prover/air/accumulation.rs:183-201) -----------------
+    def lift_and_accumulate(self, cols):
+        if not cols:
+            return None
+        out = SecureColumnByCoords(self.ctx, cols[-1].log_size)
+        self._chk(self.lib.sh_lift_and_accumulate(self.h, _ptrs([c for col in cols for c in col.columns]),
+                                                  _u32s([c.log_size for c in cols]), len(cols),
+                                                  _ptrs(out.columns)), "lift_and_accumulate")
+        return out
+
+    def accumulate(self, column, other):
+        self._chk(self.lib.sh_accumulate(self.h, _ptrs(column.columns), _ptrs(other.columns), column.log_size),
+                  "accumulate")
+        return column
+
+    @staticmethod
+    def generate_secure_powers(felt, n_powers, qmul):
+        out, acc = [], [1, 0, 0, 0]
+        for _ in range(n_powers):
+            out.append(acc)
+            acc = qmul(acc, felt)
+        return out
+
+    # ---- GrindOps (crates/stwo/src/core/proof_of_work.rs:10-14) -------------------------------
+    def grind(self, digest: bytes, pow_bits):
+        d = (ctypes.c_uint8 * 32)(*digest)
+        nonce = ctypes.c_uint64()
+        self._chk(self.lib.sh_grind(self.h, d, pow_bits, ctypes.byref(nonce)), "grind")
+        return nonce.value
+
+
+for _name, _fn in list(vars(_Ops).items()):
+    if not _name.startswith("__"):
+        setattr(HipBackend, _name, _fn)

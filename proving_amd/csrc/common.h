@@ -110,3 +110,27 @@ __device__ __forceinline__ sh::mpt sh_index_to_point(uint32_t idx, const sh::mpt
     if ((idx >> b) & 1u) r = sh::padd(r, gen_pow2[b]);
   return r;
 }
+
+// Points of a coset C = init + <step> by table lookup: C.at(j) = init + sum_l T_l[digit_l(j)]
+// with 11-bit digits, T_l[k] = (k << 11l) * step.  Built on the host, staged per call.
+struct sh_point_tab {
+  const sh::mpt* t;  // levels * 2048 points (device)
+  uint32_t levels;
+  sh::mpt init;
+};
+int sh_point_tab_make(sh_ctx* ctx, uint32_t init_index, uint32_t log_size, sh_point_tab* out);
+
+__device__ __forceinline__ sh::mpt sh_tab_point(const sh_point_tab& pt, uint32_t j) {
+  sh::mpt p = pt.init;
+  for (uint32_t l = 0; l < pt.levels; l++) p = sh::padd(p, pt.t[l * 2048 + ((j >> (11 * l)) & 2047)]);
+  return p;
+}
+// CircleDomain(half coset).at(j) (crates/stwo/src/core/poly/circle/domain.rs:50-60): the second
+// half is the conjugate of the first.
+__device__ __forceinline__ sh::mpt sh_circle_domain_point(const sh_point_tab& half, uint32_t half_log,
+                                                          uint32_t j) {
+  uint32_t hs = 1u << half_log;
+  if (j < hs) return sh_tab_point(half, j);
+  sh::mpt p = sh_tab_point(half, j - hs);
+  return {p.x, sh::mneg(p.y)};
+}

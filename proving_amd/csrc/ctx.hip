@@ -265,6 +265,28 @@ extern "C" int sh_bit_reverse_column(sh_ctx* ctx, uint32_t* d_col, uint32_t log_
   return sh_launch_check(ctx, "bit_reverse");
 }
 
+int sh_point_tab_make(sh_ctx* ctx, uint32_t init_index, uint32_t log_size, sh_point_tab* out) {
+  uint32_t levels = log_size == 0 ? 0 : (log_size + 10) / 11;
+  out->levels = levels;
+  out->init = sh::index_to_point_host(init_index);
+  out->t = nullptr;
+  if (levels == 0) return SH_OK;
+  std::vector<sh::mpt> tab(2048ull * levels);
+  uint32_t step = sh::subgroup_gen(log_size);
+  for (uint32_t l = 0; l < levels; l++) {
+    sh::mpt unit = sh::index_to_point_host(sh::idx_reduce((uint64_t)step << (11 * l)));
+    sh::mpt cur = {1, 0};
+    for (uint32_t k = 0; k < 2048; k++) {
+      tab[l * 2048 + k] = cur;
+      cur = sh::padd(cur, unit);
+    }
+  }
+  void* d;
+  SH_TRY(sh_stage(ctx, tab.data(), tab.size() * sizeof(sh::mpt), &d));
+  out->t = (const sh::mpt*)d;
+  return SH_OK;
+}
+
 namespace sh {
 mpt index_to_point_host(uint32_t index) {
   mpt res = {1, 0}, cur = {2, 1268011823u};  // M31_CIRCLE_GEN (crates/stwo/src/core/circle.rs:181-182)

@@ -59,10 +59,63 @@ __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t 
   }
 }
 
+// Butterflies of one register stage: R consecutive local layers [s, s+R) of
+// the group.  A lane owns a subgroup of 2^R words whose j differ only in bits
+// [s, s+R) (the words every butterfly of these layers connects), loads them
+// from LDS once, runs the R layers in registers and writes them back: one
+// LDS access per butterfly for R = 4 instead of four.  At layer i the twiddle
+// index h = pos >> (i+1) is shared by the 2^b pairs below the pair bit b, so a
+// stage loads 2^R - 1 twiddles for R * 2^(R-1) butterflies.
+template <bool INVERSE, int R>
+__device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl, uint32_t s, uint32_t base,
+                                          uint32_t lo, const uint32_t* __restrict__ tw, uint32_t tw_end,
+                                          uint32_t Pl) {
+  const uint32_t n_sub = 1u << (G + tl - R);
+  const uint32_t tmask = (1u << tl) - 1;
+  for (uint32_t q = threadIdx.x; q < n_sub; q += kThreads) {
+    const uint32_t t = q & tmask;
+    const uint32_t jr = q >> tl;
+    const uint32_t jf = ((jr >> s) << (s + R)) | (jr & ((1u << s) - 1));
+    uint32_t v[1 << R];
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) v[e] = lds[((jf | ((uint32_t)e << s)) << tl) | t];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int b = INVERSE ? k : R - 1 - k;
+      const uint32_t i = lo + s + b;
+      const uint32_t* tab = tw + tw_end - (1u << (Pl - (i >= 1 ? i : 1)));
+#pragma unroll
+      for (int eh = 0; eh < (1 << (R - 1 - b)); eh++) {
+        const uint32_t pos0 = base + ((jf | ((uint32_t)(eh << (b + 1)) << s)) << lo) + t;
+        const uint32_t h = pos0 >> (i + 1);
+        const uint32_t twv = i >= 1 ? tab[h] : circle_twiddle(tab, h);
+#pragma unroll
+        for (int el = 0; el < (1 << b); el++) {
+          const int a0 = (eh << (b + 1)) | el, a1 = a0 | (1 << b);
+          if (INVERSE) {  // crates/stwo/src/core/fft.rs:14-21
+            uint32_t x = v[a0], y = v[a1];
+            v[a0] = sh::madd(x, y);
+            v[a1] = sh::mmul(sh::msub(x, y), twv);
+          } else {  // crates/stwo/src/core/fft.rs:5-12
+            uint32_t m = sh::mmul(v[a1], twv);
+            uint32_t x = v[a0];
+            v[a0] = sh::madd(x, m);
+            v[a1] = sh::msub(x, m);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < (1 << R); e++) lds[((jf | ((uint32_t)e << s)) << tl) | t] = v[e];
+  }
+}
+
 // One pass of layers [lo, hi) over groups at stride 2^lo, 2^tl consecutive
 // groups per workgroup.  Layer i's table starts at tw + 2^K - 2^(Pl - i)
 // (Pl = log size of the domain whose tree layers are used; the transform of
-// a subdomain uses prefixes of its parent's layers).
+// a subdomain uses prefixes of its parent's layers).  Global words move as
+// 128-bit rows whenever four consecutive LDS words are four consecutive
+// positions (tile of >= 4 groups, or a contiguous block).
 template <bool INVERSE>
 __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* src_cols,
                                                        uint32_t* const* dst_cols, uint32_t src_mask,
@@ -81,52 +134,66 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   const uint32_t base = (g_hi << hi) | g_lo0;
   const uint32_t n_words = 1u << (G + tl);
   const uint32_t tmask = (1u << tl) - 1;
+  const bool vec = (tl >= 2 || lo == 0) && G + tl >= 2 && src_mask >= 3 &&
+                   ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
 
-  for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
-    uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-    lds[w] = src[pos & src_mask];
+  if (vec) {
+    for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
+      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+      *(uint4*)&lds[w] = *(const uint4*)&src[pos & src_mask];
+    }
+  } else {
+    for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
+      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+      lds[w] = src[pos & src_mask];
+    }
   }
   __syncthreads();
 
-  const uint32_t n_bf = n_words >> 1;
   const uint32_t tw_end = 1u << K;
-  for (uint32_t s = 0; s < G; s++) {
-    const uint32_t li = INVERSE ? s : G - 1 - s;
-    const uint32_t i = lo + li;
-    const uint32_t* tab = tw + tw_end - (1u << (Pl - (i >= 1 ? i : 1)));
-    for (uint32_t q = threadIdx.x; q < n_bf; q += kThreads) {
-      uint32_t t = q & tmask, jb = q >> tl;
-      uint32_t j0 = ((jb >> li) << (li + 1)) | (jb & ((1u << li) - 1));
-      uint32_t j1 = j0 | (1u << li);
-      uint32_t pos0 = base + (j0 << lo) + t;
-      uint32_t h = pos0 >> (i + 1);
-      uint32_t twv = i >= 1 ? tab[h] : circle_twiddle(tab, h);
-      uint32_t a0 = (j0 << tl) | t, a1 = (j1 << tl) | t;
-      uint32_t v0 = lds[a0], v1 = lds[a1];
-      if (INVERSE) {  // crates/stwo/src/core/fft.rs:14-21
-        lds[a0] = sh::madd(v0, v1);
-        lds[a1] = sh::mmul(sh::msub(v0, v1), twv);
-      } else {  // crates/stwo/src/core/fft.rs:5-12
-        uint32_t m = sh::mmul(v1, twv);
-        lds[a0] = sh::madd(v0, m);
-        lds[a1] = sh::msub(v0, m);
-      }
+  const uint32_t n_stages = (G + 3) / 4;
+  for (uint32_t st = 0; st < n_stages; st++) {
+    // stages cover [0,4), [4,8), ... ; inverse runs them upwards, forward downwards
+    const uint32_t si = INVERSE ? st : n_stages - 1 - st;
+    const uint32_t s = 4 * si;
+    const uint32_t r = G - s < 4 ? G - s : 4;
+    switch (r) {
+      case 4: fft_stage<INVERSE, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+      case 3: fft_stage<INVERSE, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+      case 2: fft_stage<INVERSE, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+      default: fft_stage<INVERSE, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
     }
     __syncthreads();
   }
 
-  for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
-    uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-    uint32_t v = lds[w];
-    dst[pos] = scale ? sh::mmul(v, scale) : v;
+  if (vec) {
+    for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
+      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+      uint4 v = *(const uint4*)&lds[w];
+      if (scale) {
+        v.x = sh::mmul(v.x, scale);
+        v.y = sh::mmul(v.y, scale);
+        v.z = sh::mmul(v.z, scale);
+        v.w = sh::mmul(v.w, scale);
+      }
+      *(uint4*)&dst[pos] = v;
+    }
+  } else {
+    for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
+      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
+      uint32_t v = lds[w];
+      dst[pos] = scale ? sh::mmul(v, scale) : v;
+    }
   }
 }
 
 // Domains of log size 1 and 2 (crates/stwo/src/prover/backend/cpu/circle.rs:33-58,
 // 203-223); c0..c2 are host-computed constants, one lane per column.
+// For evaluation, coeff_log = n < m: the layers i >= n only copy, so the
+// replicated input skips their butterflies.
 __global__ void k_fft_tiny(const uint32_t* const* src_cols, uint32_t* const* dst_cols,
                            uint32_t src_mask, uint32_t n_cols, uint32_t m, int inverse, uint32_t c0,
-                           uint32_t c1, uint32_t c2) {
+                           uint32_t c1, uint32_t c2, uint32_t n) {
   uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= n_cols) return;
   const uint32_t* src = src_cols[col];
@@ -138,9 +205,14 @@ __global__ void k_fft_tiny(const uint32_t* const* src_cols, uint32_t* const* dst
       dst[0] = sh::mmul(a, c1);
       dst[1] = sh::mmul(b, c1);
     } else {  // c0 = y
-      uint32_t t = sh::mmul(v1, c0);
-      dst[0] = sh::madd(v0, t);
-      dst[1] = sh::msub(v0, t);
+      if (n >= 1) {
+        uint32_t t = sh::mmul(v1, c0);
+        dst[0] = sh::madd(v0, t);
+        dst[1] = sh::msub(v0, t);
+      } else {
+        dst[0] = v0;
+        dst[1] = v1;
+      }
     }
     return;
   }
@@ -163,10 +235,14 @@ __global__ void k_fft_tiny(const uint32_t* const* src_cols, uint32_t* const* dst
       b = sh::msub(a, p);
       a = sh::madd(a, p);
     };
-    bf(v[0], v[2], c0);
-    bf(v[1], v[3], c0);
-    bf(v[0], v[1], c1);
-    bf(v[2], v[3], sh::mneg(c1));
+    if (n >= 2) {
+      bf(v[0], v[2], c0);
+      bf(v[1], v[3], c0);
+    }
+    if (n >= 1) {
+      bf(v[0], v[1], c1);
+      bf(v[2], v[3], sh::mneg(c1));
+    }
     for (int k = 0; k < 4; k++) dst[k] = v[k];
   }
 }
@@ -211,7 +287,7 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
       c2 = sh::mmul(sh::mmul(xyn_inv, p.x), p.y);
     }
     k_fft_tiny<<<(n_cols + 63) / 64, 64, 0, ctx->stream>>>(d_src, d_dst, 0xffffffffu, n_cols, m, 1,
-                                                           c0, c1, c2);
+                                                           c0, c1, c2, m);
     return sh_launch_check(ctx, "fft_tiny");
   }
   uint32_t scale = sh::minv(1u << m);
@@ -242,7 +318,7 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
     sh::mpt p = sh::index_to_point_host(half_init);
     uint32_t c0 = m == 1 ? p.y : p.x, c1 = p.y;
     k_fft_tiny<<<(n_cols + 63) / 64, 64, 0, ctx->stream>>>(d_src, d_dst, src_mask, n_cols, m, 0, c0,
-                                                           c1, 0);
+                                                           c1, 0, n);
     return sh_launch_check(ctx, "fft_tiny");
   }
   if (n == 0) {

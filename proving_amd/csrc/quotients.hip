@@ -1,0 +1,359 @@
+// DEEP quotients and accumulation:
+//   QuotientOps::accumulate_numerators, QuotientOps::compute_quotients_and_combine,
+//   AccumulationOps::lift_and_accumulate / accumulate.
+//
+// accumulate_numerators (CpuBackend: crates/stwo/src/prover/backend/cpu/quotients.rs:23-49):
+// for each sample batch (one point) and each row of the evaluation subdomain
+// (the first 2^(n - log_blowup) rows in bit-reversed order),
+//     partial[row] = sum_t (c_t * col_{i_t}[row] - b_t),   first_linear = sum_t a_t,
+// with (a, b, c) the complex-conjugate line coefficients times the random power
+// (crates/stwo/src/core/constraints.rs:119-134).  The sum is linear in the
+// columns, so on the device it becomes  sum_col C[col][batch] * col[row] - B[batch]
+// with C the per-(column, batch) sum of c and B the per-batch sum of b.  The SIMD
+// backend re-reads every column once per batch (simd/quotients.rs:68-77); here all
+// batches are accumulated in one pass over the columns, and when C has rank one
+// (C[col][batch] = u[col] * w[batch], the case of every column sampled at the same
+// points, because the random powers advance column by column) a single sum
+// S = sum_col u[col] * col[row] is formed and each batch output is w[batch] * S - B[batch].
+// Products are accumulated lazily in 64 bits (three 62-bit products between folds).
+//
+// compute_quotients_and_combine (cpu/quotients.rs:51-101): on the subdomain of
+// log L - b, q[row] = sum_k (partial_k[lift(row)] - first_linear_k * p.y) * den_k(p)^-1
+// with den (CM31) from crates/stwo/src/core/pcs/quotients.rs:233-256, then
+// interpolate on the subdomain and evaluate on the full domain of log L.
+#include <algorithm>
+
+#include "common.h"
+
+int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_src,
+                             uint32_t* const* d_dst, uint32_t n_cols, uint32_t m, uint32_t Pl, uint32_t half_init);
+int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_src,
+                          uint32_t* const* d_dst, uint32_t n_cols, uint32_t n, uint32_t m, uint32_t Pl,
+                          uint32_t half_init);
+
+namespace {
+
+constexpr int kMaxOut = 8;
+
+struct OutSpec {
+  uint32_t n_out;
+  uint32_t src[kMaxOut];      // which accumulated sum feeds output b
+  sh::qm31 w[kMaxOut];        // output b = w * S_src - B
+  sh::qm31 B[kMaxOut];
+};
+
+__device__ __forceinline__ void lazy_fold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
+__device__ __forceinline__ uint32_t lazy_final(uint64_t v) {
+  v = (v & sh::P) + (v >> 31);
+  v = (v & sh::P) + (v >> 31);
+  return sh::reduce_once((uint32_t)v);
+}
+
+// cols: device table of n_cols column pointers; coef: n_cols x NB x 4 words (canonical);
+// outs: device table of 4 * n_out coordinate pointers.
+template <int NB>
+__global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
+                                                   const uint32_t* __restrict__ coef, uint32_t rows,
+                                                   uint32_t* const* __restrict__ outs, OutSpec spec) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  uint64_t acc[NB][4];
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[j][c] = 0;
+  uint32_t k = 0;
+  for (uint32_t col = 0; col < n_cols; col++) {
+    const uint32_t v = cols[col][r];
+    const uint32_t* cf = coef + (size_t)col * NB * 4;
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[j][c] += (uint64_t)cf[j * 4 + c] * v;
+    if (++k == 3) {
+      k = 0;
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
+    }
+  }
+  sh::qm31 S[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+    S[j] = {{lazy_final(acc[j][0]), lazy_final(acc[j][1])}, {lazy_final(acc[j][2]), lazy_final(acc[j][3])}};
+  for (uint32_t b = 0; b < spec.n_out; b++) {
+    sh::qm31 s = S[0];
+#pragma unroll
+    for (int j = 1; j < NB; j++)
+      if (spec.src[b] == (uint32_t)j) s = S[j];
+    sh::qm31 o = sh::qsub(sh::qmul(spec.w[b], s), spec.B[b]);
+    outs[4 * b + 0][r] = o.a.a;
+    outs[4 * b + 1][r] = o.a.b;
+    outs[4 * b + 2][r] = o.b.a;
+    outs[4 * b + 3][r] = o.b.b;
+  }
+}
+
+// Per-accumulation data for the combine kernel.
+struct AccSpec {
+  uint32_t n_acc;
+  uint32_t sub_log;
+};
+
+__global__ __launch_bounds__(256) void k_combine(const uint32_t* const* __restrict__ parts,
+                                                 const uint32_t* __restrict__ acc_data, AccSpec spec,
+                                                 sh_point_tab half, uint32_t* const* __restrict__ out) {
+  const uint32_t rows = 1u << spec.sub_log;
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  sh::mpt dp = sh_circle_domain_point(half, spec.sub_log - 1, sh::bitrev(r, spec.sub_log));
+  sh::qm31 q = sh::qzero();
+  for (uint32_t k = 0; k < spec.n_acc; k++) {
+    // acc_data per k: point x (4), point y (4), first_linear (4), partial log (1) -> 13 words
+    const uint32_t* d = acc_data + 13 * k;
+    sh::cm31 prx = {d[0], d[1]}, pix = {d[2], d[3]}, pry = {d[4], d[5]}, piy = {d[6], d[7]};
+    sh::qm31 fl = sh::qload(d + 8);
+    uint32_t plog = d[12];
+    sh::cm31 den = sh::csub(sh::cmul(sh::csub(prx, {dp.x, 0}), piy), sh::cmul(sh::csub(pry, {dp.y, 0}), pix));
+    sh::cm31 dinv = sh::cinv(den);
+    uint32_t ratio = spec.sub_log - plog;
+    uint32_t li = ((r >> (ratio + 1)) << 1) + (r & 1);
+    const uint32_t* const* pk = parts + 4 * k;
+    sh::qm31 pv = {{pk[0][li], pk[1][li]}, {pk[2][li], pk[3][li]}};
+    sh::qm31 full = sh::qsub(pv, sh::qmulm(fl, dp.y));
+    q = sh::qadd(q, sh::qmulc(full, dinv));
+  }
+  out[0][r] = q.a.a;
+  out[1][r] = q.a.b;
+  out[2][r] = q.b.a;
+  out[3][r] = q.b.b;
+}
+
+// out[i] = sum_k col_k[lift_k(i)] for every coordinate.
+__global__ __launch_bounds__(256) void k_lift_acc(const uint32_t* const* __restrict__ cols,
+                                                  const uint32_t* __restrict__ logs, uint32_t n, uint32_t L,
+                                                  uint32_t* const* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1u << L)) return;
+  uint32_t s[4] = {0, 0, 0, 0};
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t ratio = L - logs[k];
+    uint32_t li = ((i >> (ratio + 1)) << 1) + (i & 1);
+#pragma unroll
+    for (int c = 0; c < 4; c++) s[c] = sh::madd(s[c], cols[4 * k + c][li]);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) out[c][i] = s[c];
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(uint32_t* const* __restrict__ col,
+                                                    const uint32_t* const* __restrict__ other, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 4; c++) col[c][i] = sh::madd(col[c][i], other[c][i]);
+}
+
+// complex_conjugate_line_coeffs (crates/stwo/src/core/constraints.rs:119-134), host side.
+void line_coeffs(const sh::qpt& p, sh::qm31 value, sh::qm31 alpha, sh::qm31* a, sh::qm31* b, sh::qm31* c) {
+  sh::qm31 av = sh::qsub(sh::qconj(value), value);
+  sh::qm31 cv = sh::qsub(sh::qconj(p.y), p.y);
+  sh::qm31 bv = sh::qsub(sh::qmul(value, cv), sh::qmul(av, p.y));
+  *a = sh::qmul(alpha, av);
+  *b = sh::qmul(alpha, bv);
+  *c = sh::qmul(alpha, cv);
+}
+
+bool qis_zero(sh::qm31 x) { return sh::qeq(x, sh::qzero()); }
+
+}  // namespace
+
+extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
+                                        uint32_t col_log, uint32_t log_blowup, const uint32_t* batch_points,
+                                        const uint32_t* batch_term_offsets, const uint32_t* term_col_idx,
+                                        const uint32_t* term_values, const uint32_t* term_random_coeffs,
+                                        uint64_t n_batches, uint32_t* const* d_outs, uint32_t* first_linear) {
+  if (!ctx || (n_batches && (!batch_points || !batch_term_offsets || !d_outs || !first_linear)))
+    return SH_ERR_INVALID;
+  if (n_batches == 0) return SH_OK;
+  SH_REQUIRE(ctx, n_cols > 0 && d_cols, "accumulate_numerators needs a non-empty set of columns");
+  SH_REQUIRE(ctx, col_log >= log_blowup && col_log <= 30, "bad column log size %u", col_log);
+  const uint64_t NB = n_batches;
+  // dense coefficients C[col][batch], B[batch], first_linear[batch]
+  std::vector<sh::qm31> C(n_cols * NB, sh::qzero()), B(NB, sh::qzero()), FL(NB, sh::qzero());
+  for (uint64_t bi = 0; bi < NB; bi++) {
+    sh::qpt p = {sh::qload(batch_points + 8 * bi), sh::qload(batch_points + 8 * bi + 4)};
+    SH_REQUIRE(ctx, !sh::qeq(p.y, sh::qconj(p.y)), "Cannot evaluate a line with a single point");
+    for (uint32_t t = batch_term_offsets[bi]; t < batch_term_offsets[bi + 1]; t++) {
+      uint32_t ci = term_col_idx[t];
+      SH_REQUIRE(ctx, ci < n_cols, "term column index %u out of range", ci);
+      sh::qm31 a, b, c;
+      line_coeffs(p, sh::qload(term_values + 4 * t), sh::qload(term_random_coeffs + 4 * t), &a, &b, &c);
+      C[ci * NB + bi] = sh::qadd(C[ci * NB + bi], c);
+      B[bi] = sh::qadd(B[bi], b);
+      FL[bi] = sh::qadd(FL[bi], a);
+    }
+  }
+  for (uint64_t bi = 0; bi < NB; bi++) sh::qstore(FL[bi], first_linear + 4 * bi);
+  // rank-one detection: C[col][b] == w[b] * u[col] with u = C[.][pivot batch]
+  uint64_t pivot_col = n_cols;
+  for (uint64_t ci = 0; ci < n_cols && pivot_col == n_cols; ci++)
+    for (uint64_t bi = 0; bi < NB; bi++)
+      if (!qis_zero(C[ci * NB + bi])) {
+        pivot_col = ci;
+        break;
+      }
+  bool rank1 = false;
+  std::vector<sh::qm31> w(NB, sh::qzero());
+  uint64_t pivot_b = 0;
+  if (pivot_col < n_cols) {
+    while (qis_zero(C[pivot_col * NB + pivot_b])) pivot_b++;
+    sh::qm31 inv = sh::qinv(C[pivot_col * NB + pivot_b]);
+    for (uint64_t bi = 0; bi < NB; bi++) w[bi] = sh::qmul(C[pivot_col * NB + bi], inv);
+    rank1 = true;
+    for (uint64_t ci = 0; ci < n_cols && rank1; ci++)
+      for (uint64_t bi = 0; bi < NB; bi++)
+        if (!sh::qeq(C[ci * NB + bi], sh::qmul(w[bi], C[ci * NB + pivot_b]))) {
+          rank1 = false;
+          break;
+        }
+  }
+  const uint32_t rows = 1u << (col_log - log_blowup);
+  void* d_cols_tab;
+  SH_TRY(sh_stage(ctx, d_cols, n_cols * sizeof(void*), &d_cols_tab));
+  auto launch = [&](int nb, const std::vector<uint32_t>& coef_words, OutSpec spec,
+                    const std::vector<const void*>& out_ptrs) -> int {
+    void* d_coef;
+    void* d_outs_tab;
+    SH_TRY(sh_stage(ctx, coef_words.data(), coef_words.size() * 4, &d_coef));
+    SH_TRY(sh_stage(ctx, out_ptrs.data(), out_ptrs.size() * sizeof(void*), &d_outs_tab));
+    unsigned grid = (rows + 255) / 256;
+    const uint32_t* const* ct = (const uint32_t* const*)d_cols_tab;
+    uint32_t* const* ot = (uint32_t* const*)d_outs_tab;
+    SpanGuard span(ctx, "quotients_accumulate_numerators");
+    switch (nb) {
+      case 1: k_accum_num<1><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
+      case 2: k_accum_num<2><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
+      case 3: k_accum_num<3><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
+      default: k_accum_num<4><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
+    }
+    return sh_launch_check(ctx, "accumulate_numerators");
+  };
+  if (rank1 || pivot_col == n_cols) {
+    // one pass: S = sum u[col] * col, outputs in groups of kMaxOut
+    std::vector<uint32_t> coef(n_cols * 4);
+    for (uint64_t ci = 0; ci < n_cols; ci++)
+      sh::qstore(pivot_col < n_cols ? C[ci * NB + pivot_b] : sh::qzero(), &coef[ci * 4]);
+    for (uint64_t b0 = 0; b0 < NB; b0 += kMaxOut) {
+      OutSpec spec = {};
+      std::vector<const void*> outp;
+      spec.n_out = (uint32_t)std::min<uint64_t>(kMaxOut, NB - b0);
+      for (uint32_t k = 0; k < spec.n_out; k++) {
+        spec.src[k] = 0;
+        spec.w[k] = w[b0 + k];
+        spec.B[k] = B[b0 + k];
+        for (int c = 0; c < 4; c++) outp.push_back(d_outs[4 * (b0 + k) + c]);
+      }
+      SH_TRY(launch(1, coef, spec, outp));
+    }
+  } else {
+    for (uint64_t b0 = 0; b0 < NB; b0 += 4) {
+      int nb = (int)std::min<uint64_t>(4, NB - b0);
+      std::vector<uint32_t> coef(n_cols * nb * 4);
+      OutSpec spec = {};
+      std::vector<const void*> outp;
+      spec.n_out = nb;
+      for (int j = 0; j < nb; j++) {
+        for (uint64_t ci = 0; ci < n_cols; ci++) sh::qstore(C[ci * NB + b0 + j], &coef[(ci * nb + j) * 4]);
+        spec.src[j] = j;
+        spec.w[j] = sh::qone();
+        spec.B[j] = B[b0 + j];
+        for (int c = 0; c < 4; c++) outp.push_back(d_outs[4 * (b0 + j) + c]);
+      }
+      SH_TRY(launch(nb, coef, spec, outp));
+    }
+  }
+  return SH_OK;
+}
+
+extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw,
+                                                const uint32_t* const* d_partials, const uint32_t* partial_logs,
+                                                const uint32_t* points, const uint32_t* first_linear,
+                                                uint64_t n_acc, uint32_t lifting_log_size, uint32_t log_blowup,
+                                                uint32_t* const* d_out) {
+  if (!ctx || !tw || !d_out || (n_acc && (!d_partials || !partial_logs || !points || !first_linear)))
+    return SH_ERR_INVALID;
+  const uint32_t L = lifting_log_size, b = log_blowup;
+  SH_REQUIRE(ctx, L >= b + 1 && L <= 30, "bad lifting log size %u / blowup %u", L, b);
+  const uint32_t s = L - b;
+  SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(L + 1), L - 1, tw->root_initial_index, tw->root_log_size),
+             "domain of log size %u is not in the twiddle tower", L);
+  std::vector<uint32_t> acc(13 * std::max<uint64_t>(n_acc, 1));
+  for (uint64_t k = 0; k < n_acc; k++) {
+    SH_REQUIRE(ctx, partial_logs[k] >= 1 && partial_logs[k] <= s, "partial numerator log size %u not in [1, %u]",
+               partial_logs[k], s);
+    for (int w = 0; w < 8; w++) acc[13 * k + w] = points[8 * k + w];
+    for (int w = 0; w < 4; w++) acc[13 * k + 8 + w] = first_linear[4 * k + w];
+    acc[13 * k + 12] = partial_logs[k];
+  }
+  // subdomain: CircleDomain(Coset(half_odds(L-1).initial, s - 1)) (domain.rs:72-82 split)
+  const uint32_t half_init = sh::subgroup_gen(L + 1);
+  sh_point_tab half;
+  SH_TRY(sh_point_tab_make(ctx, half_init, s - 1, &half));
+  void *d_acc, *d_parts;
+  SH_TRY(sh_stage(ctx, acc.data(), acc.size() * 4, &d_acc));
+  std::vector<const void*> parts(4 * std::max<uint64_t>(n_acc, 1), nullptr);
+  for (uint64_t i = 0; i < 4 * n_acc; i++) parts[i] = d_partials[i];
+  SH_TRY(sh_stage(ctx, parts.data(), parts.size() * sizeof(void*), &d_parts));
+  void* scr;
+  SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 16ull << s, &scr));
+  uint32_t* q = (uint32_t*)scr;
+  const void* qcols[8] = {q, q + (1u << s), q + (2u << s), q + (3u << s), d_out[0], d_out[1], d_out[2], d_out[3]};
+  void* d_q;
+  SH_TRY(sh_stage(ctx, qcols, sizeof(qcols), &d_q));
+  uint32_t* const* d_qtab = (uint32_t* const*)d_q;
+  {
+    SpanGuard span(ctx, "quotients_combine");
+    AccSpec spec = {(uint32_t)n_acc, s};
+    k_combine<<<((1u << s) + 255) / 256, 256, 0, ctx->stream>>>((const uint32_t* const*)d_parts,
+                                                                 (const uint32_t*)d_acc, spec, half, d_qtab);
+    SH_TRY(sh_launch_check(ctx, "combine"));
+  }
+  // interpolate on the subdomain (its twiddles are prefixes of the domain-L layers), evaluate on L
+  SH_TRY(sh_fft_interpolate_batch(ctx, tw, (const uint32_t* const*)d_qtab, d_qtab, 4, s, L, half_init));
+  SH_TRY(sh_fft_evaluate_batch(ctx, tw, (const uint32_t* const*)d_qtab, d_qtab + 4, 4, s, L, L, half_init));
+  return SH_OK;
+}
+
+extern "C" int sh_lift_and_accumulate(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
+                                      uint64_t n, uint32_t* const* d_out) {
+  if (!ctx || !d_out || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, n > 0, "lift_and_accumulate of no columns");
+  SH_REQUIRE(ctx, log_sizes[0] >= 1, "A column must be of length at least 2.");
+  for (uint64_t k = 1; k < n; k++)
+    SH_REQUIRE(ctx, log_sizes[k] > log_sizes[k - 1], "columns must be strictly ascending in size");
+  const uint32_t L = log_sizes[n - 1];
+  SH_REQUIRE(ctx, L <= 30, "log size %u too large", L);
+  void *d_tab, *d_logs, *d_o;
+  SH_TRY(sh_stage(ctx, d_cols, 4 * n * sizeof(void*), &d_tab));
+  SH_TRY(sh_stage(ctx, log_sizes, n * 4, &d_logs));
+  SH_TRY(sh_stage(ctx, d_out, 4 * sizeof(void*), &d_o));
+  SpanGuard span(ctx, "lift_and_accumulate");
+  k_lift_acc<<<((1u << L) + 255) / 256, 256, 0, ctx->stream>>>((const uint32_t* const*)d_tab,
+                                                               (const uint32_t*)d_logs, (uint32_t)n, L,
+                                                               (uint32_t* const*)d_o);
+  return sh_launch_check(ctx, "lift_and_accumulate");
+}
+
+extern "C" int sh_accumulate(sh_ctx* ctx, uint32_t* const* d_col, const uint32_t* const* d_other, uint32_t log_size) {
+  if (!ctx || !d_col || !d_other) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, log_size <= 30, "log size %u too large", log_size);
+  void *d_a, *d_b;
+  SH_TRY(sh_stage(ctx, d_col, 4 * sizeof(void*), &d_a));
+  SH_TRY(sh_stage(ctx, d_other, 4 * sizeof(void*), &d_b));
+  SpanGuard span(ctx, "accumulate");
+  k_accumulate<<<((1u << log_size) + 255) / 256, 256, 0, ctx->stream>>>((uint32_t* const*)d_a,
+                                                                        (const uint32_t* const*)d_b, 1u << log_size);
+  return sh_launch_check(ctx, "accumulate");
+}

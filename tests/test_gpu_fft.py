@@ -51,7 +51,7 @@ def test_interpolate_matches_cpu(ctx, backend, log):
     tree.free()
 
 
-@pytest.mark.parametrize("n,m", [(1, 1), (2, 2), (1, 2), (3, 3), (3, 4), (0, 4), (5, 7), (12, 13),
+@pytest.mark.parametrize("n,m", [(1, 1), (2, 2), (1, 2), (0, 1), (0, 2), (3, 3), (3, 4), (0, 4), (5, 7), (12, 13),
                                  (13, 14), (14, 15), (16, 17), (17, 18), (10, 14), (18, 19)])
 def test_evaluate_matches_cpu(ctx, backend, n, m):
     rng = np.random.default_rng(100 * n + m)
