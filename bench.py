@@ -93,6 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["uneven", "plain"], default="uneven")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build runs on a "
@@ -117,6 +118,11 @@ def main():
     ldes = [BaseColumn(ctx, lg + b) for lg in logs]
     layers = [HashLayer(ctx, k) for k in range(L + 1)]
     log("[bench] inputs resident in HBM")
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        be.autotune_fft(logs, b, tree)
+        tuned = {m: be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})}
+        log(f"[bench] autotuned FFT pass shapes in {time.perf_counter() - t0:.1f} s: {tuned}")
 
     _, _, _, root0 = be.commit_trace(evals, b, tree, coeffs, ldes, layers)
     for _ in range(args.warmup):
@@ -185,6 +191,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "largest_kernel_class": biggest,
+        "fft_tuning": {str(m): be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})},
         "root": root.hex(),
     }
     if not args.no_cpu_baseline:

@@ -69,6 +69,21 @@ int sh_memset(sh_ctx *ctx, void *d_dst, int value, uint64_t bytes);
  * SIMD crates/stwo/src/prover/backend/simd/bit_reverse.rs:25-107). */
 int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
 
+/* Tuning of the FFT pass shape for transforms of log size log_size (no
+ * counterpart in the reference; the SIMD backend's split is fixed at compile
+ * time): contiguous block pass of 2^block_log words, strided passes of
+ * stride_layers layers over 2^tile_log groups, register stages of at most
+ * max_stage layers.  Defaults 13, 8, 5, 4.  Results never depend on it. */
+int sh_set_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
+                      uint32_t tile_log, uint32_t max_stage);
+int sh_get_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t out[4]);
+/* Autotuner (no counterpart in the reference): for the column sizes of a
+ * workload, times candidate pass shapes of every interpolation and
+ * evaluation size it uses, on batches of the workload's width, and installs
+ * the fastest in the context. */
+int sh_autotune_fft(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *log_sizes, uint64_t n,
+                    uint32_t log_blowup);
+
 /* ---------------------------------------------------------------- PolyOps */
 /* PolyOps::precompute_twiddles (crates/stwo/src/prover/poly/circle/ops.rs:143;
  * SIMD crates/stwo/src/prover/backend/simd/circle.rs:431-462).  Root coset is

@@ -57,6 +57,9 @@ SIGNATURES = {
     "sh_lift_and_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
     "sh_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u32]),
     "sh_grind": (c_int, [c_vp, c_vp, c_u32, c_u64p]),
+    "sh_set_fft_tuning": (c_int, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32]),
+    "sh_get_fft_tuning": (c_int, [c_vp, c_u32, c_vp]),
+    "sh_autotune_fft": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32]),
 }
 
 _lib = None

@@ -284,6 +284,20 @@ class HipBackend:
                                                   len(cols), _ptrs(layers), root), "commit_mixed")
         return layers, bytes(root)
 
+    # ---- pass-shape tuning (no reference counterpart) --------------------------------------------
+    def autotune_fft(self, log_sizes, log_blowup, twiddles):
+        self._chk(self.lib.sh_autotune_fft(self.h, twiddles.h, _u32s(log_sizes), len(log_sizes), log_blowup),
+                  "autotune_fft")
+
+    def fft_tuning(self, log_size):
+        out = (ctypes.c_uint32 * 4)()
+        self._chk(self.lib.sh_get_fft_tuning(self.h, log_size, out), "get_fft_tuning")
+        return tuple(out)
+
+    def set_fft_tuning(self, log_size, block_log, stride_layers, tile_log, max_stage):
+        self._chk(self.lib.sh_set_fft_tuning(self.h, log_size, block_log, stride_layers, tile_log, max_stage),
+                  "set_fft_tuning")
+
     # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------
     def commit_trace(self, evals, log_blowup, twiddles, coeffs=None, ldes=None, layers=None,
                      want_root=True):

@@ -1,6 +1,16 @@
 // Batched trace commitment: interpolate + LDE + lifted Merkle tree over many
 // columns of uneven sizes (TreeBuilder::extend_evals followed by
 // CommitmentTreeProver::new, crates/stwo/src/prover/pcs/mod.rs:325-334, 360-386).
+//
+// Scheduling: the FFT chains (interpolate, then LDE) of the size classes run
+// back to back, smallest first, on an auxiliary stream; the leaf hashing runs
+// on the main stream, one size group at a time in ascending size, each group
+// waiting only for its own class's LDE.  A small group (few rows, many
+// columns, long per-row hash chains, too few lanes to fill the device) thus
+// runs beside the next class's FFT instead of after it.  FFT classes are not
+// run beside each other: each already fills the device, and running two
+// concurrently was measured slower than running them in turn.  The inner
+// layers follow on the main stream.
 #include <algorithm>
 
 #include "common.h"
@@ -11,7 +21,7 @@ int sh_evaluate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* cons
                         const uint32_t* coeff_logs, const uint32_t* domain_logs, uint32_t* const* h_dst,
                         uint64_t n);
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
-                         uint32_t L, uint32_t* d_out);
+                         uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready);
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers);
 
 extern "C" int sh_commit_trace(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
@@ -21,24 +31,49 @@ extern "C" int sh_commit_trace(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   if (!ctx || !tw || !d_layers || (n && (!d_evals || !d_coeffs || !log_sizes || !d_ldes)))
     return SH_ERR_INVALID;
   SH_REQUIRE(ctx, n > 0, "sh_commit_trace needs at least one column");
-  SH_TRY(sh_interpolate_grouped(ctx, tw, d_evals, d_coeffs, log_sizes, n));
-  std::vector<uint32_t> dl(n);
-  uint32_t L = 0;
-  for (uint64_t i = 0; i < n; i++) {
-    dl[i] = log_sizes[i] + log_blowup;
-    L = std::max(L, dl[i]);
+  SH_REQUIRE(ctx, ctx->stream == ctx->main_stream, "sh_commit_trace called from a forked stream");
+  // size classes, ascending
+  std::vector<uint32_t> classes;
+  for (uint64_t i = 0; i < n; i++) classes.push_back(log_sizes[i]);
+  std::sort(classes.begin(), classes.end());
+  classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
+  const uint32_t L = classes.back() + log_blowup;
+
+  // fork: every auxiliary stream starts after the work already on the main stream
+  hipEvent_t fork = sh_sync_event(ctx, 0);
+  SH_CHECK_HIP(ctx, hipEventRecord(fork, ctx->main_stream));
+  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamWaitEvent(a, fork, 0));
+  std::vector<hipEvent_t> ready(classes.size());
+  for (size_t k = 0; k < classes.size(); k++) {
+    const uint32_t m = classes[k];
+    std::vector<const uint32_t*> ev;
+    std::vector<uint32_t*> co, ld;
+    for (uint64_t i = 0; i < n; i++)
+      if (log_sizes[i] == m) {
+        ev.push_back(d_evals[i]);
+        co.push_back(d_coeffs[i]);
+        ld.push_back(d_ldes[i]);
+      }
+    std::vector<uint32_t> cl(ev.size(), m), dl(ev.size(), m + log_blowup);
+    StreamScope scope(ctx, ctx->aux[0]);
+    SH_TRY(sh_interpolate_grouped(ctx, tw, ev.data(), co.data(), cl.data(), ev.size()));
+    SH_TRY(sh_evaluate_grouped(ctx, tw, (const uint32_t* const*)co.data(), cl.data(), dl.data(), ld.data(),
+                               ev.size()));
+    ready[k] = sh_sync_event(ctx, 1 + k);
+    SH_CHECK_HIP(ctx, hipEventRecord(ready[k], ctx->stream));
   }
-  SH_TRY(sh_evaluate_grouped(ctx, tw, (const uint32_t* const*)d_coeffs, log_sizes, dl.data(), d_ldes, n));
+  // leaves on the main stream; group k (LDE log classes[k] + b) waits for class k
   std::vector<uint64_t> order(n);
   for (uint64_t i = 0; i < n; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) { return dl[x] < dl[y]; });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint64_t x, uint64_t y) { return log_sizes[x] < log_sizes[y]; });
   std::vector<const uint32_t*> cols(n);
   std::vector<uint32_t> logs(n);
   for (uint64_t i = 0; i < n; i++) {
     cols[i] = d_ldes[order[i]];
-    logs[i] = dl[order[i]];
+    logs[i] = log_sizes[order[i]] + log_blowup;
   }
-  SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), n, L, (uint32_t*)d_layers[L]));
+  SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), n, L, (uint32_t*)d_layers[L], ready.data()));
   SH_TRY(sh_build_layers_impl(ctx, L, (uint32_t* const*)d_layers));
   if (root) {
     SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/stwo_hip.h"
@@ -23,14 +24,16 @@ struct sh_scratch {
 
 struct sh_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;     // the stream every entry point orders its work on
+  hipStream_t main_stream = nullptr;
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};  // fork/join helpers inside one call
   std::string last_error;
-  // pinned host ring + device mirror for per-call tables
+  // pinned host arena + device mirror for per-call tables; identical tables are
+  // reused (content-addressed), so steady-state calls issue no copies
   uint8_t* h_ring = nullptr;
   uint8_t* d_ring = nullptr;
   uint64_t ring_size = 0, ring_off = 0;
-  hipEvent_t ring_fence = nullptr;
-  bool ring_fence_valid = false;
+  std::unordered_map<std::string, uint64_t> table_cache;
   // profiling (hipEvent pairs around each kernel launch, tagged by span class)
   bool profile = false;
   std::vector<sh_prof_rec> prof;
@@ -38,6 +41,11 @@ struct sh_ctx {
   // named scratch buffers
   sh_scratch scratch[8];
   hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+  std::vector<hipEvent_t> sync_events;  // fork/join events (no timing)
+  // FFT pass shape per transform log size (see fft.hip)
+  struct FftTune {
+    uint8_t block_log, stride_layers, tile_log, max_stage;
+  } fft_tune[32];
 };
 
 struct sh_twiddles {
@@ -74,9 +82,11 @@ int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...);
     if (_r != SH_OK) return _r; \
   } while (0)
 
-// Copies a small host table into the device ring (stream-ordered) and returns
-// its device address.
+// Copies a small host table into the device arena (stream-ordered, once per
+// distinct content) and returns its device address.
 int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev);
+// The i-th fork/join event of the context (created on first use).
+hipEvent_t sh_sync_event(sh_ctx* ctx, size_t i);
 // Ensures scratch buffer `slot` holds at least `bytes` and returns it.
 int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out);
 // Profiling spans around launches.
@@ -134,3 +144,11 @@ __device__ __forceinline__ sh::mpt sh_circle_domain_point(const sh_point_tab& ha
   sh::mpt p = sh_tab_point(half, j - hs);
   return {p.x, sh::mneg(p.y)};
 }
+
+// Runs the enclosed launches on another stream of the context.
+struct StreamScope {
+  sh_ctx* ctx;
+  hipStream_t saved;
+  StreamScope(sh_ctx* c, hipStream_t s) : ctx(c), saved(c->stream) { c->stream = s; }
+  ~StreamScope() { ctx->stream = saved; }
+};

@@ -8,7 +8,7 @@
 
 #include "common.h"
 
-static const uint64_t kRingBytes = 4ull << 20;
+static const uint64_t kRingBytes = 16ull << 20;
 
 int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) {
@@ -37,15 +37,17 @@ extern "C" int sh_ctx_create(int device, sh_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return SH_ERR_HIP;
   sh_ctx* ctx = new sh_ctx();
   ctx->device = device;
-  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->h_ring, kRingBytes, hipHostMallocDefault) != hipSuccess ||
+  bool ok = hipStreamCreateWithFlags(&ctx->main_stream, hipStreamNonBlocking) == hipSuccess;
+  for (auto& a : ctx->aux) ok = ok && hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess;
+  ctx->stream = ctx->main_stream;
+  if (!ok || hipHostMalloc((void**)&ctx->h_ring, kRingBytes, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&ctx->d_ring, kRingBytes) != hipSuccess ||
-      hipEventCreateWithFlags(&ctx->ring_fence, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->timer_start) != hipSuccess || hipEventCreate(&ctx->timer_stop) != hipSuccess) {
     sh_ctx_destroy(ctx);
     return SH_ERR_HIP;
   }
   ctx->ring_size = kRingBytes;
+  for (auto& t : ctx->fft_tune) t = {13, 8, 5, 4};
   *out = ctx;
   return SH_OK;
 }
@@ -59,14 +61,18 @@ extern "C" int sh_ctx_destroy(sh_ctx* ctx) {
     hipEventDestroy(r.stop);
   }
   for (auto e : ctx->event_pool) hipEventDestroy(e);
+  for (auto e : ctx->sync_events) hipEventDestroy(e);
   for (auto& s : ctx->scratch)
     if (s.ptr) hipFree(s.ptr);
+  for (auto a : ctx->aux)
+    if (a) hipStreamSynchronize(a);
   if (ctx->d_ring) hipFree(ctx->d_ring);
   if (ctx->h_ring) hipHostFree(ctx->h_ring);
-  if (ctx->ring_fence) hipEventDestroy(ctx->ring_fence);
   if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
   if (ctx->timer_stop) hipEventDestroy(ctx->timer_stop);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  for (auto a : ctx->aux)
+    if (a) hipStreamDestroy(a);
+  if (ctx->main_stream) hipStreamDestroy(ctx->main_stream);
   delete ctx;
   return SH_OK;
 }
@@ -81,20 +87,36 @@ extern "C" int sh_sync(sh_ctx* ctx) {
   return SH_OK;
 }
 
-// The ring is reused front to back; before wrapping, wait until the copies
-// and kernels that read the previous round of tables have completed.
+// Tables are appended to the arena and never rewritten until the arena is
+// reset, which first waits for every stream of the context: a table that an
+// enqueued kernel reads stays valid, and a table seen before is reused without
+// another copy (its bytes are compared in full, not just hashed).
 int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
   uint64_t need = (bytes + 255) & ~255ull;
-  SH_REQUIRE(ctx, need <= ctx->ring_size, "per-call table of %llu bytes exceeds the staging ring",
+  SH_REQUIRE(ctx, need <= ctx->ring_size, "per-call table of %llu bytes exceeds the staging arena",
              (unsigned long long)bytes);
+  std::string key((const char*)host, bytes);
+  auto it = ctx->table_cache.find(key);
+  if (it != ctx->table_cache.end()) {
+    *dev = ctx->d_ring + it->second;
+    return SH_OK;
+  }
   if (ctx->ring_off + need > ctx->ring_size) {
-    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
+    for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
     ctx->ring_off = 0;
+    ctx->table_cache.clear();
   }
   uint8_t* h = ctx->h_ring + ctx->ring_off;
   uint8_t* d = ctx->d_ring + ctx->ring_off;
   memcpy(h, host, bytes);
   SH_CHECK_HIP(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (key.size() <= 64 * 1024) {
+    // A cached table may later be read from another stream of the context: make
+    // the first copy complete before any reuse can be enqueued.
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->table_cache.emplace(std::move(key), ctx->ring_off);
+  }
   ctx->ring_off += need;
   *dev = d;
   return SH_OK;
@@ -104,7 +126,8 @@ int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out) {
   sh_scratch& s = ctx->scratch[slot];
   if (s.bytes < bytes) {
     if (s.ptr) {
-      SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
+      for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
       SH_CHECK_HIP(ctx, hipFree(s.ptr));
       s.ptr = nullptr;
       s.bytes = 0;
@@ -117,6 +140,15 @@ int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out) {
   }
   *out = s.ptr;
   return SH_OK;
+}
+
+hipEvent_t sh_sync_event(sh_ctx* ctx, size_t i) {
+  while (ctx->sync_events.size() <= i) {
+    hipEvent_t e = nullptr;
+    hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    ctx->sync_events.push_back(e);
+  }
+  return ctx->sync_events[i];
 }
 
 static hipEvent_t pool_event(sh_ctx* ctx) {
@@ -208,7 +240,8 @@ extern "C" int sh_alloc(sh_ctx* ctx, uint64_t bytes, void** d_out) {
 extern "C" int sh_free(sh_ctx* ctx, void* d) {
   if (!ctx) return SH_ERR_INVALID;
   if (!d) return SH_OK;
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
+  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
   SH_CHECK_HIP(ctx, hipFree(d));
   return SH_OK;
 }
@@ -321,3 +354,25 @@ bool is_doubling_of(uint32_t init, uint32_t log, uint32_t root_init, uint32_t ro
   return (uint32_t)ri == (init & 0x7fffffffu);
 }
 }  // namespace sh
+
+extern "C" int sh_set_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
+                                 uint32_t tile_log, uint32_t max_stage) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, log_size < 32, "log size %u out of range", log_size);
+  SH_REQUIRE(ctx, block_log >= 1 && block_log <= 13, "block_log %u not in [1, 13]", block_log);
+  SH_REQUIRE(ctx, tile_log <= 6 && stride_layers >= 1 && stride_layers + tile_log <= 13,
+             "strided pass of %u layers x 2^%u groups exceeds 32 KiB of LDS", stride_layers, tile_log);
+  SH_REQUIRE(ctx, max_stage >= 1 && max_stage <= 5, "max_stage %u not in [1, 5]", max_stage);
+  ctx->fft_tune[log_size] = {(uint8_t)block_log, (uint8_t)stride_layers, (uint8_t)tile_log, (uint8_t)max_stage};
+  return SH_OK;
+}
+
+extern "C" int sh_get_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t out[4]) {
+  if (!ctx || !out || log_size >= 32) return SH_ERR_INVALID;
+  const auto& t = ctx->fft_tune[log_size];
+  out[0] = t.block_log;
+  out[1] = t.stride_layers;
+  out[2] = t.tile_log;
+  out[3] = t.max_stage;
+  return SH_OK;
+}

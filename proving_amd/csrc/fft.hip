@@ -25,23 +25,22 @@
 namespace {
 
 constexpr uint32_t kThreads = 256;
-constexpr uint32_t kBlockLog = 13;  // contiguous pass: up to 2^13 words (32 KiB LDS)
-constexpr uint32_t kStrideLayers = 8;
-constexpr uint32_t kTileLog = 5;  // 32 consecutive groups per strided pass
 
 struct FftPass {
   uint32_t lo, hi, tl;
 };
 
-// Layers [0, top) split into passes, ascending.
-std::vector<FftPass> plan_passes(uint32_t top) {
+// Layers [0, top) split into passes, ascending: a contiguous block pass of up
+// to 2^block_log words, then strided passes of stride_layers layers over tiles
+// of 2^tile_log groups.
+std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
   std::vector<FftPass> v;
-  uint32_t bh = top < kBlockLog ? top : kBlockLog;
+  uint32_t bh = top < t.block_log ? top : t.block_log;
   v.push_back({0, bh, 0});
   uint32_t l = bh;
   while (l < top) {
-    uint32_t g = top - l < kStrideLayers ? top - l : kStrideLayers;
-    uint32_t tl = l < kTileLog ? l : kTileLog;
+    uint32_t g = top - l < t.stride_layers ? top - l : t.stride_layers;
+    uint32_t tl = l < t.tile_log ? l : t.tile_log;
     v.push_back({l, l + g, tl});
     l += g;
   }
@@ -77,8 +76,26 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
     const uint32_t jr = q >> tl;
     const uint32_t jf = ((jr >> s) << (s + R)) | (jr & ((1u << s) - 1));
     uint32_t v[1 << R];
+    // s == 0 in a contiguous block: the lane's 2^R words are adjacent, so they
+    // move as 128-bit LDS accesses instead of 2^R lanes-strided words (which
+    // would serialise on a few banks)
+    const bool vec = s == 0 && tl == 0;
+    if constexpr (R >= 2) {
+      if (vec) {
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) v[e] = lds[((jf | ((uint32_t)e << s)) << tl) | t];
+        for (int e = 0; e < (1 << R); e += 4) {
+          uint4 q4 = *(const uint4*)&lds[jf + e];
+          v[e] = q4.x;
+          v[e + 1] = q4.y;
+          v[e + 2] = q4.z;
+          v[e + 3] = q4.w;
+        }
+      }
+    }
+    if (!vec || R < 2) {
+#pragma unroll
+      for (int e = 0; e < (1 << R); e++) v[e] = lds[((jf | ((uint32_t)e << s)) << tl) | t];
+    }
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const int b = INVERSE ? k : R - 1 - k;
@@ -105,8 +122,17 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
         }
       }
     }
+    if constexpr (R >= 2) {
+      if (vec) {
 #pragma unroll
-    for (int e = 0; e < (1 << R); e++) lds[((jf | ((uint32_t)e << s)) << tl) | t] = v[e];
+        for (int e = 0; e < (1 << R); e += 4)
+          *(uint4*)&lds[jf + e] = make_uint4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+      }
+    }
+    if (!vec || R < 2) {
+#pragma unroll
+      for (int e = 0; e < (1 << R); e++) lds[((jf | ((uint32_t)e << s)) << tl) | t] = v[e];
+    }
   }
 }
 
@@ -121,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
                                                        uint32_t* const* dst_cols, uint32_t src_mask,
                                                        const uint32_t* __restrict__ tw, uint32_t K,
                                                        uint32_t Pl, uint32_t lo, uint32_t hi,
-                                                       uint32_t tl, uint32_t scale) {
+                                                       uint32_t tl, uint32_t scale, uint32_t max_stage) {
   extern __shared__ uint32_t lds[];
   const uint32_t G = hi - lo;
   const uint32_t col = blockIdx.y;
@@ -151,13 +177,16 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   __syncthreads();
 
   const uint32_t tw_end = 1u << K;
-  const uint32_t n_stages = (G + 3) / 4;
+  // G layers in stages of at most max_stage, as even as possible (13 = 5 + 4 + 4);
+  // inverse runs them upwards, forward downwards
+  const uint32_t n_stages = (G + max_stage - 1) / max_stage;
+  const uint32_t st_base = G / n_stages, st_extra = G % n_stages;
   for (uint32_t st = 0; st < n_stages; st++) {
-    // stages cover [0,4), [4,8), ... ; inverse runs them upwards, forward downwards
     const uint32_t si = INVERSE ? st : n_stages - 1 - st;
-    const uint32_t s = 4 * si;
-    const uint32_t r = G - s < 4 ? G - s : 4;
+    const uint32_t s = si * st_base + (si < st_extra ? si : st_extra);
+    const uint32_t r = st_base + (si < st_extra ? 1 : 0);
     switch (r) {
+      case 5: fft_stage<INVERSE, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
       case 4: fft_stage<INVERSE, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
       case 3: fft_stage<INVERSE, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
       case 2: fft_stage<INVERSE, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
@@ -291,7 +320,8 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
     return sh_launch_check(ctx, "fft_tiny");
   }
   uint32_t scale = sh::minv(1u << m);
-  auto passes = plan_passes(m);
+  const sh_ctx::FftTune tune = ctx->fft_tune[m];
+  auto passes = plan_passes(m, tune);
   for (size_t k = 0; k < passes.size(); k++) {
     const FftPass& p = passes[k];
     uint32_t G = p.hi - p.lo;
@@ -301,7 +331,7 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
     SpanGuard span(ctx, "fft_ifft_pass");
     k_fft_pass<true><<<grid, kThreads, lds, ctx->stream>>>(
         s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi, p.tl,
-        k + 1 == passes.size() ? scale : 0u);
+        k + 1 == passes.size() ? scale : 0u, tune.max_stage);
     SH_TRY(sh_launch_check(ctx, "ifft_pass"));
   }
   return SH_OK;
@@ -327,7 +357,8 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
     k_replicate<<<grid, 256, 0, ctx->stream>>>(d_src, d_dst, m);
     return sh_launch_check(ctx, "replicate");
   }
-  auto passes = plan_passes(n);
+  const sh_ctx::FftTune tune = ctx->fft_tune[m];
+  auto passes = plan_passes(n, tune);
   for (int k = (int)passes.size() - 1; k >= 0; k--) {
     const FftPass& p = passes[k];
     uint32_t G = p.hi - p.lo;
@@ -337,7 +368,8 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
     const uint32_t* const* s = first ? d_src : (const uint32_t* const*)d_dst;
     SpanGuard span(ctx, "fft_rfft_pass");
     k_fft_pass<false><<<grid, kThreads, lds, ctx->stream>>>(
-        s, d_dst, first ? src_mask : 0xffffffffu, tw->tw, tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u);
+        s, d_dst, first ? src_mask : 0xffffffffu, tw->tw, tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u,
+        tune.max_stage);
     SH_TRY(sh_launch_check(ctx, "rfft_pass"));
   }
   return SH_OK;
@@ -429,4 +461,89 @@ extern "C" int sh_evaluate_polynomials(sh_ctx* ctx, const sh_twiddles* tw, const
   std::vector<uint32_t> dl(n);
   for (uint64_t i = 0; i < n; i++) dl[i] = coeff_logs[i] + log_blowup;
   return sh_evaluate_grouped(ctx, tw, d_coeffs, coeff_logs, dl.data(), d_outs, n);
+}
+
+// Autotuner: for every transform size the workload uses (interpolations of
+// log n, evaluations onto log n + log_blowup), times each candidate pass shape
+// on a batch of the class's real width and keeps the fastest.  Inputs are
+// scratch columns; the shape only changes speed, never results.
+extern "C" int sh_autotune_fft(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* log_sizes, uint64_t n,
+                               uint32_t log_blowup) {
+  if (!ctx || !tw || (n && !log_sizes)) return SH_ERR_INVALID;
+  std::vector<std::pair<uint32_t, uint32_t>> cls;  // (log, count)
+  for (uint64_t i = 0; i < n; i++) {
+    SH_TRY(check_domain(ctx, tw, log_sizes[i] + log_blowup));
+    bool seen = false;
+    for (auto& c : cls)
+      if (c.first == log_sizes[i]) {
+        c.second++;
+        seen = true;
+      }
+    if (!seen) cls.push_back({log_sizes[i], 1});
+  }
+  std::vector<sh_ctx::FftTune> cands;
+  for (uint8_t blk : {11, 12, 13})
+    for (uint8_t st : {3, 4, 5, 6, 7, 8})
+      for (uint8_t tl : {3, 4, 5})
+        for (uint8_t ms : {3, 4, 5})
+          if (st + tl <= 13) cands.push_back({blk, st, tl, ms});
+  hipEvent_t e0, e1;
+  SH_CHECK_HIP(ctx, hipEventCreate(&e0));
+  SH_CHECK_HIP(ctx, hipEventCreate(&e1));
+  int rc = SH_OK;
+  for (auto& c : cls) {
+    const uint32_t lg = c.first, cnt = c.second;
+    if (lg + log_blowup <= 2) continue;
+    uint32_t* src = nullptr;
+    uint32_t* dst = nullptr;
+    if (hipMalloc(&src, (4ull << lg) * cnt) != hipSuccess ||
+        hipMalloc(&dst, (4ull << (lg + log_blowup)) * cnt) != hipSuccess) {
+      if (src) hipFree(src);
+      rc = sh_fail(ctx, SH_ERR_OOM, "autotune scratch allocation failed");
+      break;
+    }
+    std::vector<const void*> tab;
+    for (uint32_t k = 0; k < cnt; k++) tab.push_back(src + ((uint64_t)k << lg));
+    for (uint32_t k = 0; k < cnt; k++) tab.push_back(dst + ((uint64_t)k << (lg + log_blowup)));
+    void* d_tab = nullptr;
+    rc = sh_stage(ctx, tab.data(), tab.size() * sizeof(void*), &d_tab);
+    const uint32_t* const* ts = (const uint32_t* const*)d_tab;
+    uint32_t* const* td = (uint32_t* const*)((const void**)d_tab + cnt);
+    for (int which = 0; which < 2 && rc == SH_OK; which++) {
+      // which 0: interpolation of log lg (in place on src); 1: evaluation lg -> lg + b
+      const uint32_t m = which == 0 ? lg : lg + log_blowup;
+      if (m <= 2 || (which == 1 && log_blowup == 0)) continue;
+      float best = 1e30f;
+      sh_ctx::FftTune best_t = ctx->fft_tune[m];
+      for (const auto& t : cands) {
+        if (t.block_log >= m && (t.stride_layers != 8 || t.tile_log != 5)) continue;  // same plan
+        ctx->fft_tune[m] = t;
+        float total = 0.f;
+        for (int rep = 0; rep < 3 && rc == SH_OK; rep++) {
+          SH_CHECK_HIP(ctx, hipEventRecord(e0, ctx->stream));
+          if (which == 0)
+            rc = sh_fft_interpolate_batch(ctx, tw, ts, (uint32_t* const*)ts, cnt, m, m, sh::subgroup_gen(m + 1));
+          else
+            rc = sh_fft_evaluate_batch(ctx, tw, ts, td, cnt, lg, m, m, sh::subgroup_gen(m + 1));
+          SH_CHECK_HIP(ctx, hipEventRecord(e1, ctx->stream));
+          SH_CHECK_HIP(ctx, hipEventSynchronize(e1));
+          float ms = 0.f;
+          SH_CHECK_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+          if (rep > 0) total += ms;  // first run warms caches
+        }
+        if (total < best) {
+          best = total;
+          best_t = t;
+        }
+      }
+      ctx->fft_tune[m] = best_t;
+    }
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(src);
+    hipFree(dst);
+    if (rc != SH_OK) break;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return rc;
 }

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_commit_on_layer(uint32_t log_size,
 
 // Leaves for columns already sorted ascending (host table h_cols).
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
-                         uint32_t L, uint32_t* d_out) {
+                         uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready) {
   if (n == 0) {
     SpanGuard span(ctx, "merkle_empty");
     k_empty_hash<<<1, 1, 0, ctx->stream>>>(d_out);
@@ -291,6 +291,7 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
         a.hash_out = (uint32_t*)tmp;
       }
     }
+    if (group_ready) SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream, group_ready[g], 0));
     SpanGuard span(ctx, "merkle_leaves_group");
     k_leaves_group<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(a);
     SH_TRY(sh_launch_check(ctx, "leaves_group"));
@@ -330,7 +331,7 @@ int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers) {
 extern "C" int sh_merkle_build_leaves(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
                                       uint64_t n, uint32_t lifting_log_size, uint8_t* d_out) {
   if (!ctx || !d_out || (n && (!d_cols || !log_sizes))) return SH_ERR_INVALID;
-  return sh_build_leaves_impl(ctx, d_cols, log_sizes, n, lifting_log_size, (uint32_t*)d_out);
+  return sh_build_leaves_impl(ctx, d_cols, log_sizes, n, lifting_log_size, (uint32_t*)d_out, nullptr);
 }
 
 extern "C" int sh_merkle_build_next_layer(sh_ctx* ctx, const uint8_t* d_prev, uint32_t prev_log_size,
@@ -363,7 +364,7 @@ extern "C" int sh_merkle_commit_lifted(sh_ctx* ctx, const uint32_t* const* d_col
     logs[i] = log_sizes[order[i]];
   }
   SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), n, lifting_log_size,
-                              (uint32_t*)d_layers[lifting_log_size]));
+                              (uint32_t*)d_layers[lifting_log_size], nullptr));
   SH_TRY(sh_build_layers_impl(ctx, lifting_log_size, (uint32_t* const*)d_layers));
   if (root) {
     SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));

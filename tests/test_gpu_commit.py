@@ -62,3 +62,16 @@ def test_commit_repeatable(ctx, backend):
     for _ in range(3):
         _, _, _, r2 = backend.commit_trace(evals, 1, tree, coeffs=coeffs, ldes=ldes, layers=layers)
         assert r2 == root
+
+
+def test_autotune_keeps_results(ctx, backend):
+    """The pass shape changes only speed: commit before and after autotuning gives the same tree."""
+    logs = [9] * 5 + [14] * 4 + [15] * 2
+    host, tree, evals, coeffs, ldes, layers, root = run_commit(ctx, backend, logs, 9)
+    backend.autotune_fft(logs, 1, tree)
+    for m in (9, 10, 14, 15, 16):
+        backend.set_fft_tuning(m, 11, 3, 3, 3)
+    _, _, _, r1 = backend.commit_trace(evals, 1, tree, coeffs=coeffs, ldes=ldes, layers=layers)
+    backend.autotune_fft(logs, 1, tree)
+    _, _, _, r2 = backend.commit_trace(evals, 1, tree, coeffs=coeffs, ldes=ldes, layers=layers)
+    assert r1 == root and r2 == root
