@@ -73,7 +73,9 @@ int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
  * counterpart in the reference; the SIMD backend's split is fixed at compile
  * time): contiguous block pass of 2^block_log words, strided passes of
  * stride_layers layers over 2^tile_log groups, register stages of at most
- * max_stage layers.  Defaults 13, 8, 5, 4.  Results never depend on it. */
+ * max_stage layers.  Defaults are the autotuner's picks measured on MI355X
+ * for the benchmark's sizes (block 2^12 or 2^13, 5-8 layers, 32 groups,
+ * stages of 3-5).  Results never depend on it. */
 int sh_set_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
                       uint32_t tile_log, uint32_t max_stage);
 int sh_get_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t out[4]);

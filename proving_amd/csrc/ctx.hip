@@ -47,7 +47,16 @@ extern "C" int sh_ctx_create(int device, sh_ctx** out) {
     return SH_ERR_HIP;
   }
   ctx->ring_size = kRingBytes;
-  for (auto& t : ctx->fft_tune) t = {13, 8, 5, 4};
+  // Built-in pass shapes: the autotuner's picks for the uneven-commit size
+  // classes on MI355X (interpolations of 2^12..2^20, evaluations of 2^13..2^21),
+  // block pass of 2^12 elsewhere.  sh_autotune_fft refines them per workload.
+  for (uint32_t m = 0; m < 32; m++) ctx->fft_tune[m] = {(uint8_t)(m <= 13 ? 13 : 12), 8, 5, 4};
+  ctx->fft_tune[13] = {12, 8, 4, 5};
+  ctx->fft_tune[16] = {11, 8, 5, 3};
+  ctx->fft_tune[17] = {11, 5, 5, 3};
+  ctx->fft_tune[18] = {12, 7, 5, 5};
+  ctx->fft_tune[19] = {12, 6, 5, 4};
+  ctx->fft_tune[20] = {12, 8, 5, 5};
   *out = ctx;
   return SH_OK;
 }

@@ -47,6 +47,13 @@ std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
   return v;
 }
 
+// LDS word-address swizzle: bits 2-5 ^= bits 6-9.  It permutes words inside
+// each 64-word row and keeps aligned 4-word quads intact (so 128-bit LDS
+// accesses still work), and it is linear over XOR, so the address of a lane's
+// element e is sw(lane part) ^ sw(uniform part).  Without it, lanes whose
+// addresses differ by multiples of 64 words hit the same bank.
+__device__ __forceinline__ uint32_t sw(uint32_t a) { return a ^ (((a >> 6) & 15u) << 2); }
+
 __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t h) {
   // crates/stwo/src/prover/backend/cpu/circle.rs:306-323: [x, y] -> [y, -y, -x, x]
   uint32_t x = l1[(h >> 2) * 2], y = l1[(h >> 2) * 2 + 1];
@@ -76,6 +83,7 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
     const uint32_t jr = q >> tl;
     const uint32_t jf = ((jr >> s) << (s + R)) | (jr & ((1u << s) - 1));
     uint32_t v[1 << R];
+    const uint32_t lane_addr = sw((jf << tl) | t);
     // s == 0 in a contiguous block: the lane's 2^R words are adjacent, so they
     // move as 128-bit LDS accesses instead of 2^R lanes-strided words (which
     // would serialise on a few banks)
@@ -84,7 +92,7 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
       if (vec) {
 #pragma unroll
         for (int e = 0; e < (1 << R); e += 4) {
-          uint4 q4 = *(const uint4*)&lds[jf + e];
+          uint4 q4 = *(const uint4*)&lds[lane_addr ^ sw((uint32_t)e)];
           v[e] = q4.x;
           v[e + 1] = q4.y;
           v[e + 2] = q4.z;
@@ -94,7 +102,7 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
     }
     if (!vec || R < 2) {
 #pragma unroll
-      for (int e = 0; e < (1 << R); e++) v[e] = lds[((jf | ((uint32_t)e << s)) << tl) | t];
+      for (int e = 0; e < (1 << R); e++) v[e] = lds[lane_addr ^ sw((uint32_t)e << (s + tl))];
     }
 #pragma unroll
     for (int k = 0; k < R; k++) {
@@ -126,12 +134,12 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
       if (vec) {
 #pragma unroll
         for (int e = 0; e < (1 << R); e += 4)
-          *(uint4*)&lds[jf + e] = make_uint4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+          *(uint4*)&lds[lane_addr ^ sw((uint32_t)e)] = make_uint4(v[e], v[e + 1], v[e + 2], v[e + 3]);
       }
     }
     if (!vec || R < 2) {
 #pragma unroll
-      for (int e = 0; e < (1 << R); e++) lds[((jf | ((uint32_t)e << s)) << tl) | t] = v[e];
+      for (int e = 0; e < (1 << R); e++) lds[lane_addr ^ sw((uint32_t)e << (s + tl))] = v[e];
     }
   }
 }
@@ -166,12 +174,12 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   if (vec) {
     for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
       uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      *(uint4*)&lds[w] = *(const uint4*)&src[pos & src_mask];
+      *(uint4*)&lds[sw(w)] = *(const uint4*)&src[pos & src_mask];
     }
   } else {
     for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
       uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      lds[w] = src[pos & src_mask];
+      lds[sw(w)] = src[pos & src_mask];
     }
   }
   __syncthreads();
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   if (vec) {
     for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
       uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      uint4 v = *(const uint4*)&lds[w];
+      uint4 v = *(const uint4*)&lds[sw(w)];
       if (scale) {
         v.x = sh::mmul(v.x, scale);
         v.y = sh::mmul(v.y, scale);
@@ -210,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   } else {
     for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
       uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      uint32_t v = lds[w];
+      uint32_t v = lds[sw(w)];
       dst[pos] = scale ? sh::mmul(v, scale) : v;
     }
   }

@@ -148,8 +148,8 @@ __global__ __launch_bounds__(kThreads) void k_next_layers(const uint32_t* prev, 
     uint4* o = (uint4*)(outs.p[0] + 8 * i);
     o[0] = make_uint4(h[0], h[1], h[2], h[3]);
     o[1] = make_uint4(h[4], h[5], h[6], h[7]);
-#pragma unroll
-    for (int w = 0; w < 8; w++) node[threadIdx.x * 8 + w] = h[w];
+    *(uint4*)&node[threadIdx.x * 8] = make_uint4(h[0], h[1], h[2], h[3]);
+    *(uint4*)&node[threadIdx.x * 8 + 4] = make_uint4(h[4], h[5], h[6], h[7]);
   }
   uint32_t width = kThreads;  // nodes of the current level held by this workgroup
   for (uint32_t l = 1; l < levels; l++) {
@@ -162,15 +162,21 @@ __global__ __launch_bounds__(kThreads) void k_next_layers(const uint32_t* prev, 
     if (tid < width && gi < nl) {
       uint32_t m[16];
 #pragma unroll
-      for (int w = 0; w < 16; w++) m[w] = node[tid * 16 + w];
+      for (int w = 0; w < 16; w += 4) {
+        uint4 q = *(const uint4*)&node[tid * 16 + w];
+        m[w] = q.x;
+        m[w + 1] = q.y;
+        m[w + 2] = q.z;
+        m[w + 3] = q.w;
+      }
       sh::b2_init(h);
       sh::b2_compress(h, m, 64, true);
       mine = 1;
     }
     __syncthreads();
     if (mine) {
-#pragma unroll
-      for (int w = 0; w < 8; w++) node[tid * 8 + w] = h[w];
+      *(uint4*)&node[tid * 8] = make_uint4(h[0], h[1], h[2], h[3]);
+      *(uint4*)&node[tid * 8 + 4] = make_uint4(h[4], h[5], h[6], h[7]);
       uint4* o = (uint4*)(outs.p[l] + 8 * gi);
       o[0] = make_uint4(h[0], h[1], h[2], h[3]);
       o[1] = make_uint4(h[4], h[5], h[6], h[7]);
