@@ -130,24 +130,53 @@ def main():
     ctx.sync()
     log(f"[bench] warmup done ({args.warmup} steps)")
 
+    def timed(step_fn, steps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.timer_start()
+        for _ in range(steps):
+            step_fn()
+        gpu_ms = ctx.timer_stop()
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        return max(wall, gpu_ms / 1e3)
+
+    # (1) eager batched commit, with per-launch HIP-event spans for the kernel table / roofline
     ctx.profile(True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.timer_start()
-    for _ in range(args.steps):
-        be.commit_trace(evals, b, tree, coeffs, ldes, layers, want_root=False)
-    gpu_ms = ctx.timer_stop()
-    ctx.sync()
-    wall = time.perf_counter() - t0
+    sec_eager = timed(lambda: be.commit_trace(evals, b, tree, coeffs, ldes, layers, want_root=False),
+                      args.steps)
     prof = ctx.profile_report()
     ctx.profile(False)
-    root = layers[0].to_cpu()[0].tobytes()
-    assert root == root0, "commit is not deterministic across steps"
-    log(f"[bench] timed {args.steps} steps: {gpu_ms:.3f} ms on the stream, {wall * 1e3:.3f} ms wall")
+    assert layers[0].to_cpu()[0].tobytes() == root0, "commit is not deterministic across steps"
+    log(f"[bench] eager batched: {sec_eager * 1e3 / args.steps:.3f} ms/step")
 
-    sec = max(wall, gpu_ms / 1e3)
+    # (2) the same commit captured as one HIP graph: the headline number
+    graph = be.commit_trace_graph(evals, b, tree, coeffs, ldes, layers)
+    for _ in range(args.warmup):
+        graph.launch()
+    sec = timed(graph.launch, args.steps)
+    root = graph.root()
+    assert root == root0, "graph replay differs from the eager commit"
+    graph.free()
+    log(f"[bench] graph batched: {sec * 1e3 / args.steps:.3f} ms/step")
     ms_per_step = sec * 1e3 / args.steps
     value = n_cells * args.steps / sec
+
+    # (3) plain per-column drop-in: one interpolate and one evaluate call per column (as the
+    # reference's PolyOps calls are made column by column), then one lifted Merkle commit
+    plain_steps = max(1, args.steps // 4)
+    for e, c in zip(evals, coeffs):
+        c.copy_from(e)
+    def plain_step():
+        for c in coeffs:
+            be.interpolate(c, tree)
+        for c, l in zip(coeffs, ldes):
+            be.evaluate(c, c.log_size + b, tree, l)
+        be.commit_lifted(ldes, L, layers=layers, want_root=False)
+    plain_step()
+    assert layers[0].to_cpu()[0].tobytes() == root0, "plain per-column path differs from the batched commit"
+    sec_plain = timed(plain_step, plain_steps)
+    log(f"[bench] plain per-column drop-in: {sec_plain * 1e3 / plain_steps:.3f} ms/step")
 
     work_per = algorithmic_work(logs, b)
     kernels = {}
@@ -190,6 +219,12 @@ def main():
                    "pipeline": "interpolate + LDE + lifted Blake2s Merkle (sh_commit_trace)"},
         "roofline": roofline,
         "kernels": kernels,
+        "variants": {
+            "graph_batched_ms_per_step": round(ms_per_step, 4),
+            "eager_batched_ms_per_step": round(sec_eager * 1e3 / args.steps, 4),
+            "plain_per_column_ms_per_step": round(sec_plain * 1e3 / plain_steps, 4),
+            "graph_vs_plain": round(sec_plain / plain_steps / (sec / args.steps), 2),
+        },
         "largest_kernel_class": biggest,
         "fft_tuning": {str(m): be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})},
         "root": root.hex(),

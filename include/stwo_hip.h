@@ -65,6 +65,8 @@ int sh_free(sh_ctx *ctx, void *d);
 int sh_upload(sh_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int sh_download(sh_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 int sh_memset(sh_ctx *ctx, void *d_dst, int value, uint64_t bytes);
+/* Column clone (device to device, stream-ordered). */
+int sh_copy(sh_ctx *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 /* ColumnOps::bit_reverse_column (crates/stwo/src/prover/backend/mod.rs:37-43;
  * SIMD crates/stwo/src/prover/backend/simd/bit_reverse.rs:25-107). */
 int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
@@ -230,6 +232,19 @@ int sh_accumulate(sh_ctx *ctx, uint32_t *const *d_col, const uint32_t *const *d_
  * the smallest nonce whose proof-of-work hash over the channel digest has at
  * least pow_bits trailing zero bits. */
 int sh_grind(sh_ctx *ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t *nonce);
+
+/* ---------------------------------------------------------------- graph capture */
+/* hipGraph of sh_commit_trace with these exact arguments (no counterpart in the
+ * reference): one graph launch replays interpolate + LDE + Merkle on the same
+ * buffers.  The graph owns its staging tables and scratch; the buffers passed
+ * here must outlive it.  Contents of d_evals may change between launches. */
+typedef struct sh_graph sh_graph;
+int sh_commit_trace_graph_create(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_evals,
+                                 uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
+                                 uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
+                                 sh_graph **out);
+int sh_graph_launch(sh_ctx *ctx, sh_graph *g);
+int sh_graph_destroy(sh_ctx *ctx, sh_graph *g);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ SIGNATURES = {
     "sh_upload": (c_int, [c_vp, c_vp, c_vp, c_u64]),
     "sh_download": (c_int, [c_vp, c_vp, c_vp, c_u64]),
     "sh_memset": (c_int, [c_vp, c_vp, c_int, c_u64]),
+    "sh_copy": (c_int, [c_vp, c_vp, c_vp, c_u64]),
     "sh_bit_reverse_column": (c_int, [c_vp, c_vp, c_u32]),
     "sh_twiddles_create": (c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
     "sh_twiddles_destroy": (c_int, [c_vp, c_vp]),
@@ -60,6 +61,10 @@ SIGNATURES = {
     "sh_set_fft_tuning": (c_int, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32]),
     "sh_get_fft_tuning": (c_int, [c_vp, c_u32, c_vp]),
     "sh_autotune_fft": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32]),
+    "sh_commit_trace_graph_create": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp,
+                                             ctypes.POINTER(c_vp)]),
+    "sh_graph_launch": (c_int, [c_vp, c_vp]),
+    "sh_graph_destroy": (c_int, [c_vp, c_vp]),
 }
 
 _lib = None

@@ -130,6 +130,11 @@ class BaseColumn(DeviceBuffer):
     def to_cpu(self):
         return self.download(np.uint32, len(self))
 
+    def copy_from(self, other):
+        assert other.nbytes <= self.nbytes
+        self.ctx.check(self.ctx.lib.sh_copy(self.ctx.h, self.ptr, other.ptr, other.nbytes), "sh_copy")
+        return self
+
     def bit_reverse(self):
         self.ctx.check(self.ctx.lib.sh_bit_reverse_column(self.ctx.h, self.ptr, self.log_size),
                        "bit_reverse_column")
@@ -197,6 +202,25 @@ class TwiddleTree:
         self.h = None
 
 
+class CommitGraph:
+    """A captured batched commitment; launch() replays it on the context stream."""
+
+    def __init__(self, ctx, h, layers):
+        self.ctx, self.h, self.layers = ctx, h, layers
+
+    def launch(self):
+        self.ctx.check(self.ctx.lib.sh_graph_launch(self.ctx.h, self.h), "graph_launch")
+
+    def root(self):
+        self.ctx.sync()
+        return self.layers[0].to_cpu()[0].tobytes()
+
+    def free(self):
+        if self.h is not None and not self.ctx._closed:
+            self.ctx.lib.sh_graph_destroy(self.ctx.h, self.h)
+        self.h = None
+
+
 def canonic_half_initial(log_size):
     """CanonicCoset(log_size).half_coset initial index (half_odds(log_size - 1))."""
     return 1 << (31 - (log_size + 1))
@@ -260,14 +284,14 @@ class HipBackend:
                   "build_next_layer")
         return out
 
-    def commit_lifted(self, cols, lifting_log_size):
-        """MerkleProverLifted::commit: returns (layers root-first, root bytes)."""
-        layers = [HashLayer(self.ctx, k) for k in range(lifting_log_size + 1)]
-        root = (ctypes.c_uint8 * 32)()
+    def commit_lifted(self, cols, lifting_log_size, layers=None, want_root=True):
+        """MerkleProverLifted::commit: returns (layers root-first, root bytes or None)."""
+        layers = layers or [HashLayer(self.ctx, k) for k in range(lifting_log_size + 1)]
+        root = (ctypes.c_uint8 * 32)() if want_root else None
         self._chk(self.lib.sh_merkle_commit_lifted(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
                                                    len(cols), lifting_log_size, _ptrs(layers), root),
                   "commit_lifted")
-        return layers, bytes(root)
+        return layers, (bytes(root) if want_root else None)
 
     # ---- MerkleOps (crates/stwo/src/prover/vcs/ops.rs:8-29) ------------------------------------
     def commit_on_layer(self, log_size, prev, cols):
@@ -283,6 +307,14 @@ class HipBackend:
         self._chk(self.lib.sh_merkle_commit_mixed(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
                                                   len(cols), _ptrs(layers), root), "commit_mixed")
         return layers, bytes(root)
+
+    def commit_trace_graph(self, evals, log_blowup, twiddles, coeffs, ldes, layers):
+        """Captures commit_trace on these buffers as a HIP graph; returns a CommitGraph."""
+        h = ctypes.c_void_p()
+        self._chk(self.lib.sh_commit_trace_graph_create(
+            self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s([e.log_size for e in evals]), len(evals),
+            log_blowup, _ptrs(ldes), _ptrs(layers), ctypes.byref(h)), "commit_trace_graph_create")
+        return CommitGraph(self.ctx, h, layers)
 
     # ---- pass-shape tuning (no reference counterpart) --------------------------------------------
     def autotune_fft(self, log_sizes, log_blowup, twiddles):

@@ -110,6 +110,10 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
     *dev = ctx->d_ring + it->second;
     return SH_OK;
   }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  SH_CHECK_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cap));
+  SH_REQUIRE(ctx, cap == hipStreamCaptureStatusNone,
+             "a per-call table was not staged before graph capture (run the call eagerly first)");
   if (ctx->ring_off + need > ctx->ring_size) {
     SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
     for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
@@ -274,6 +278,13 @@ __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+extern "C" int sh_copy(sh_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes) {
+  if (!ctx) return SH_ERR_INVALID;
+  if (bytes == 0) return SH_OK;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return SH_OK;
 }
 
 extern "C" int sh_memset(sh_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
