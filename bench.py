@@ -94,6 +94,15 @@ def main():
     ap.add_argument("--workload", choices=["uneven", "plain"], default="uneven")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fft-tuning", default=None,
+                    help="JSON {log_size: [block_log, stride_layers, tile_log, max_stage]} to use instead "
+                         "of autotuning (the 'fft_tuning' field of an earlier run), so that a profiled run "
+                         "launches exactly the measured kernels")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain per-column and host-to-device variants (their smaller FFT "
+                         "launches would mix into a profiler's per-kernel averages)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device-to-device once (counter calibration run) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build runs on a "
@@ -102,6 +111,18 @@ def main():
 
     from proving_amd.backend import BaseColumn, Context, HashLayer, HipBackend
     from proving_amd.workloads import PLAIN_COMMIT, UNEVEN_COMMIT, cells as count_cells, splitmix64_columns
+
+    if args.copy_calibration:
+        ctx = Context(0)
+        n = 256 << 20
+        src, dst = ctx.alloc(n), ctx.alloc(n)
+        ctx.check(ctx.lib.sh_memset(ctx.h, src.ptr, 1, n), "memset")
+        ctx.sync()
+        ctx.check(ctx.lib.sh_copy(ctx.h, dst.ptr, src.ptr, n), "copy")
+        ctx.sync()
+        print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+        ctx.close()
+        return
 
     work = UNEVEN_COMMIT if args.workload == "uneven" else PLAIN_COMMIT
     logs, b, seed = work["logs"], work["log_blowup"], work["seed"]
@@ -118,7 +139,11 @@ def main():
     ldes = [BaseColumn(ctx, lg + b) for lg in logs]
     layers = [HashLayer(ctx, k) for k in range(L + 1)]
     log("[bench] inputs resident in HBM")
-    if not args.no_autotune:
+    if args.fft_tuning:
+        for m, t in json.loads(args.fft_tuning).items():
+            be.set_fft_tuning(int(m), *[int(x) for x in t])
+        log(f"[bench] FFT pass shapes set from --fft-tuning")
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         be.autotune_fft(logs, b, tree)
         tuned = {m: be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})}
@@ -157,26 +182,43 @@ def main():
     sec = timed(graph.launch, args.steps)
     root = graph.root()
     assert root == root0, "graph replay differs from the eager commit"
-    graph.free()
     log(f"[bench] graph batched: {sec * 1e3 / args.steps:.3f} ms/step")
     ms_per_step = sec * 1e3 / args.steps
     value = n_cells * args.steps / sec
 
-    # (3) plain per-column drop-in: one interpolate and one evaluate call per column (as the
-    # reference's PolyOps calls are made column by column), then one lifted Merkle commit
-    plain_steps = max(1, args.steps // 4)
-    for e, c in zip(evals, coeffs):
-        c.copy_from(e)
-    def plain_step():
-        for c in coeffs:
-            be.interpolate(c, tree)
-        for c, l in zip(coeffs, ldes):
-            be.evaluate(c, c.log_size + b, tree, l)
-        be.commit_lifted(ldes, L, layers=layers, want_root=False)
-    plain_step()
-    assert layers[0].to_cpu()[0].tobytes() == root0, "plain per-column path differs from the batched commit"
-    sec_plain = timed(plain_step, plain_steps)
-    log(f"[bench] plain per-column drop-in: {sec_plain * 1e3 / plain_steps:.3f} ms/step")
+    variants = {"graph_batched_ms_per_step": round(ms_per_step, 4),
+                "eager_batched_ms_per_step": round(sec_eager * 1e3 / args.steps, 4)}
+    if not args.no_plain:
+        # (3) host-inclusive: the columns start in (pageable) host memory and cross PCIe first
+        h2d_steps = max(1, args.steps // 4)
+        def h2d_step():
+            for e, h in zip(evals, host):
+                e.upload(h)
+            graph.launch()
+        sec_h2d = timed(h2d_step, h2d_steps)
+        assert graph.root() == root0
+        variants["h2d_inclusive_ms_per_step"] = round(sec_h2d * 1e3 / h2d_steps, 4)
+        log(f"[bench] host-to-device inclusive: {sec_h2d * 1e3 / h2d_steps:.3f} ms/step")
+    graph.free()
+
+    if not args.no_plain:
+        # (4) plain per-column drop-in: one interpolate and one evaluate call per column (as the
+        # reference's PolyOps calls are made column by column), then one lifted Merkle commit
+        plain_steps = max(1, args.steps // 4)
+        for e, c in zip(evals, coeffs):
+            c.copy_from(e)
+        def plain_step():
+            for c in coeffs:
+                be.interpolate(c, tree)
+            for c, l in zip(coeffs, ldes):
+                be.evaluate(c, c.log_size + b, tree, l)
+            be.commit_lifted(ldes, L, layers=layers, want_root=False)
+        plain_step()
+        assert layers[0].to_cpu()[0].tobytes() == root0, "plain per-column path differs from the batched commit"
+        sec_plain = timed(plain_step, plain_steps)
+        log(f"[bench] plain per-column drop-in: {sec_plain * 1e3 / plain_steps:.3f} ms/step")
+        variants["plain_per_column_ms_per_step"] = round(sec_plain * 1e3 / plain_steps, 4)
+        variants["graph_vs_plain"] = round(sec_plain / plain_steps / (sec / args.steps), 2)
 
     work_per = algorithmic_work(logs, b)
     kernels = {}
@@ -219,12 +261,7 @@ def main():
                    "pipeline": "interpolate + LDE + lifted Blake2s Merkle (sh_commit_trace)"},
         "roofline": roofline,
         "kernels": kernels,
-        "variants": {
-            "graph_batched_ms_per_step": round(ms_per_step, 4),
-            "eager_batched_ms_per_step": round(sec_eager * 1e3 / args.steps, 4),
-            "plain_per_column_ms_per_step": round(sec_plain * 1e3 / plain_steps, 4),
-            "graph_vs_plain": round(sec_plain / plain_steps / (sec / args.steps), 2),
-        },
+        "variants": variants,
         "largest_kernel_class": biggest,
         "fft_tuning": {str(m): be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})},
         "root": root.hex(),

@@ -1,0 +1,79 @@
+/*
+ * stwo_prover.h — host-side protocol drivers above the backend C-ABI
+ * (stwo_hip.h).  The reference implements these generically over its Backend
+ * trait; Rust cannot be built here, so they are restated in C++ and drive the
+ * HIP backend directly.  Each entry point cites the reference function it
+ * restates.  Conventions as in stwo_hip.h.
+ */
+#ifndef STWO_PROVER_H
+#define STWO_PROVER_H
+
+#include <stdint.h>
+
+#include "stwo_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- channel */
+/* Blake2sChannel (crates/stwo/src/core/channel/blake2s.rs:16-117): plain state. */
+typedef struct {
+  uint8_t digest[32];
+  uint32_t n_draws;
+} sh_channel;
+void sh_channel_mix_u32s(sh_channel *ch, const uint32_t *words, uint64_t n);
+void sh_channel_mix_u64(sh_channel *ch, uint64_t value);
+/* mix_felts: n QM31 values, 4 words each. */
+void sh_channel_mix_felts(sh_channel *ch, const uint32_t *felts, uint64_t n);
+/* MerkleChannel::mix_hash (crates/stwo/src/core/vcs_lifted/blake2_merkle.rs:56-58). */
+void sh_channel_mix_root(sh_channel *ch, const uint8_t root[32]);
+void sh_channel_draw_u32s(sh_channel *ch, uint32_t out[8]);
+void sh_channel_draw_secure_felt(sh_channel *ch, uint32_t out[4]);
+/* CirclePoint::get_random_point (crates/stwo/src/core/circle.rs:150-160). */
+void sh_channel_draw_random_point(sh_channel *ch, uint32_t out[8]);
+/* draw_queries (crates/stwo/src/core/queries.rs:11-28): n unsorted positions. */
+void sh_channel_draw_queries(sh_channel *ch, uint32_t log_domain_size, uint32_t n, uint32_t *out);
+
+/* ---------------------------------------------------------------- Merkle decommitment */
+/* MerkleProverLifted::decommit hash witness (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162)
+ * for a lifted tree of height L whose layers are d_layers[0..L] (layer k: 2^k hashes).
+ * positions: sorted, deduplicated leaf indices.  Writes up to cap hashes (32 bytes each) to
+ * witness and their count to n_witness. */
+int sh_merkle_decommit(sh_ctx *ctx, const uint8_t *const *d_layers, uint32_t L, const uint32_t *positions,
+                       uint64_t n_positions, uint8_t *witness, uint64_t cap, uint64_t *n_witness);
+/* Column::at for many (column, row) pairs: out[i] = d_cols[i][rows[i]] (one gather launch). */
+int sh_gather(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *rows, uint64_t n, uint32_t *out);
+
+/* ---------------------------------------------------------------- FRI */
+typedef struct {
+  uint32_t pow_bits, log_blowup_factor, log_last_layer_degree_bound, n_queries, fold_step;
+} sh_fri_params; /* FriParams (crates/stwo/src/core/fri.rs:31-43) */
+
+/* FriProver::commit + FriProver::decommit (crates/stwo/src/prover/fri.rs:113-284) on a QM31
+ * evaluation on CanonicCoset(log_size) (4 device coordinate arrays).  Advances the channel.
+ * Writes the FriProof in CairoSerde felt encoding (crates/cairo-serialize/src/serialize.rs:66-90)
+ * to felts (each felt < 2^64) and the sorted, deduplicated query positions to
+ * query_positions (at most n_queries).  fold_step must be 1 (the production setting; packed
+ * leaves are not implemented). */
+int sh_fri_prove(sh_ctx *ctx, const sh_twiddles *tw, sh_channel *ch, const sh_fri_params *params,
+                 const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
+                 uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);
+
+/* ---------------------------------------------------------------- DEEP quotients */
+/* compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160): columns are the
+ * committed evaluations of every tree, flattened in tree/column order; column i has samples
+ * [sample_offsets[i], sample_offsets[i+1]) with points (8 words) and values (4 words).  Builds
+ * the samples with random powers and periodicity samples
+ * (crates/stwo/src/core/pcs/quotients.rs:274-319), accumulates numerators per column size,
+ * lifts and accumulates per point, and combines into d_out (4 arrays, 2^lifting_log_size). */
+int sh_fri_quotients(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_cols,
+                     const uint32_t *col_log_sizes, uint64_t n_cols, const uint32_t *sample_offsets,
+                     const uint32_t *sample_points, const uint32_t *sample_values,
+                     const uint32_t random_coeff[4], uint32_t lifting_log_size, uint32_t log_blowup,
+                     uint32_t *const d_out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,428 @@
+"""Restatement of the reference's host protocol over the C oracle: lifted-Merkle decommitment and
+verification, FRI prover and verifier, DEEP-quotient driver, FriProof felt encoding.
+
+TEST INFRASTRUCTURE ONLY (the checker for the C++ drivers); never imported by the product path.
+Each function cites the reference function it restates.
+"""
+import numpy as np
+
+import oracle as O
+
+P = O.P
+
+
+# ---------------------------------------------------------------- small QM31 helpers
+def qadd(a, b):
+    return [(int(x) + int(y)) % P for x, y in zip(a, b)]
+
+
+def qsub(a, b):
+    return [(int(x) - int(y)) % P for x, y in zip(a, b)]
+
+
+def qmul(a, b):
+    return [int(v) for v in O.qm31_mul([int(x) for x in a], [int(x) for x in b])]
+
+
+def qinv(a):
+    return [int(v) for v in O.qm31_inv([int(x) for x in a])]
+
+
+def qmulm(a, m):
+    return [int(x) * int(m) % P for x in a]
+
+
+def bitrev(i, n):
+    return int(bin(i)[2:].zfill(n)[::-1], 2) if n else i
+
+
+def coset_at(init, log, i):
+    """Coset(init, log).at(i) (crates/stwo/src/core/circle.rs:357-359)."""
+    return O.index_to_point((init + (1 << (31 - log)) * i) % (1 << 31))
+
+
+def circle_domain_at(half_init, half_log, i):
+    return O.circle_domain_at(half_init, half_log, i)
+
+
+# ---------------------------------------------------------------- Merkle (lifted)
+def merkle_decommit(layers, positions):
+    """Hash witness of MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:
+    118-152); layers root-first as returned by oracle.merkle_commit_lifted."""
+    L = len(layers) - 1
+    q = sorted(set(positions))
+    witness = []
+    for layer in range(L - 1, -1, -1):
+        prev = layers[layer + 1]
+        nxt = []
+        i = 0
+        while i < len(q):
+            first = q[i]
+            if i + 1 < len(q) and q[i + 1] == first ^ 1:
+                i += 2
+            else:
+                witness.append(prev[first ^ 1].tobytes())
+                i += 1
+            nxt.append(first >> 1)
+        q = nxt
+    return witness
+
+
+def merkle_verify(root, height, positions, rows, witness):
+    """MerkleVerifierLifted::verify for columns of one size (crates/stwo/src/core/vcs_lifted/
+    verifier.rs:115-212): rows[k] holds every column's value at positions[k]."""
+    leaves = {}
+    for pos, row in zip(positions, rows):
+        data = b"".join(int(v).to_bytes(4, "little") for v in row)
+        h = O.blake2s(data)
+        assert leaves.setdefault(pos, h) == h
+    prev = sorted(leaves.items())
+    wit = iter(witness)
+    for _ in range(height):
+        cur = []
+        i = 0
+        while i < len(prev):
+            idx0, h0 = prev[i]
+            if i + 1 < len(prev) and prev[i + 1][0] == idx0 ^ 1:
+                children = (h0, prev[i + 1][1])
+                i += 2
+            else:
+                w = next(wit, None)
+                if w is None:
+                    return False
+                children = (h0, w) if idx0 & 1 == 0 else (w, h0)
+                i += 1
+            cur.append((idx0 >> 1, O.blake2s(children[0] + children[1])))
+        prev = cur
+    if next(wit, None) is not None:
+        return False
+    return len(prev) == 1 and prev[0][1] == root
+
+
+# ---------------------------------------------------------------- FRI prover
+def line_interpolate(values_bitrev, init, log):
+    """LineEvaluation::interpolate (crates/stwo/src/prover/line.rs:54-64) + line_ifft
+    (crates/stwo/src/prover/line.rs:88-99): coefficients in bit-reversed order."""
+    v = [list(values_bitrev[bitrev(i, log)]) for i in range(1 << log)]
+    d_init, d_log = init, log
+    while d_log > 0:
+        size = 1 << d_log
+        for c in range(0, len(v), size):
+            for i in range(size // 2):
+                xi = O.m31_inv(int(coset_at(d_init, d_log, i)[0]))
+                a, b = v[c + i], v[c + size // 2 + i]
+                v[c + i], v[c + size // 2 + i] = qadd(a, b), qmulm(qsub(a, b), xi)
+        d_init, d_log = (d_init * 2) % (1 << 31), d_log - 1
+    inv = O.m31_inv(len(v))
+    return [qmulm(x, inv) for x in v]
+
+
+def _decommit_layer(col, layers, queries):
+    """compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438),
+    fold_step 1, then the layer's Merkle decommitment."""
+    positions, witness_evals = [], []
+    i = 0
+    while i < len(queries):
+        j = i
+        while j < len(queries) and queries[j] >> 1 == queries[i] >> 1:
+            j += 1
+        start = (queries[i] >> 1) << 1
+        for p in (start, start + 1):
+            positions.append(p)
+            if p not in queries[i:j]:
+                witness_evals.append([int(col[c][p]) for c in range(4)])
+        i = j
+    return {"fri_witness": witness_evals, "hash_witness": merkle_decommit(layers, positions),
+            "commitment": layers[0][0].tobytes()}
+
+
+def fri_prove(column, log_size, params, ch):
+    """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284), fold_step 1.
+    column: (4, 2^log_size) QM31 evaluation on CanonicCoset(log_size).  Returns (proof, queries)."""
+    assert params["fold_step"] == 1
+    last_log = params["log_last_layer_degree_bound"] + params["log_blowup_factor"]
+    first_layers = O.merkle_commit_lifted(list(column), log_size)
+    ch.mix_hash(first_layers[0][0].tobytes())
+    alpha = ch.draw_secure_felt()
+    cur = O.fold_circle_into_line(column, log_size, alpha)
+    line_log, line_init = log_size - 1, O.canonic_half_initial(log_size)
+    inner = []
+    while line_log > last_log:
+        layers = O.merkle_commit_lifted(list(cur), line_log)
+        ch.mix_hash(layers[0][0].tobytes())
+        alpha = ch.draw_secure_felt()
+        inner.append((cur, layers))
+        cur = O.fold_line(cur, line_init, line_log, alpha)
+        line_log, line_init = line_log - 1, (line_init * 2) % (1 << 31)
+    coeffs = line_interpolate([cur[:, i] for i in range(cur.shape[1])], line_init, line_log)
+    ordered = [coeffs[bitrev(i, line_log)] for i in range(len(coeffs))]
+    bound = 1 << params["log_last_layer_degree_bound"]
+    assert all(not any(c) for c in ordered[bound:]), "invalid degree"
+    kept = ordered[:bound]
+    lb = bound.bit_length() - 1
+    last_poly = [kept[bitrev(i, lb)] for i in range(bound)]
+    ch.mix_felts(np.array(last_poly, np.uint32).reshape(-1))
+    nonce = O.grind(ch.digest, params["pow_bits"])
+    ch.mix_u64(nonce)
+    raw = []
+    while len(raw) < params["n_queries"]:
+        for w in ch.draw_u32s():
+            if len(raw) < params["n_queries"]:
+                raw.append(int(w) & ((1 << log_size) - 1))
+    queries = sorted(set(raw))
+    first = _decommit_layer(column, first_layers, queries)
+    inner_proofs = []
+    lq = queries
+    for col, layers in inner:
+        lq = sorted(set(q >> 1 for q in lq))
+        inner_proofs.append(_decommit_layer(col, layers, lq))
+    proof = {"first_layer": first, "inner_layers": inner_proofs, "last_layer_poly": last_poly,
+             "log_last_layer_degree_bound": params["log_last_layer_degree_bound"], "proof_of_work": nonce}
+    return proof, queries
+
+
+def serialize_fri_proof(proof):
+    """CairoSerde felt encoding (crates/cairo-serialize/src/serialize.rs:21-90, 167-211)."""
+    out = []
+
+    def layer(lp):
+        out.append(len(lp["fri_witness"]))
+        for q in lp["fri_witness"]:
+            out.extend(int(x) for x in q)
+        out.append(len(lp["hash_witness"]))
+        for h in lp["hash_witness"]:
+            out.extend(int.from_bytes(h[4 * i:4 * i + 4], "little") for i in range(8))
+        out.extend(int.from_bytes(lp["commitment"][4 * i:4 * i + 4], "little") for i in range(8))
+
+    layer(proof["first_layer"])
+    out.append(len(proof["inner_layers"]))
+    for lp in proof["inner_layers"]:
+        layer(lp)
+    out.append(len(proof["last_layer_poly"]))
+    for q in proof["last_layer_poly"]:
+        out.extend(int(x) for x in q)
+    out.append(proof["log_last_layer_degree_bound"])
+    out.append(proof["proof_of_work"])
+    return out
+
+
+def deserialize_fri_proof(felts):
+    it = iter(felts)
+
+    def hash_():
+        return b"".join(int(next(it)).to_bytes(4, "little") for _ in range(8))
+
+    def layer():
+        fw = [[next(it) for _ in range(4)] for _ in range(next(it))]
+        hw = [hash_() for _ in range(next(it))]
+        return {"fri_witness": fw, "hash_witness": hw, "commitment": hash_()}
+
+    first = layer()
+    inner = [layer() for _ in range(next(it))]
+    lp = [[next(it) for _ in range(4)] for _ in range(next(it))]
+    lb = next(it)
+    pow_ = next(it)
+    assert next(it, None) is None, "trailing felts"
+    return {"first_layer": first, "inner_layers": inner, "last_layer_poly": lp,
+            "log_last_layer_degree_bound": lb, "proof_of_work": pow_}
+
+
+# ---------------------------------------------------------------- FRI verifier
+def _line_eval_at_point(coeffs_bitrev, x):
+    """LinePoly::eval_at_point (crates/stwo/src/core/poly/line.rs:131-139)."""
+    doublings = []
+    for _ in range(len(coeffs_bitrev).bit_length() - 1):
+        doublings.append(x)
+        s = qmul(x, x)
+        x = qsub(qadd(s, s), [1, 0, 0, 0])
+    vals = [list(c) for c in coeffs_bitrev]
+    for f in reversed(doublings):
+        vals = [qadd(vals[2 * t], qmul(vals[2 * t + 1], f)) for t in range(len(vals) // 2)]
+    return vals[0]
+
+
+def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn):
+    """FriVerifier::commit / verify_proof_of_work / sample_query_positions / decommit
+    (crates/stwo/src/core/fri.rs:156-378), fold_step 1.  first_layer_query_evals_fn(queries)
+    returns the column's QM31 values at the sorted queries.  Returns the queries, raises on
+    failure."""
+    b = params["log_blowup_factor"]
+    ch.mix_hash(proof["first_layer"]["commitment"])
+    col_log = log_degree_bound + b
+    alpha0 = ch.draw_secure_felt()
+    layer_bound = log_degree_bound - 1
+    line_init, line_log = O.canonic_half_initial(layer_bound + b + 1), layer_bound + b
+    inner = []
+    for k, lp in enumerate(proof["inner_layers"]):
+        ch.mix_hash(lp["commitment"])
+        if k == len(proof["inner_layers"]) - 1:
+            assert layer_bound - params["log_last_layer_degree_bound"] == 1, "InvalidNumFriLayers"
+        inner.append((line_init, line_log, ch.draw_secure_felt(), lp))
+        layer_bound -= 1
+        line_init, line_log = (line_init * 2) % (1 << 31), line_log - 1
+    assert layer_bound == params["log_last_layer_degree_bound"], "InvalidNumFriLayers"
+    assert len(proof["last_layer_poly"]) <= 1 << params["log_last_layer_degree_bound"]
+    ch.mix_felts(np.array(proof["last_layer_poly"], np.uint32).reshape(-1))
+    assert O.verify_pow_nonce(ch.digest, params["pow_bits"], proof["proof_of_work"]), "ProofOfWork"
+    ch.mix_u64(proof["proof_of_work"])
+    raw = []
+    while len(raw) < params["n_queries"]:
+        for w in ch.draw_u32s():
+            if len(raw) < params["n_queries"]:
+                raw.append(int(w) & ((1 << col_log) - 1))
+    queries = sorted(set(raw))
+
+    def rebuild(qs, evals, lp, log):
+        """compute_decommitment_positions_and_rebuild_evals (fri.rs:629-671)."""
+        wit = iter(lp["fri_witness"])
+        ev = iter(evals)
+        positions, subsets = [], []
+        i = 0
+        while i < len(qs):
+            j = i
+            while j < len(qs) and qs[j] >> 1 == qs[i] >> 1:
+                j += 1
+            start = (qs[i] >> 1) << 1
+            sub = []
+            for p in (start, start + 1):
+                positions.append(p)
+                sub.append(list(next(ev)) if p in qs[i:j] else list(next(wit)))
+            subsets.append((start, sub))
+            i = j
+        assert next(wit, None) is None, "EvaluationsInvalid"
+        rows = [s for _, sub in subsets for s in sub]
+        assert merkle_verify(lp["commitment"], log, positions, rows, lp["hash_witness"]), "CommitmentInvalid"
+        return subsets
+
+    # first layer: fold_circle on each subset (fri.rs:727-756)
+    subsets = rebuild(queries, first_layer_query_evals_fn(queries), proof["first_layer"], col_log)
+    half_init = O.canonic_half_initial(col_log)
+    evals = []
+    for start, sub in subsets:
+        p = circle_domain_at(half_init, col_log - 1, bitrev(start, col_log))
+        yi = O.m31_inv(int(p[1]))
+        f0, f1 = qadd(sub[0], sub[1]), qmulm(qsub(sub[0], sub[1]), yi)
+        evals.append(qadd(qmul(alpha0, f1), f0))
+    qs = sorted(set(q >> 1 for q in queries))
+    # inner layers: verify_and_fold (fri.rs:561-621), fold_coset of size 2 (fri.rs:785-804)
+    for li, llog, alpha, lp in inner:
+        subsets = rebuild(qs, evals, lp, llog)
+        evals = []
+        for start, sub in subsets:
+            x = coset_at(li, llog, bitrev(start, llog))[0]
+            xi = O.m31_inv(int(x))
+            f0, f1 = qadd(sub[0], sub[1]), qmulm(qsub(sub[0], sub[1]), xi)
+            evals.append(qadd(f0, qmul(alpha, f1)))
+        qs = sorted(set(q >> 1 for q in qs))
+    # last layer (fri.rs:347-362)
+    for q, e in zip(qs, evals):
+        x = coset_at(line_init, line_log, bitrev(q, line_log))[0]
+        assert e == _line_eval_at_point(proof["last_layer_poly"], [int(x), 0, 0, 0]), "LastLayerEvaluationsInvalid"
+    return queries
+
+
+# ---------------------------------------------------------------- DEEP quotients
+def _build_samples(column_logs, samples, random_coeff, L):
+    """build_samples_with_randomness_and_periodicity (crates/stwo/src/core/pcs/quotients.rs:
+    274-319): per column, [(point[8], value[4], random power)]; a column with two samples gets a
+    periodicity sample first, at the second point shifted by CanonicCoset(L).step() doubled
+    log_size times."""
+    pw = [1, 0, 0, 0]
+    swr = []
+    for log, smp in zip(column_logs, samples):
+        out = []
+        if len(smp) == 2:
+            g = O.index_to_point(((1 << (31 - L)) << log) % (1 << 31))
+            pt = [int(x) for x in smp[1][0]]
+            x, y = pt[:4], pt[4:]
+            gx, gy = [int(g[0]), 0, 0, 0], [int(g[1]), 0, 0, 0]
+            sx = qsub(qmul(x, gx), qmul(y, gy))
+            sy = qadd(qmul(x, gy), qmul(y, gx))
+            out.append((sx + sy, [int(v) for v in smp[1][1]], pw))
+            pw = qmul(pw, random_coeff)
+        for pt, v in smp:
+            out.append(([int(x) for x in pt], [int(x) for x in v], pw))
+            pw = qmul(pw, random_coeff)
+        swr.append(out)
+    return swr
+
+
+def _batches(swr):
+    """ColumnSampleBatch::new_vec (crates/stwo/src/core/pcs/quotients.rs:78-95): group by point in
+    order of first occurrence; terms (column index, value, random power)."""
+    batches = {}
+    for ci, smp in enumerate(swr):
+        for pt, v, r in smp:
+            batches.setdefault(tuple(pt), []).append((ci, v, r))
+    return batches
+
+
+def _cmul(a, b):
+    return [(a[0] * b[0] - a[1] * b[1]) % P, (a[0] * b[1] + a[1] * b[0]) % P]
+
+
+def _cinv(a):
+    n = O.m31_inv((a[0] * a[0] + a[1] * a[1]) % P)
+    return [a[0] * n % P, (P - a[1]) * n % P]
+
+
+def lifted_value(col, L, pos):
+    """A column of 2^n values read at a position of the 2^L lifted domain
+    (crates/stwo/src/prover/vcs_lifted/prover.rs:108-116)."""
+    n = len(col).bit_length() - 1
+    return int(col[((pos >> (L - n + 1)) << 1) + (pos & 1)])
+
+
+def fri_answers(column_logs, samples, random_coeff, positions, queried_values, L):
+    """fri_answers + accumulate_row_quotients + denominator_inverses (crates/stwo/src/core/pcs/
+    quotients.rs:115-258): the DEEP quotient at each query position of the lifted domain, from
+    the columns' lifted values there (queried_values[i][k]: column i at positions[k])."""
+    batches = _batches(_build_samples(column_logs, samples, random_coeff, L))
+    consts = {pt: [O.line_coeffs(list(pt), v, r) for _, v, r in terms] for pt, terms in batches.items()}
+    half = O.canonic_half_initial(L)
+    res = []
+    for k, pos in enumerate(positions):
+        px, py = (int(c) for c in circle_domain_at(half, L - 1, bitrev(pos, L)))
+        acc = [0, 0, 0, 0]
+        for pt, terms in batches.items():
+            num = [0, 0, 0, 0]
+            for (ci, _, _), co in zip(terms, consts[pt]):
+                a, bb, c = [int(x) for x in co[:4]], [int(x) for x in co[4:8]], [int(x) for x in co[8:]]
+                val = qmulm(c, queried_values[ci][k])
+                num = qadd(num, qsub(val, qadd(qmulm(a, py), bb)))
+            # (Re(p.x) - px) * Im(p.y) - (Re(p.y) - py) * Im(p.x), an element of CM31
+            t0 = _cmul([(pt[0] - px) % P, pt[1]], [pt[6], pt[7]])
+            t1 = _cmul([(pt[4] - py) % P, pt[5]], [pt[2], pt[3]])
+            den = _cinv([(t0[0] - t1[0]) % P, (t0[1] - t1[1]) % P])
+            acc = qadd(acc, _cmul(num[:2], den) + _cmul(num[2:], den))
+        res.append(acc)
+    return res
+
+
+def fri_quotients(columns, samples, random_coeff, L, b, tw, itw):
+    """compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160) with
+    build_samples_with_randomness_and_periodicity (crates/stwo/src/core/pcs/quotients.rs:274-319).
+    columns: evaluations (uint32 arrays); samples[i]: list of (point[8], value[4])."""
+    swr = _build_samples([len(c).bit_length() - 1 for c in columns], samples, random_coeff, L)
+    accs = []
+    for log in sorted(set(len(c).bit_length() - 1 for c in columns)):
+        idx = [i for i, c in enumerate(columns) if len(c).bit_length() - 1 == log]
+        for pt, terms in _batches([swr[i] for i in idx]).items():
+            coeffs = np.stack([O.line_coeffs(list(pt), v, r) for _, v, r in terms])
+            part, fl = O.accumulate_numerators([columns[i] for i in idx], [ci for ci, _, _ in terms], coeffs, b)
+            accs.append((list(pt), part, fl))
+    accs.sort(key=lambda a: tuple(a[0]))
+    per_point = []
+    i = 0
+    while i < len(accs):
+        j = i
+        while j < len(accs) and accs[j][0] == accs[i][0]:
+            j += 1
+        fl = [0, 0, 0, 0]
+        for a in accs[i:j]:
+            fl = qadd(fl, a[2])
+        part = O.lift_and_accumulate([a[1] for a in accs[i:j]])
+        per_point.append((accs[i][0], part, fl))
+        i = j
+    return O.compute_quotients_and_combine([p[1] for p in per_point], [p[0] for p in per_point],
+                                           [p[2] for p in per_point], L, b, tw, itw)

@@ -65,6 +65,19 @@ SIGNATURES = {
                                              ctypes.POINTER(c_vp)]),
     "sh_graph_launch": (c_int, [c_vp, c_vp]),
     "sh_graph_destroy": (c_int, [c_vp, c_vp]),
+    # stwo_prover.h
+    "sh_channel_mix_u32s": (None, [c_vp, c_vp, c_u64]),
+    "sh_channel_mix_u64": (None, [c_vp, c_u64]),
+    "sh_channel_mix_felts": (None, [c_vp, c_vp, c_u64]),
+    "sh_channel_mix_root": (None, [c_vp, c_vp]),
+    "sh_channel_draw_u32s": (None, [c_vp, c_vp]),
+    "sh_channel_draw_secure_felt": (None, [c_vp, c_vp]),
+    "sh_channel_draw_random_point": (None, [c_vp, c_vp]),
+    "sh_channel_draw_queries": (None, [c_vp, c_u32, c_u32, c_vp]),
+    "sh_merkle_decommit": (c_int, [c_vp, c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u64p]),
+    "sh_gather": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
+    "sh_fri_prove": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),
+    "sh_fri_quotients": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
 }
 
 _lib = None

@@ -463,3 +463,107 @@ class _Ops:
 for _name, _fn in list(vars(_Ops).items()):
     if not _name.startswith("__"):
         setattr(HipBackend, _name, _fn)
+
+
+
+class ChannelState(ctypes.Structure):
+    """sh_channel: Blake2sChannel state (crates/stwo/src/core/channel/blake2s.rs:16-20)."""
+    _fields_ = [("digest", ctypes.c_uint8 * 32), ("n_draws", ctypes.c_uint32)]
+
+
+class Channel:
+    """Blake2sChannel over the C++ host implementation."""
+
+    def __init__(self, lib=None):
+        self.lib = lib or load()
+        self.s = ChannelState()
+
+    @property
+    def digest(self):
+        return bytes(self.s.digest)
+
+    def mix_u32s(self, words):
+        self.lib.sh_channel_mix_u32s(ctypes.byref(self.s), _u32s(list(words)), len(words))
+
+    def mix_u64(self, v):
+        self.lib.sh_channel_mix_u64(ctypes.byref(self.s), v)
+
+    def mix_felts(self, felts):
+        w = [int(x) for f in felts for x in f]
+        self.lib.sh_channel_mix_felts(ctypes.byref(self.s), _u32s(w), len(w) // 4)
+
+    def mix_root(self, root: bytes):
+        self.lib.sh_channel_mix_root(ctypes.byref(self.s), (ctypes.c_uint8 * 32)(*root))
+
+    def draw_secure_felt(self):
+        out = (ctypes.c_uint32 * 4)()
+        self.lib.sh_channel_draw_secure_felt(ctypes.byref(self.s), out)
+        return np.array(out, np.uint32)
+
+    def draw_random_point(self):
+        out = (ctypes.c_uint32 * 8)()
+        self.lib.sh_channel_draw_random_point(ctypes.byref(self.s), out)
+        return np.array(out, np.uint32)
+
+    def draw_queries(self, log_domain_size, n):
+        out = (ctypes.c_uint32 * max(1, n))()
+        self.lib.sh_channel_draw_queries(ctypes.byref(self.s), log_domain_size, n, out)
+        return list(out)[:n]
+
+
+class FriParams(ctypes.Structure):
+    """FriParams (crates/stwo/src/core/fri.rs:31-43)."""
+    _fields_ = [("pow_bits", ctypes.c_uint32), ("log_blowup_factor", ctypes.c_uint32),
+                ("log_last_layer_degree_bound", ctypes.c_uint32), ("n_queries", ctypes.c_uint32),
+                ("fold_step", ctypes.c_uint32)]
+
+
+def merkle_decommit(backend, layers, positions):
+    """Hash witness of MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:
+    97-162); layers root-first as returned by commit_lifted; positions sorted and deduplicated."""
+    L = len(layers) - 1
+    cap = max(1, len(positions) * max(1, L))
+    wit = (ctypes.c_uint8 * (32 * cap))()
+    nw = ctypes.c_uint64()
+    backend._chk(backend.lib.sh_merkle_decommit(backend.h, _ptrs(layers), L, _u32s(positions), len(positions),
+                                                wit, cap, ctypes.byref(nw)), "merkle_decommit")
+    raw = bytes(wit)
+    return [raw[32 * i:32 * i + 32] for i in range(nw.value)]
+
+
+def gather(backend, cols, rows):
+    """out[i] = cols[i].at(rows[i]) in one launch."""
+    out = (ctypes.c_uint32 * max(1, len(rows)))()
+    backend._chk(backend.lib.sh_gather(backend.h, _ptrs(cols), _u32s(rows), len(rows), out), "gather")
+    return [int(x) for x in out[:len(rows)]]
+
+
+def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22):
+    """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284) on a device QM31
+    column; returns (proof felts, sorted query positions)."""
+    felts = (ctypes.c_uint64 * cap)()
+    nf = ctypes.c_uint64()
+    qp = (ctypes.c_uint32 * max(1, params.n_queries))()
+    nq = ctypes.c_uint64()
+    backend._chk(backend.lib.sh_fri_prove(backend.h, twiddles.h, ctypes.byref(channel.s), ctypes.byref(params),
+                                          _ptrs(column.columns), column.log_size, felts, cap, ctypes.byref(nf),
+                                          qp, ctypes.byref(nq)), "fri_prove")
+    return [int(x) for x in felts[:nf.value]], list(qp)[:nq.value]
+
+
+def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log_size, log_blowup, out=None):
+    """compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160).
+    columns: committed evaluations (BaseColumn) in tree/column order; samples[i]: list of
+    (point[8], value[4]) for column i."""
+    out = out or SecureColumnByCoords(backend.ctx, lifting_log_size)
+    offs, pts, vals = [0], [], []
+    for smp in samples:
+        for pt, v in smp:
+            pts.extend(int(x) for x in pt)
+            vals.extend(int(x) for x in v)
+        offs.append(offs[-1] + len(smp))
+    backend._chk(backend.lib.sh_fri_quotients(
+        backend.h, twiddles.h, _ptrs(columns), _u32s([c.log_size for c in columns]), len(columns), _u32s(offs),
+        _u32s(pts), _u32s(vals), _u32s(list(random_coeff)), lifting_log_size, log_blowup, _ptrs(out.columns)),
+        "fri_quotients")
+    return out
