@@ -55,14 +55,11 @@ std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
 __device__ __forceinline__ uint32_t sw(uint32_t a) { return a ^ (((a >> 6) & 15u) << 2); }
 
 __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t h) {
-  // crates/stwo/src/prover/backend/cpu/circle.rs:306-323: [x, y] -> [y, -y, -x, x]
-  uint32_t x = l1[(h >> 2) * 2], y = l1[(h >> 2) * 2 + 1];
-  switch (h & 3) {
-    case 0: return y;
-    case 1: return sh::mneg(y);
-    case 2: return sh::mneg(x);
-    default: return x;
-  }
+  // crates/stwo/src/prover/backend/cpu/circle.rs:306-323: [x, y] -> [y, -y, -x, x];
+  // branch-free (lanes of one wave take different cases), one 64-bit load
+  const uint2 xy = ((const uint2*)l1)[h >> 2];
+  const uint32_t v = (h & 2) ? xy.x : xy.y;
+  return ((h ^ (h >> 1)) & 1) ? sh::mneg(v) : v;
 }
 
 // Butterflies of one register stage: R consecutive local layers [s, s+R) of
@@ -72,7 +69,7 @@ __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t 
 // LDS access per butterfly for R = 4 instead of four.  At layer i the twiddle
 // index h = pos >> (i+1) is shared by the 2^b pairs below the pair bit b, so a
 // stage loads 2^R - 1 twiddles for R * 2^(R-1) butterflies.
-template <bool INVERSE, int R>
+template <bool INVERSE, bool CIRCLE, int R>
 __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl, uint32_t s, uint32_t base,
                                           uint32_t lo, const uint32_t* __restrict__ tw, uint32_t tw_end,
                                           uint32_t Pl) {
@@ -104,16 +101,23 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
 #pragma unroll
       for (int e = 0; e < (1 << R); e++) v[e] = lds[lane_addr ^ sw((uint32_t)e << (s + tl))];
     }
+    // position of the lane's first word; the pair (eh, el) of layer i = lo+s+b
+    // sits at pb + eh * 2^(i+1) + el * 2^(lo+s), since jf is zero in bits
+    // [s, s+R), so its twiddle index is (pb >> (i+1)) + eh: one address per layer
+    const uint32_t pb = base + (jf << lo) + t;
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const int b = INVERSE ? k : R - 1 - k;
       const uint32_t i = lo + s + b;
-      const uint32_t* tab = tw + tw_end - (1u << (Pl - (i >= 1 ? i : 1)));
+      // CIRCLE: this stage holds layer 0 (lo = s = 0), the circle layer, read
+      // from the pairs of line layer 1
+      const bool circ = CIRCLE && b == 0;
+      const uint32_t* tab = tw + tw_end - (1u << (Pl - (circ ? 1 : i)));
+      const uint32_t hb = pb >> (i + 1);
+      const uint32_t* tp = tab + hb;  // twiddle eh at a constant offset
 #pragma unroll
       for (int eh = 0; eh < (1 << (R - 1 - b)); eh++) {
-        const uint32_t pos0 = base + ((jf | ((uint32_t)(eh << (b + 1)) << s)) << lo) + t;
-        const uint32_t h = pos0 >> (i + 1);
-        const uint32_t twv = i >= 1 ? tab[h] : circle_twiddle(tab, h);
+        const uint32_t twv = circ ? circle_twiddle(tab, hb + eh) : tp[eh];
 #pragma unroll
         for (int el = 0; el < (1 << b); el++) {
           const int a0 = (eh << (b + 1)) | el, a1 = a0 | (1 << b);
@@ -193,12 +197,22 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
     const uint32_t si = INVERSE ? st : n_stages - 1 - st;
     const uint32_t s = si * st_base + (si < st_extra ? si : st_extra);
     const uint32_t r = st_base + (si < st_extra ? 1 : 0);
-    switch (r) {
-      case 5: fft_stage<INVERSE, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      case 4: fft_stage<INVERSE, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      case 3: fft_stage<INVERSE, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      case 2: fft_stage<INVERSE, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      default: fft_stage<INVERSE, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+    if (lo + s == 0) {
+      switch (r) {
+        case 5: fft_stage<INVERSE, true, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 4: fft_stage<INVERSE, true, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 3: fft_stage<INVERSE, true, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 2: fft_stage<INVERSE, true, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        default: fft_stage<INVERSE, true, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+      }
+    } else {
+      switch (r) {
+        case 5: fft_stage<INVERSE, false, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 4: fft_stage<INVERSE, false, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 3: fft_stage<INVERSE, false, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        case 2: fft_stage<INVERSE, false, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+        default: fft_stage<INVERSE, false, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
+      }
     }
     __syncthreads();
   }

@@ -19,7 +19,12 @@ SH_HD uint32_t reduce_once(uint32_t v) {
   return w < v ? w : v;
 }
 SH_HD uint32_t madd(uint32_t a, uint32_t b) { return reduce_once(a + b); }
-SH_HD uint32_t msub(uint32_t a, uint32_t b) { return reduce_once(a + P - b); }
+// a - b wraps below zero to 2^32 - (b - a), where adding P lands in [0, P):
+// the smaller of d and d + P is the canonical difference (3 ops)
+SH_HD uint32_t msub(uint32_t a, uint32_t b) {
+  uint32_t d = a - b, e = d + P;
+  return e < d ? e : d;
+}
 SH_HD uint32_t mneg(uint32_t a) { return reduce_once(P - a); }
 // a*b mod P for a, b < 2^31: the 62-bit product splits as hi*2^31 + lo and
 // 2^31 = 1 (mod P), so the residue is hi + lo < 2P.
