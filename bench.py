@@ -67,6 +67,23 @@ def algorithmic_work(logs, log_blowup):
     }
 
 
+def profiled_traffic(kernel, tuning):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary of the
+    uneven commit (two rocprofv3 --pmc runs, calibrated by a copy of known size), and whether
+    those runs used the same FFT pass shapes as this run."""
+    import re
+    path = os.path.join(ROOT, "profiles", "commit_uneven512_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    text = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch %s (\d+)" % re.escape(kernel), text)
+    t = re.search(r"--fft-tuning '([^']+)'", text)
+    if not m:
+        return None, None
+    same = t is not None and json.loads(t.group(1)) == {k: list(v) for k, v in tuning.items()}
+    return int(m.group(1)), same
+
+
 def cpu_baseline(work, expected_root):
     import oracle as O
     from proving_amd.workloads import splitmix64_columns
@@ -239,9 +256,15 @@ def main():
     bytes_per_launch = work_per[dom]["bytes"] / n_launch
     avg_s = prof[dom]["ms"] / prof[dom]["n"] / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
+    tuning_now = {str(m): list(be.fft_tuning(m)) for m in sorted(set(logs) | {lg + b for lg in logs})}
+    traffic, same_shapes = profiled_traffic(dom, tuning_now) if args.workload == "uneven" else (None, None)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+    if traffic is not None:
+        roofline["traffic_source"] = ("profiles/commit_uneven512_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + "
+                                      "WRITE_SIZE x 1.0 per dispatch)")
+        roofline["traffic_pass_shapes_match"] = same_shapes
     biggest = max(prof, key=lambda k: prof[k]["ms"])
     out = {
         "metric": "committed trace cells/sec at log-20 LDE",

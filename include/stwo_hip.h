@@ -115,6 +115,13 @@ int sh_evaluate_polynomials(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *
                             const uint32_t *coeff_logs, uint64_t n, uint32_t log_blowup,
                             uint32_t *const *d_outs);
 
+/* PolyOps::split_at_mid (crates/stwo/src/prover/poly/circle/ops.rs:145-165;
+ * CpuBackend crates/stwo/src/prover/backend/cpu/circle.rs:266-271): the
+ * 2^log_size coefficients split into halves, p = p_left + pi^(log_size-2)(x) * p_right;
+ * d_left and d_right receive 2^(log_size-1) words each.  d_left may equal d_coeffs. */
+int sh_split_at_mid(sh_ctx *ctx, const uint32_t *d_coeffs, uint32_t log_size, uint32_t *d_left,
+                    uint32_t *d_right);
+
 /* ---------------------------------------------------------------- Merkle */
 /* MerkleOpsLifted::build_leaves (crates/stwo/src/prover/vcs_lifted/ops.rs:14-15;
  * SIMD crates/stwo/src/prover/backend/simd/blake2s_lifted.rs:46-229).  Columns

@@ -352,6 +352,14 @@ def _u32arr(vals):
 class _Ops:
     """Trait methods beyond PolyOps / Merkle (bound onto HipBackend below)."""
 
+    # ---- PolyOps::split_at_mid (crates/stwo/src/prover/poly/circle/ops.rs:145-165) ------------
+    def split_at_mid(self, poly):
+        """Consumes poly (its buffer becomes the left half); returns (left, right)."""
+        right = BaseColumn(self.ctx, poly.log_size - 1)
+        self._chk(self.lib.sh_split_at_mid(self.h, poly.ptr, poly.log_size, poly.ptr, right.ptr), "split_at_mid")
+        poly.log_size -= 1
+        return poly, right
+
     # ---- PolyOps: OOD sampling (crates/stwo/src/prover/poly/circle/ops.rs:44-80) ---------------
     def eval_at_point(self, coeffs, point):
         out = (ctypes.c_uint32 * 4)()

@@ -569,3 +569,23 @@ extern "C" int sh_autotune_fft(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   hipEventDestroy(e1);
   return rc;
 }
+
+// PolyOps::split_at_mid (crates/stwo/src/prover/poly/circle/ops.rs:145-165;
+// CpuBackend crates/stwo/src/prover/backend/cpu/circle.rs:266-271): with the
+// coefficients in natural FFT-basis order the split is the two halves of the
+// vector (p = p_left + pi^(L-2)(x) * p_right).  d_left may equal d_coeffs (the
+// left half then stays in place); neither output may overlap the right half.
+extern "C" int sh_split_at_mid(sh_ctx* ctx, const uint32_t* d_coeffs, uint32_t log_size, uint32_t* d_left,
+                               uint32_t* d_right) {
+  if (!ctx || !d_coeffs || !d_left || !d_right) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, log_size >= 1 && log_size < 32, "split_at_mid needs at least 2 coefficients (log %u)", log_size);
+  const uint64_t half = 1ull << (log_size - 1);
+  const uint32_t* right = d_coeffs + half;
+  SH_REQUIRE(ctx, !(d_right < right + half && right < d_right + half) && !(d_left < right + half && right < d_left + half),
+             "split_at_mid outputs overlap the right half of the input");
+  SpanGuard span(ctx, "split_at_mid");
+  if (d_left != d_coeffs)
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_left, d_coeffs, 4 * half, hipMemcpyDeviceToDevice, ctx->stream));
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_right, right, 4 * half, hipMemcpyDeviceToDevice, ctx->stream));
+  return SH_OK;
+}

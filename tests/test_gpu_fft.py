@@ -111,3 +111,22 @@ def test_domain_not_in_tower_rejected(ctx, backend):
     col = BaseColumn(ctx, 10)
     with pytest.raises(HipError, match="not a doubling"):
         backend.interpolate(col, tree)
+
+
+@pytest.mark.parametrize("log", [1, 2, 5, 12, 17])
+def test_split_at_mid(ctx, backend, log):
+    """PolyOps::split_at_mid: halves of the coefficient vector, left in place."""
+    rng = np.random.default_rng(40 + log)
+    coeffs = rand_col(rng, log)
+    poly = BaseColumn.from_cpu(ctx, coeffs)
+    left, right = backend.split_at_mid(poly)
+    assert left.log_size == right.log_size == log - 1
+    np.testing.assert_array_equal(left.to_cpu(), coeffs[: 1 << (log - 1)])
+    np.testing.assert_array_equal(right.to_cpu(), coeffs[1 << (log - 1):])
+
+
+def test_split_at_mid_rejects_overlap(ctx, backend):
+    from proving_amd import HipError
+    col = BaseColumn(ctx, 6)
+    with pytest.raises(HipError, match="overlap"):
+        ctx.check(backend.lib.sh_split_at_mid(backend.h, col.ptr, 6, col.ptr, col.ptr + 4 * 40), "split_at_mid")

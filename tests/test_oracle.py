@@ -398,3 +398,35 @@ def test_barycentric_equals_eval_at_point():
         pt = ch.random_point()
         w = O.barycentric_weights(log, pt)
         assert list(O.barycentric_eval(ev, w)) == list(O.eval_at_point(coeffs, pt))
+
+
+def _pi_q(x):
+    """pi(x) = 2x^2 - 1 over QM31 (crates/stwo/src/core/circle.rs:31-36, double_x)."""
+    sq = [int(v) for v in O.qm31_mul(x, x)]
+    return [(2 * sq[0] - 1) % P] + [2 * v % P for v in sq[1:]]
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 6, 10])
+def test_split_at_mid_identity(L):
+    """test_secure_circle_poly_split_at_mid (crates/stwo/src/prover/poly/circle/secure_poly.rs:
+    152-170): p(z) = p_left(z) + pi^(L-2)(z.x) * p_right(z) at a random QM31 point, with the
+    halves of the coefficient vector (L = 1: the factor is y, the basis element of bit 0)."""
+    rng = np.random.default_rng(L)
+    coeffs = rng.integers(0, P, 1 << L, dtype=np.uint32)
+    ch = O.Channel()
+    ch.mix_u64(L)
+    z = [int(v) for v in ch.random_point()]
+    left, right = coeffs[: 1 << (L - 1)], coeffs[1 << (L - 1):]
+    if L == 1:
+        f = z[4:]
+    else:
+        f = z[:4]
+        for _ in range(L - 2):
+            f = _pi_q(f)
+    lhs = [int(v) for v in O.eval_at_point(coeffs, z)]
+    rv = [int(v) for v in O.eval_at_point(right, z)]
+    lv = [int(v) for v in O.eval_at_point(left, z)]
+    fr = [int(v) for v in O.qm31_mul(f, rv)]
+    assert lhs == [(a + b) % P for a, b in zip(lv, fr)]
+    # a dropped factor fails: p(z) != p_left(z) + p_right(z) in general
+    assert lhs != [(a + b) % P for a, b in zip(lv, rv)]
