@@ -240,6 +240,19 @@ int sh_accumulate(sh_ctx *ctx, uint32_t *const *d_col, const uint32_t *const *d_
  * least pow_bits trailing zero bits. */
 int sh_grind(sh_ctx *ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t *nonce);
 
+/* ---------------------------------------------------------------- composition */
+/* ComponentProver::evaluate_constraint_quotients_on_domain
+ * (crates/constraint-framework/src/prover/component_prover.rs:118-193) for the wide-Fibonacci
+ * AIR (WideFibonacciEval, crates/examples/src/wide_fibonacci/mod.rs:77-98): n_cols trace
+ * columns d_cols[i] (committed evaluations, at least 2^eval_log words each), constraints
+ * c_{j+2} - (c_j^2 + c_{j+1}^2) for j = 0..n_cols-3 weighted by powers[4j..4j+4] (the
+ * accumulator's random-coefficient powers in constraint order), divided by the trace coset's
+ * vanishing polynomial, added into d_accum (4 arrays of 2^eval_log) on the evaluation domain
+ * CanonicCoset(eval_log + log_expansion).circle_domain().split(log_expansion).0. */
+int sh_wide_fibonacci_composition(sh_ctx *ctx, const uint32_t *const *d_cols, uint64_t n_cols,
+                                  uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
+                                  const uint32_t *powers, uint32_t *const d_accum[4]);
+
 /* ---------------------------------------------------------------- graph capture */
 /* hipGraph of sh_commit_trace with these exact arguments (no counterpart in the
  * reference): one graph launch replays interpolate + LDE + Merkle on the same

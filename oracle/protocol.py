@@ -426,3 +426,81 @@ def fri_quotients(columns, samples, random_coeff, L, b, tw, itw):
         i = j
     return O.compute_quotients_and_combine([p[1] for p in per_point], [p[0] for p in per_point],
                                            [p[2] for p in per_point], L, b, tw, itw)
+
+
+# ---------------------------------------------------------------- wide-Fibonacci composition
+def _m31_point_add(p, q):
+    return ((p[0] * q[0] - p[1] * q[1]) % P, (p[0] * q[1] + p[1] * q[0]) % P)
+
+
+def coset_vanishing_m31(coset_initial_idx, coset_log, p):
+    """coset_vanishing (crates/stwo/src/core/constraints.rs:12-35) at an M31 point, with point
+    arithmetic: p - initial + step/2, then x doubled coset_log - 1 times."""
+    init = O.index_to_point(coset_initial_idx)
+    half_step = O.index_to_point((1 << (31 - coset_log)) >> 1)
+    q = _m31_point_add((int(p[0]), int(p[1])), (int(init[0]), (P - int(init[1])) % P))
+    q = _m31_point_add(q, (int(half_step[0]), int(half_step[1])))
+    x = q[0]
+    for _ in range(1, coset_log):
+        x = (2 * x * x - 1) % P
+    return x
+
+
+def wide_fib_composition(cols, trace_log, eval_log, log_expansion, powers):
+    """evaluate_constraint_quotients_on_domain for WideFibonacciEval
+    (crates/constraint-framework/src/prover/component_prover.rs:77-193, SubDomain mode;
+    crates/examples/src/wide_fibonacci/mod.rs:87-98): the 4 coordinate columns of
+    sum_j powers[j] * (c_{j+2} - c_j^2 - c_{j+1}^2) * denom_inv[row >> trace_log]."""
+    n = 1 << eval_log
+    c = [np.asarray(col[:n], np.uint64) for col in cols]
+    acc = np.zeros((4, n), np.uint64)
+    for j in range(len(cols) - 2):
+        k = (c[j + 2] + 2 * P - (c[j] * c[j] % P) - (c[j + 1] * c[j + 1] % P)) % P
+        for t in range(4):
+            acc[t] = (acc[t] + k * np.uint64(int(powers[j][t])) % P) % P
+    # denominators on the evaluation domain CanonicCoset(eval_log + e).circle_domain().split(e).0
+    half_init = 1 << (31 - (eval_log + log_expansion + 1))
+    n_den = 1 << (eval_log - trace_log)
+    den = [0] * n_den
+    for i in range(n_den):
+        p = O.circle_domain_at(half_init, eval_log - 1, i)
+        v = coset_vanishing_m31(1 << (31 - (trace_log + 1)), trace_log, p)
+        den[bitrev(i, eval_log - trace_log)] = O.m31_inv(v)
+    row_d = np.array(den, np.uint64)[np.arange(n) >> trace_log]
+    return (acc * row_d % P).astype(np.uint32)
+
+
+def from_partial_evals(e):
+    """SecureField::from_partial_evals: e0 + e1*i + e2*u + e3*iu (crates/stwo/src/core/fields/qm31.rs)."""
+    out = [0, 0, 0, 0]
+    for t in range(4):
+        basis = [0, 0, 0, 0]
+        basis[t] = 1
+        out = qadd(out, qmul(e[t], basis))
+    return out
+
+
+def wide_fib_composition_at_point(mask_values, alpha, point, trace_log):
+    """FrameworkComponent::evaluate_constraint_quotients_at_point (crates/constraint-framework/src/
+    component.rs:220-243) with PointEvaluationAccumulator (crates/stwo/src/core/air/accumulation.rs:
+    18-33): Horner over the constraints in order, each times 1/coset_vanishing(trace coset, point).
+    mask_values[k] is trace column k's value at the point (QM31)."""
+    x, y = [int(v) for v in point[:4]], [int(v) for v in point[4:]]
+    init = O.index_to_point(1 << (31 - (trace_log + 1)))
+    half_step = O.index_to_point((1 << (31 - trace_log)) >> 1)
+    # point - init + step/2 over QM31 x M31 points
+    def add_m31(px, py, q):
+        return (qsub(qmulm(px, q[0]), qmulm(py, q[1])), qadd(qmulm(px, q[1]), qmulm(py, q[0])))
+    px, py = add_m31(x, y, (int(init[0]), (P - int(init[1])) % P))
+    px, py = add_m31(px, py, (int(half_step[0]), int(half_step[1])))
+    v = px
+    for _ in range(1, trace_log):
+        s = qmul(v, v)
+        v = qsub(qadd(s, s), [1, 0, 0, 0])
+    dinv = qinv(v)
+    acc = [0, 0, 0, 0]
+    for j in range(len(mask_values) - 2):
+        a, b, c = mask_values[j], mask_values[j + 1], mask_values[j + 2]
+        cons = qsub(c, qadd(qmul(a, a), qmul(b, b)))
+        acc = qadd(qmul(acc, alpha), qmul(dinv, cons))
+    return acc

@@ -460,6 +460,18 @@ class _Ops:
             acc = qmul(acc, felt)
         return out
 
+    # ---- ComponentProver of the wide-Fibonacci AIR ------------------------------------------
+    def wide_fibonacci_composition(self, cols, trace_log, eval_log, powers, accum, log_expansion=0):
+        """evaluate_constraint_quotients_on_domain (crates/constraint-framework/src/prover/
+        component_prover.rs:118-193) for WideFibonacciEval: adds the constraint quotients of the
+        trace columns (committed evaluations) into accum (SecureColumnByCoords of 2^eval_log);
+        powers[j] is constraint j's random-coefficient power."""
+        flat = [int(x) for p in powers for x in p]
+        self._chk(self.lib.sh_wide_fibonacci_composition(self.h, _ptrs(cols), len(cols), trace_log, eval_log,
+                                                         log_expansion, _u32s(flat), _ptrs(accum.columns)),
+                  "wide_fibonacci_composition")
+        return accum
+
     # ---- GrindOps (crates/stwo/src/core/proof_of_work.rs:10-14) -------------------------------
     def grind(self, digest: bytes, pow_bits):
         d = (ctypes.c_uint8 * 32)(*digest)

@@ -1,0 +1,126 @@
+// Composition polynomial of the wide-Fibonacci AIR on the evaluation domain.
+//
+// ComponentProver::evaluate_constraint_quotients_on_domain
+// (crates/constraint-framework/src/prover/component_prover.rs:118-193) for
+// WideFibonacciEval (crates/examples/src/wide_fibonacci/mod.rs:77-98): row r of
+// the evaluation domain (bit-reversed order, the first 2^eval_log rows of the
+// committed evaluations) reads the N trace columns c_0..c_{N-1} and adds
+//   sum_j  w_j * (c_{j+2} - (c_j^2 + c_{j+1}^2))  *  denom_inv[r >> trace_log]
+// to the QM31 accumulator, where w_j is the accumulator's random-coefficient
+// power of constraint j (alpha^(M-1-j) for the only component, M = N - 2) and
+// denom_inv[i] = 1 / coset_vanishing(trace coset, eval_domain.at(i)), bit-reversed
+// (get_constraint_quotient_inputs, component_prover.rs:77-110).
+//
+// One lane per row; every load of a column is coalesced across the wave.  The
+// weights are uniform across lanes (scalar loads).  The QM31 sum is kept as four
+// unreduced 64-bit accumulators, folded every three constraints, so a
+// constraint costs two squarings, one subtraction and four 32x32->64 multiplies.
+#include "common.h"
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+
+__device__ __forceinline__ uint64_t fold64(uint64_t v) {  // v mod P, up to 2^33 + small
+  return (v & sh::P) + (v >> 31);
+}
+
+__global__ __launch_bounds__(kThreads) void k_wide_fib_composition(const uint32_t* const* __restrict__ cols,
+                                                                   uint32_t n_cols, uint32_t eval_log,
+                                                                   uint32_t trace_log,
+                                                                   const uint32_t* __restrict__ w,
+                                                                   const uint32_t* __restrict__ denom_inv,
+                                                                   uint32_t* const* __restrict__ acc) {
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  if (r >= (1u << eval_log)) return;
+  uint32_t a = cols[0][r], b = cols[1][r];
+  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  const uint32_t m = n_cols - 2;
+  uint32_t j = 0;
+  while (j < m) {
+    const uint32_t end = j + 3 < m ? j + 3 : m;
+#pragma unroll 3
+    for (; j < end; j++) {
+      const uint32_t c = cols[j + 2][r];
+      const uint32_t k = sh::msub(c, sh::madd(sh::mmul(a, a), sh::mmul(b, b)));
+      const uint4 wj = *(const uint4*)&w[4 * j];
+      s0 += (uint64_t)k * wj.x;
+      s1 += (uint64_t)k * wj.y;
+      s2 += (uint64_t)k * wj.z;
+      s3 += (uint64_t)k * wj.w;
+      a = b;
+      b = c;
+    }
+    // three products of < 2^62 each on top of a folded value (< 2^34) stay below 2^64
+    s0 = fold64(s0);
+    s1 = fold64(s1);
+    s2 = fold64(s2);
+    s3 = fold64(s3);
+  }
+  const uint32_t d = denom_inv[r >> trace_log];
+  auto finish = [&](uint64_t s, uint32_t* out) {
+    const uint32_t v = sh::reduce_once((uint32_t)fold64(s));  // s < 2^34: one more fold is < 2P
+    out[r] = sh::madd(out[r], sh::mmul(v, d));
+  };
+  finish(s0, acc[0]);
+  finish(s1, acc[1]);
+  finish(s2, acc[2]);
+  finish(s3, acc[3]);
+}
+
+}  // namespace
+
+// coset_vanishing (crates/stwo/src/core/constraints.rs:12-35) of a coset given by
+// index (initial, log) at the M31 point of index p: the x coordinate of
+// (p - initial + step/2) doubled log - 1 times.
+static uint32_t coset_vanishing_idx(uint32_t coset_initial, uint32_t coset_log, uint32_t p) {
+  const uint32_t half_step = sh::subgroup_gen(coset_log) >> 1;
+  uint64_t q = (uint64_t)p + (1ull << 31) - coset_initial + half_step;
+  q <<= (coset_log - 1);
+  return sh::index_to_point_host(sh::idx_reduce(q)).x;
+}
+
+extern "C" int sh_wide_fibonacci_composition(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
+                                             uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
+                                             const uint32_t* powers, uint32_t* const d_accum[4]) {
+  if (!ctx || !d_cols || !powers || !d_accum) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, n_cols >= 3 && n_cols <= 4096, "wide Fibonacci needs 3..4096 columns (got %llu)",
+             (unsigned long long)n_cols);
+  SH_REQUIRE(ctx, trace_log >= 1 && eval_log >= trace_log && eval_log <= 30 && eval_log - trace_log <= 8 &&
+                      log_expansion <= 4,
+             "invalid composition domain (trace %u, eval %u, expansion %u)", trace_log, eval_log, log_expansion);
+  // evaluation domain: CanonicCoset(eval_log + log_expansion).circle_domain().split(log_expansion).0,
+  // i.e. the circle domain whose half coset starts at G_(eval_log + log_expansion + 1) and has
+  // log size eval_log - 1 (crates/constraint-framework/src/prover/component_prover.rs:234-238)
+  const uint32_t half_init = sh::subgroup_gen(eval_log + log_expansion + 1);
+  const uint32_t half_step = sh::subgroup_gen(eval_log - 1);
+  const uint32_t n_denom = 1u << (eval_log - trace_log);
+  std::vector<uint32_t> denom(n_denom);
+  // trace coset: CanonicCoset(trace_log).coset() = odds of G_trace_log
+  const uint32_t trace_init = sh::subgroup_gen(trace_log + 1);
+  for (uint32_t i = 0; i < n_denom; i++) {
+    // CircleDomain::at (crates/stwo/src/core/poly/circle/domain.rs): half coset, then its negation
+    const uint32_t hs = 1u << (eval_log - 1);
+    uint32_t p;
+    if (i < hs) {
+      p = sh::idx_reduce((uint64_t)half_init + (uint64_t)half_step * i);
+    } else {
+      p = sh::idx_reduce((1ull << 31) - sh::idx_reduce((uint64_t)half_init + (uint64_t)half_step * (i - hs)));
+    }
+    const uint32_t v = coset_vanishing_idx(trace_init, trace_log, p);
+    SH_REQUIRE(ctx, v != 0, "the evaluation domain meets the trace domain");
+    denom[sh::bitrev(i, eval_log - trace_log)] = sh::minv(v);
+  }
+  // staged tables: column pointers, weights (4 words per constraint), denominators
+  void *d_ptrs, *d_w, *d_den, *d_acc;
+  SH_TRY(sh_stage(ctx, d_cols, n_cols * sizeof(void*), &d_ptrs));
+  SH_TRY(sh_stage(ctx, powers, 16 * (n_cols - 2), &d_w));
+  SH_TRY(sh_stage(ctx, denom.data(), 4ull * n_denom, &d_den));
+  SH_TRY(sh_stage(ctx, d_accum, 4 * sizeof(void*), &d_acc));
+  SpanGuard span(ctx, "composition");
+  const uint32_t blocks = (uint32_t)(((1ull << eval_log) + kThreads - 1) / kThreads);
+  k_wide_fib_composition<<<blocks, kThreads, 0, ctx->stream>>>(
+      (const uint32_t* const*)d_ptrs, (uint32_t)n_cols, eval_log, trace_log, (const uint32_t*)d_w,
+      (const uint32_t*)d_den, (uint32_t* const*)d_acc);
+  return sh_launch_check(ctx, "wide_fib_composition");
+}

@@ -46,3 +46,29 @@ def splitmix64_columns(seed, log_sizes):
 
 def cells(log_sizes):
     return sum(1 << lg for lg in log_sizes)
+
+
+# FRI + DEEP phase: 200 columns x 2^21 (LDE 2^22), seed 2, 3 out-of-domain points from
+# Blake2sChannel::default() + mix_u64(2), FriParams{pow 0, blowup 1, last layer 3, 32 queries}.
+FRI_DEEP = {"name": "fri_deep_200x2^21", "n_cols": 200, "log_size": 21, "seed": 2, "n_points": 3,
+            "fri": {"pow_bits": 0, "log_blowup_factor": 1, "log_last_layer_degree_bound": 3,
+                    "n_queries": 32, "fold_step": 1}}
+# Full prove of the wide-Fibonacci AIR: 256 columns x 2^20, seed 3, FriParams::new(26, 0, 1, 70, 1).
+WIDE_FIB = {"name": "wide_fib_256x2^20", "n_cols": 256, "log_size": 20, "seed": 3,
+            "fri": {"pow_bits": 26, "log_blowup_factor": 1, "log_last_layer_degree_bound": 0,
+                    "n_queries": 70, "fold_step": 1}}
+
+
+def wide_fib_trace(n_cols, log_size, seed):
+    """generate_trace (crates/examples/src/wide_fibonacci/mod.rs:27-48) with row inputs
+    (c0, c1) drawn from the shared generator (two columns of 2^log_size): c_k = c_{k-2}^2 +
+    c_{k-1}^2 for k >= 2.  Returns n_cols uint32 columns (bit-reversed evaluations on
+    CanonicCoset(log_size))."""
+    a, b = splitmix64_columns(seed, [log_size, log_size])
+    cols = [a, b]
+    x, y = a.astype(np.uint64), b.astype(np.uint64)
+    for _ in range(2, n_cols):
+        z = (x * x % P + y * y % P) % P
+        cols.append(z.astype(np.uint32))
+        x, y = y, z
+    return cols

@@ -211,3 +211,44 @@ def test_fri_prove_rejects_high_degree():
     col = rng.integers(0, P, (4, 1 << 6), dtype=np.uint32)
     with pytest.raises(AssertionError, match="invalid degree"):
         PR.fri_prove(col, 6, fri_test_params(), O.Channel())
+
+
+# ---------------------------------------------------------------- wide-Fibonacci composition
+def _composition_case(n_cols, log_n, seed, tamper=False):
+    from proving_amd.workloads import wide_fib_trace
+    trace = wide_fib_trace(n_cols, log_n, seed)
+    if tamper:
+        trace[n_cols // 2][3] = (int(trace[n_cols // 2][3]) + 1) % P
+    L = log_n + 1
+    tw, itw = O.canonic_twiddles(L)
+    coeffs = [O.interpolate(itw, c, log_n) for c in trace]
+    ldes = [O.evaluate(tw, co, L) for co in coeffs]
+    ch = O.Channel()
+    ch.mix_u64(seed)
+    alpha = [int(v) for v in ch.draw_secure_felt()]
+    z = [int(v) for v in ch.random_point()]
+    m = n_cols - 2
+    pw = [[1, 0, 0, 0]]
+    for _ in range(m - 1):
+        pw.append(PR.qmul(pw[-1], alpha))
+    powers = pw[::-1]  # constraint j gets alpha^(m-1-j)
+    comp = PR.wide_fib_composition(ldes, log_n, L, 0, powers)
+    comp_coeffs = [O.interpolate(itw, comp[t], L) for t in range(4)]
+    lhs = PR.from_partial_evals([[int(v) for v in O.eval_at_point(comp_coeffs[t], z)] for t in range(4)])
+    mask = [[int(v) for v in O.eval_at_point(co, z)] for co in coeffs]
+    rhs = PR.wide_fib_composition_at_point(mask, alpha, z, log_n)
+    return lhs, rhs, comp_coeffs
+
+
+@pytest.mark.parametrize("n_cols,log_n", [(3, 3), (5, 4), (16, 6), (40, 5)])
+def test_wide_fib_composition_matches_point_evaluation(n_cols, log_n):
+    """The domain-side composition (interpolated) agrees with the verifier's point-side
+    evaluation at a random out-of-domain point (crates/stwo/src/core/verifier.rs:102-117), which
+    fixes the weights' order, the denominators and the constraint."""
+    lhs, rhs, _ = _composition_case(n_cols, log_n, 7)
+    assert lhs == rhs
+
+
+def test_wide_fib_composition_detects_a_bad_row():
+    lhs, rhs, _ = _composition_case(16, 5, 8, tamper=True)
+    assert lhs != rhs
