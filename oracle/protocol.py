@@ -504,3 +504,231 @@ def wide_fib_composition_at_point(mask_values, alpha, point, trace_log):
         cons = qsub(c, qadd(qmul(a, a), qmul(b, b)))
         acc = qadd(qmul(acc, alpha), qmul(dinv, cons))
     return acc
+
+
+# ---------------------------------------------------------------- PCS and prove / verify
+def _hash_words(h):
+    return [int.from_bytes(h[4 * i:4 * i + 4], "little") for i in range(8)]
+
+
+def _words_hash(w):
+    return b"".join(int(x).to_bytes(4, "little") for x in w)
+
+
+def _double_point(pt):
+    x, y = list(pt[:4]), list(pt[4:])
+    xx = qmul(x, x)
+    return qsub(qadd(xx, xx), [1, 0, 0, 0]) + qmulm(qmul(x, y), 2)
+
+
+def _repeated_double(pt, n):
+    for _ in range(n):
+        pt = _double_point(pt)
+    return pt
+
+
+def pp_query_positions(query_positions, max_log_size, pp_max_log_size):
+    """prepare_preprocessed_query_positions (crates/stwo/src/core/pcs/utils.rs:185-203)."""
+    if pp_max_log_size == 0:
+        return []
+    if max_log_size < pp_max_log_size:
+        return [((p >> 1) << (pp_max_log_size - max_log_size + 1)) + (p & 1) for p in query_positions]
+    return [((p >> (max_log_size - pp_max_log_size + 1)) << 1) + (p & 1) for p in query_positions]
+
+
+class OracleTree:
+    """CommitmentTreeProver on the oracle: coefficients, evaluations (blown up), lifted tree."""
+
+    def __init__(self, coeffs, evals, lifting_log_size):
+        self.coeffs, self.evals, self.L = coeffs, evals, lifting_log_size
+        if evals:
+            self.layers = O.merkle_commit_lifted(evals, lifting_log_size)
+        else:
+            assert lifting_log_size == 0
+            self.layers = [np.frombuffer(O.blake2s(b""), np.uint8).reshape(1, 32)]
+        self.root = self.layers[0][0].tobytes()
+
+    def decommit(self, positions):
+        queried = [[lifted_value(e, self.L, p) for p in positions] for e in self.evals]
+        witness = merkle_decommit(self.layers, positions) if self.L > 0 else []
+        return queried, witness
+
+
+def pcs_prove_values(trees, sampled_points, ch, fri, tw, itw, oods_pow):
+    """CommitmentSchemeProver::prove_values (crates/stwo/src/prover/pcs/mod.rs:224-315) on the
+    oracle; returns the proof dict (same fields as the device driver's)."""
+    L = trees[-1].L
+    b = fri["log_blowup_factor"]
+    sampled_values, flat_cols, flat_samples = [], [], []
+    for tree, pts_tree in zip(trees, sampled_points):
+        vt = []
+        for c, e, pts in zip(tree.coeffs, tree.evals, pts_tree):
+            n_e = len(e).bit_length() - 1
+            vals = [[int(v) for v in O.eval_at_point(c, _repeated_double(p, L - n_e))] for p in pts]
+            vt.append(vals)
+            flat_cols.append(e)
+            flat_samples.append(list(zip([list(p) for p in pts], vals)))
+        sampled_values.append(vt)
+    ch.mix_felts(np.array([v for t in sampled_values for col in t for v in col], np.uint32).reshape(-1))
+    rc = [int(v) for v in ch.draw_secure_felt()]
+    quot = fri_quotients(flat_cols, flat_samples, rc, L, b, tw, itw)
+    fproof, queries = fri_prove(quot, L, fri, ch)
+    pp = pp_query_positions(queries, L, trees[0].L)
+    qv, dec = [], []
+    for i, t in enumerate(trees):
+        q, w = t.decommit(pp if i == 0 else queries)
+        qv.append(q)
+        dec.append(w)
+    return {"fri_params": dict(fri), "commitments": [t.root for t in trees], "oods_proof_of_work": oods_pow,
+            "sampled_values": sampled_values, "decommitments": dec, "queried_values": qv,
+            "fri_felts": serialize_fri_proof(fproof), "query_positions": queries}
+
+
+def serialize_pcs_proof(proof):
+    """CairoSerialize for CommitmentSchemeProof (crates/cairo-serialize/src/serialize.rs:99-143)."""
+    f = proof["fri_params"]
+    out = [f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"], f["fold_step"]]
+    out.append(len(proof["commitments"]))
+    for h in proof["commitments"]:
+        out.extend(_hash_words(h))
+    out.append(proof["oods_proof_of_work"])
+    out.append(len(proof["sampled_values"]))
+    for t in proof["sampled_values"]:
+        out.append(len(t))
+        for col in t:
+            out.append(len(col))
+            for v in col:
+                out.extend(int(x) for x in v)
+    out.append(len(proof["decommitments"]))
+    for w in proof["decommitments"]:
+        out.append(len(w))
+        for h in w:
+            out.extend(_hash_words(h))
+    out.append(len(proof["queried_values"]))
+    for t in proof["queried_values"]:
+        out.append(len(t))
+        for col in t:
+            out.append(len(col))
+            out.extend(int(x) for x in col)
+    out.extend(proof["fri_felts"])
+    return out
+
+
+def deserialize_pcs_proof(felts):
+    it = iter(felts)
+    nx = lambda: int(next(it))
+    fri = {k: nx() for k in ("pow_bits", "log_blowup_factor", "log_last_layer_degree_bound", "n_queries",
+                             "fold_step")}
+    commitments = [_words_hash([nx() for _ in range(8)]) for _ in range(nx())]
+    oods_pow = nx()
+    sampled = [[[[nx() for _ in range(4)] for _ in range(nx())] for _ in range(nx())] for _ in range(nx())]
+    dec = [[_words_hash([nx() for _ in range(8)]) for _ in range(nx())] for _ in range(nx())]
+    qv = [[[nx() for _ in range(nx())] for _ in range(nx())] for _ in range(nx())]
+    fri_felts = list(it)
+    return {"fri_params": fri, "commitments": commitments, "oods_proof_of_work": oods_pow, "sampled_values": sampled,
+            "decommitments": dec, "queried_values": qv, "fri_proof": deserialize_fri_proof(fri_felts)}
+
+
+def _verify_tree(root, height, col_logs, positions, queried, witness):
+    """MerkleVerifierLifted::verify (crates/stwo/src/core/vcs_lifted/verifier.rs:115-212)."""
+    if height == 0:
+        return True
+    order = sorted(range(len(col_logs)), key=lambda i: col_logs[i])
+    rows = [[queried[i][k] for i in order] for k in range(len(positions))]
+    return merkle_verify(root, height, positions, rows, witness)
+
+
+def pcs_verify_values(tree_roots, tree_heights, tree_col_logs, sampled_points, proof, ch, fri):
+    """CommitmentSchemeVerifier::verify_values (crates/stwo/src/core/pcs/verifier.rs:57-130).
+    tree_col_logs: extended (blown-up) log sizes per tree. Raises AssertionError on failure."""
+    ch.mix_felts(np.array([v for t in proof["sampled_values"] for col in t for v in col],
+                          np.uint32).reshape(-1))
+    rc = [int(v) for v in ch.draw_secure_felt()]
+    L = tree_heights[-1]
+    b = fri["log_blowup_factor"]
+    flat_logs, flat_samples, flat_cols_q = [], [], []
+    for logs_t, pts_t, vals_t in zip(tree_col_logs, sampled_points, proof["sampled_values"]):
+        for lg, pts, vals in zip(logs_t, pts_t, vals_t):
+            flat_logs.append(lg)
+            flat_samples.append(list(zip(pts, vals)))
+    state = {}
+
+    def first_layer_evals(queries):
+        pp = pp_query_positions(queries, L, tree_heights[0])
+        for i, (root, h, logs) in enumerate(zip(tree_roots, tree_heights, tree_col_logs)):
+            pos = pp if i == 0 else queries
+            assert _verify_tree(root, h, logs, pos, proof["queried_values"][i], proof["decommitments"][i]), \
+                f"tree {i} decommitment invalid"
+        qv = [col for t in proof["queried_values"] for col in t]
+        return fri_answers(flat_logs, flat_samples, rc, queries, qv, L)
+
+    fri_verify(proof["fri_proof"], fri, ch, L - b, first_layer_evals)
+    return True
+
+
+def wide_fib_prove(trace, log_n, fri, tw, itw):
+    """The prover side of test_wide_fib_prove_with_blake (crates/examples/src/wide_fibonacci/
+    mod.rs:183-220) through prove_ex (crates/stwo/src/prover/mod.rs:38-135), on the oracle.
+    Requires log_blowup_factor 1. Returns (felts, proof dict)."""
+    b = fri["log_blowup_factor"]
+    assert b == 1
+    L = log_n + b
+    ch = O.Channel()
+    trees = [OracleTree([], [], 0)]
+    ch.mix_hash(trees[0].root)
+    coeffs = [O.interpolate(itw, c, log_n) for c in trace]
+    trees.append(OracleTree(coeffs, [O.evaluate(tw, c, L) for c in coeffs], L))
+    ch.mix_hash(trees[1].root)
+    alpha = [int(v) for v in ch.draw_secure_felt()]
+    m = len(trace) - 2
+    pw = [[1, 0, 0, 0]]
+    for _ in range(m - 1):
+        pw.append(qmul(pw[-1], alpha))
+    comp = wide_fib_composition(trees[1].evals, log_n, log_n + 1, 0, pw[::-1])
+    comp_coeffs = [O.interpolate(itw, comp[t], log_n + 1) for t in range(4)]
+    half = 1 << log_n
+    polys = [c[:half] for c in comp_coeffs] + [c[half:] for c in comp_coeffs]   # split_at_mid
+    trees.append(OracleTree(polys, [O.evaluate(tw, p, L) for p in polys], L))
+    ch.mix_hash(trees[2].root)
+    oods_pow = O.grind(ch.digest, 16)
+    ch.mix_u64(oods_pow)
+    z = [int(v) for v in ch.random_point()]
+    pts = [[], [[z] for _ in trace], [[z] for _ in range(8)]]
+    proof = pcs_prove_values(trees, pts, ch, fri, tw, itw, oods_pow)
+    return serialize_pcs_proof(proof), proof
+
+
+def wide_fib_verify(felts, n_cols, log_n, fri):
+    """verify (crates/stwo/src/core/verifier.rs:19-119) for one wide-Fibonacci component, after
+    the commitment-scheme setup of test_wide_fib_prove_with_blake (preprocessed tree empty and
+    not lifted; trace lifting log_n + blowup). Raises AssertionError on a bad proof."""
+    proof = deserialize_pcs_proof(felts)
+    assert proof["fri_params"] == dict(fri), "config mismatch"
+    b = fri["log_blowup_factor"]
+    L = log_n + b
+    roots = proof["commitments"]
+    assert len(roots) == 3, "expected preprocessed, trace and composition trees"
+    ch = O.Channel()
+    ch.mix_hash(roots[0])
+    ch.mix_hash(roots[1])
+    alpha = [int(v) for v in ch.draw_secure_felt()]
+    split_bound = (log_n + 1) - 1                      # composition_log_degree_bound - COMPOSITION_LOG_SPLIT
+    L = max(L, split_bound + b)
+    max_log_degree_bound = L - b
+    ch.mix_hash(roots[2])
+    assert O.verify_pow_nonce(ch.digest, 16, proof["oods_proof_of_work"]), "OodsProofOfWork"
+    ch.mix_u64(proof["oods_proof_of_work"])
+    z = [int(v) for v in ch.random_point()]
+    pts = [[], [[z] for _ in range(n_cols)], [[z] for _ in range(8)]]
+    sv = proof["sampled_values"]
+    assert [len(t) for t in sv] == [0, n_cols, 8] and all(len(c) == 1 for t in sv[1:] for c in t), "structure"
+    comp = [c[0] for c in sv[2]]
+    left = from_partial_evals(comp[:4])
+    right = from_partial_evals(comp[4:])
+    zx = _repeated_double(z, max_log_degree_bound - 1)[:4]
+    lhs = qadd(left, qmul(zx, right))                   # extract_composition_oods_eval
+    rhs = wide_fib_composition_at_point([c[0] for c in sv[1]], alpha, z, max_log_degree_bound)
+    assert lhs == rhs, "OodsNotMatching"
+    heights = [0, L, L]
+    col_logs = [[], [log_n + b] * n_cols, [max_log_degree_bound + b] * 8]
+    return pcs_verify_values(roots, heights, col_logs, pts, proof, ch, fri)

@@ -1,0 +1,226 @@
+"""Host protocol above the HIP backend: the commitment scheme prover and `prove()` for the
+wide-Fibonacci AIR, with the proof written as CairoSerde felts.
+
+Restates, over device-resident columns:
+- CommitmentSchemeProver::commit / prove_values (crates/stwo/src/prover/pcs/mod.rs:81-315);
+- CommitmentTreeProver::new / decommit (crates/stwo/src/prover/pcs/mod.rs:360-399);
+- prove_ex (crates/stwo/src/prover/mod.rs:38-135) for one FrameworkComponent of
+  WideFibonacciEval (crates/examples/src/wide_fibonacci/mod.rs:77-98);
+- CommitmentSchemeProof / StarkProof felt encoding (crates/cairo-serialize/src/serialize.rs:99-153).
+
+Every kernel runs on the device; the host handles the channel, a few QM31 scalars, and the
+small proof vectors. There is no CPU fallback: without libstwo_hip.so nothing here imports.
+"""
+import ctypes
+
+import numpy as np
+
+from . import field as F
+from .backend import (BaseColumn, Channel, FriParams, HashLayer, SecureColumnByCoords, canonic_half_initial, fri_prove,
+                      fri_quotients, gather, merkle_decommit)
+
+OODS_POW_BITS = 16  # crates/stwo/src/core/proof_of_work.rs:8
+COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
+
+
+def _hash_words(h):
+    return [int.from_bytes(h[4 * i:4 * i + 4], "little") for i in range(8)]
+
+
+class CommitmentTree:
+    """CommitmentTreeProver: coefficient columns, their evaluations on the blown-up domain, and
+    the lifted Merkle tree over the evaluations (layers root-first)."""
+
+    def __init__(self, coeffs, evals, layers, root, lifting_log_size):
+        self.coeffs, self.evals, self.layers = coeffs, evals, layers
+        self.root, self.lifting_log_size = root, lifting_log_size
+
+    def decommit(self, backend, positions):
+        """CommitmentTreeProver::decommit (crates/stwo/src/prover/pcs/mod.rs:392-398) ->
+        MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162):
+        queried values per column at every position (duplicates kept), and the hash witness."""
+        L = self.lifting_log_size
+        cols, rows = [], []
+        for e in self.evals:
+            shift = L - e.log_size
+            cols.extend([e] * len(positions))
+            rows.extend(((p >> (shift + 1)) << 1) + (p & 1) for p in positions)
+        flat = gather(backend, cols, rows) if rows else []  # one launch for every (column, row)
+        k = len(positions)
+        queried = [flat[i * k:(i + 1) * k] for i in range(len(self.evals))]
+        witness = merkle_decommit(backend, self.layers, sorted(set(positions))) if L > 0 else []
+        return queried, witness
+
+
+class CommitmentSchemeProver:
+    """CommitmentSchemeProver over the HIP backend (crates/stwo/src/prover/pcs/mod.rs:48-315)."""
+
+    def __init__(self, backend, twiddles, fri, trace_lifting_log_size, preprocessed_lifting_log_size):
+        self.be, self.tw = backend, twiddles
+        self.fri = dict(fri)
+        self.trace_lifting_log_size = trace_lifting_log_size
+        self.preprocessed_lifting_log_size = preprocessed_lifting_log_size
+        self.trees = []
+
+    def lifting_log_size(self, tree_index):
+        """PcsConfig::lifting_log_size (crates/stwo/src/core/pcs/mod.rs:70-76)."""
+        return self.preprocessed_lifting_log_size if tree_index == 0 else self.trace_lifting_log_size
+
+    def _commit(self, coeffs, evals, channel):
+        ctx = self.be.ctx
+        L = self.lifting_log_size(len(self.trees))
+        if not evals:
+            assert L == 0, "lifting_log_size must be 0 when no columns are committed"
+        layers, root = self.be.commit_lifted(evals, L)
+        channel.mix_root(root)
+        self.trees.append(CommitmentTree(coeffs, evals, layers, root, L))
+        return self.trees[-1]
+
+    def commit_evals(self, evals, channel):
+        """TreeBuilder::extend_evals + commit (crates/stwo/src/prover/pcs/mod.rs:325-334, 81-93):
+        interpolate, evaluate on the blown-up domain, commit.  One batched sh_commit_trace when
+        the tree height is the largest column's blown-up size (the usual case)."""
+        b = self.fri["log_blowup_factor"]
+        L = self.lifting_log_size(len(self.trees))
+        if evals and L == max(e.log_size for e in evals) + b:
+            coeffs = [BaseColumn(self.be.ctx, e.log_size) for e in evals]
+            ldes = [BaseColumn(self.be.ctx, e.log_size + b) for e in evals]
+            layers = [HashLayer(self.be.ctx, k) for k in range(L + 1)]
+            _, _, _, root = self.be.commit_trace(evals, b, self.tw, coeffs=coeffs, ldes=ldes, layers=layers)
+            channel.mix_root(root)
+            self.trees.append(CommitmentTree(coeffs, ldes, layers, root, L))
+            return self.trees[-1]
+        coeffs = []
+        for e in evals:
+            c = BaseColumn(self.be.ctx, e.log_size)
+            c.copy_from(e)
+            coeffs.append(c)
+        self.be.interpolate_columns(coeffs, self.tw)
+        return self.commit_polys(coeffs, channel)
+
+    def commit_polys(self, coeffs, channel):
+        """TreeBuilder::extend_polys + commit: evaluate with the blowup, then commit."""
+        b = self.fri["log_blowup_factor"]
+        ldes = self.be.evaluate_polynomials(coeffs, b, self.tw) if coeffs else []
+        return self._commit(coeffs, ldes, channel)
+
+    def prove_values(self, sampled_points, channel, oods_proof_of_work):
+        """prove_values (crates/stwo/src/prover/pcs/mod.rs:224-315). sampled_points[tree][col] is
+        a list of QM31 circle points (8 words). Returns (felts of the CommitmentSchemeProof, the
+        proof as a dict)."""
+        L = self.trees[-1].lifting_log_size
+        b = self.fri["log_blowup_factor"]
+        # compute_samples (crates/stwo/src/prover/pcs/mod.rs:176-222): coefficient evaluation at the
+        # point doubled down to the column's own domain
+        sampled_values = []
+        flat_cols, flat_samples = [], []
+        for tree, pts_tree in zip(self.trees, sampled_points):
+            vals_tree = []
+            for c, e, pts in zip(tree.coeffs, tree.evals, pts_tree):
+                vals = [list(self.be.eval_at_point(c, F.qpoint_repeated_double(p, L - e.log_size))) for p in pts]
+                vals_tree.append(vals)
+                flat_cols.append(e)
+                flat_samples.append([(list(p), v) for p, v in zip(pts, vals)])
+            sampled_values.append(vals_tree)
+        channel.mix_felts([v for t in sampled_values for col in t for v in col])
+        random_coeff = [int(x) for x in channel.draw_secure_felt()]
+        quotients = fri_quotients(self.be, self.tw, flat_cols, flat_samples, random_coeff, L, b)
+        params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
+                           self.fri["fold_step"])
+        fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients)
+        quotients.free()
+        pp_height = self.trees[0].lifting_log_size
+        pp_positions = prepare_preprocessed_query_positions(queries, L, pp_height)
+        queried_values, decommitments = [], []
+        for i, tree in enumerate(self.trees):
+            qv, wit = tree.decommit(self.be, pp_positions if i == 0 else queries)
+            queried_values.append(qv)
+            decommitments.append(wit)
+        proof = {"fri_params": dict(self.fri), "commitments": [t.root for t in self.trees],
+                 "oods_proof_of_work": oods_proof_of_work, "sampled_values": sampled_values,
+                 "decommitments": decommitments, "queried_values": queried_values, "fri_felts": fri_felts,
+                 "query_positions": queries}
+        return serialize_commitment_scheme_proof(proof), proof
+
+
+def prepare_preprocessed_query_positions(query_positions, max_log_size, pp_max_log_size):
+    """crates/stwo/src/core/pcs/utils.rs:185-203."""
+    if pp_max_log_size == 0:
+        return []
+    if max_log_size < pp_max_log_size:
+        return [((p >> 1) << (pp_max_log_size - max_log_size + 1)) + (p & 1) for p in query_positions]
+    return [((p >> (max_log_size - pp_max_log_size + 1)) << 1) + (p & 1) for p in query_positions]
+
+
+def serialize_commitment_scheme_proof(proof):
+    """CairoSerialize for CommitmentSchemeProof (crates/cairo-serialize/src/serialize.rs:121-143):
+    PcsConfig (its FriParams only), commitments, oods_proof_of_work, sampled_values,
+    decommitments, queried_values, fri_proof; vectors are length-prefixed, a hash is 8 LE u32,
+    a QM31 4 felts."""
+    f = proof["fri_params"]
+    out = [f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"], f["fold_step"]]
+    out.append(len(proof["commitments"]))
+    for h in proof["commitments"]:
+        out.extend(_hash_words(h))
+    out.append(proof["oods_proof_of_work"])
+    out.append(len(proof["sampled_values"]))
+    for tree in proof["sampled_values"]:
+        out.append(len(tree))
+        for col in tree:
+            out.append(len(col))
+            for v in col:
+                out.extend(int(x) for x in v)
+    out.append(len(proof["decommitments"]))
+    for wit in proof["decommitments"]:
+        out.append(len(wit))
+        for h in wit:
+            out.extend(_hash_words(h))
+    out.append(len(proof["queried_values"]))
+    for tree in proof["queried_values"]:
+        out.append(len(tree))
+        for col in tree:
+            out.append(len(col))
+            out.extend(int(x) for x in col)
+    out.extend(proof["fri_felts"])
+    return out
+
+
+def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None):
+    """test_wide_fib_prove_with_blake's prover side (crates/examples/src/wide_fibonacci/mod.rs:
+    183-220) through prove_ex (crates/stwo/src/prover/mod.rs:38-135): empty preprocessed tree,
+    the trace tree, the composition polynomial (evaluated on the committed evaluations,
+    interpolated, split at the middle, committed as 8 coordinate polynomials), the OODS
+    proof of work, the OODS point and prove_values.  trace: device columns (BaseColumn, bit-
+    reversed evaluations on CanonicCoset(log_n)).  Returns (felts, proof dict, the channel)."""
+    ctx = backend.ctx
+    b = fri["log_blowup_factor"]
+    n_cols = len(trace)
+    channel = channel or Channel(backend.lib)
+    scheme = CommitmentSchemeProver(backend, twiddles, fri, log_n + b, 0)
+    scheme.commit_evals([], channel)                       # preprocessed trace (empty)
+    trace_tree = scheme.commit_evals(trace, channel)       # trace
+    # compute_composition_polynomial (crates/stwo/src/prover/air/component_prover.rs:103-131)
+    alpha = [int(x) for x in channel.draw_secure_felt()]
+    m = n_cols - 2
+    powers = F.secure_powers(alpha, m)[::-1]               # ColumnAccumulator powers, reversed
+    eval_log = log_n + 1                                   # max_constraint_log_degree_bound
+    assert 1 <= b, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
+    log_expansion = b - 1                                  # EvaluationMode::infer: SubDomain
+    assert log_expansion == 0, "only log_blowup_factor 1 (SubDomain {log_expansion: 0}) is driven here"
+    acc = SecureColumnByCoords(ctx, eval_log)
+    for c in acc.columns:
+        ctx.check(ctx.lib.sh_memset(ctx.h, c.ptr, 0, c.nbytes), "memset")
+    backend.wide_fibonacci_composition(trace_tree.evals, log_n, eval_log, powers, acc, log_expansion)
+    backend.interpolate_columns(acc.columns, twiddles)    # DomainEvaluationAccumulator::finalize
+    halves = [backend.split_at_mid(c) for c in acc.columns]
+    comp_polys = [h[0] for h in halves] + [h[1] for h in halves]
+    scheme.commit_polys(comp_polys, channel)
+    # OODS proof of work and point (crates/stwo/src/prover/mod.rs:74-81)
+    oods_pow = backend.grind(channel.digest, OODS_POW_BITS)
+    channel.mix_u64(oods_pow)
+    z = [int(x) for x in channel.draw_random_point()]
+    sampled_points = [[], [[z] for _ in range(n_cols)], [[z] for _ in range(8)]]
+    felts, proof = scheme.prove_values(sampled_points, channel, oods_pow)
+    proof["composition_random_coeff"] = alpha
+    proof["oods_point"] = z
+    return felts, proof, channel

@@ -1,0 +1,57 @@
+"""End-to-end proofs on the device: the wide-Fibonacci AIR proved through the commitment
+scheme (trace commit, composition, OODS sampling, DEEP quotients, FRI, decommitment) with the
+proof written as CairoSerde felts.  Small cases are byte-identical to the oracle's prover; every
+proof is accepted by the oracle's restatement of the reference verifier, and a tampered proof
+is rejected."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import protocol as PR
+from proving_amd.backend import BaseColumn
+from proving_amd.prover import prove_wide_fibonacci
+from proving_amd.workloads import WIDE_FIB, wide_fib_trace
+
+pytestmark = pytest.mark.gpu
+P = O.P
+
+
+def params(pow_bits=4, last=0, n_queries=3):
+    return {"pow_bits": pow_bits, "log_blowup_factor": 1, "log_last_layer_degree_bound": last,
+            "n_queries": n_queries, "fold_step": 1}
+
+
+@pytest.mark.parametrize("n_cols,log_n,fri", [
+    (4, 4, params()), (12, 6, params(6, 1, 9)), (32, 10, params(8, 2, 20)), (64, 13, params(10, 0, 70))])
+def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n, fri):
+    trace = wide_fib_trace(n_cols, log_n, 100 + n_cols)
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    exp_felts, _ = PR.wide_fib_prove(trace, log_n, fri, tw, itw)
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
+    felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri)
+    assert felts == exp_felts
+    assert PR.wide_fib_verify(felts, n_cols, log_n, fri)
+
+
+def test_wide_fib_bad_trace_rejected(ctx, backend):
+    n_cols, log_n = 16, 8
+    trace = wide_fib_trace(n_cols, log_n, 5)
+    trace[7][100] = (int(trace[7][100]) + 1) % P
+    fri = params()
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    felts, _, _ = prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri)
+    with pytest.raises(AssertionError):
+        PR.wide_fib_verify(felts, n_cols, log_n, fri)
+
+
+def test_wide_fib_full_size_verifies(ctx, backend):
+    """256 columns x 2^20 with FriParams::new(26, 0, 1, 70, 1) (the production defaults): the
+    proof from the device verifies with the oracle verifier."""
+    w = WIDE_FIB
+    trace = wide_fib_trace(w["n_cols"], w["log_size"], w["seed"])
+    tree = backend.precompute_canonic_twiddles(w["log_size"] + 1)
+    cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
+    del trace
+    felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, w["log_size"], w["fri"])
+    assert PR.wide_fib_verify(felts, w["n_cols"], w["log_size"], w["fri"])

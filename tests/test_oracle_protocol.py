@@ -252,3 +252,46 @@ def test_wide_fib_composition_matches_point_evaluation(n_cols, log_n):
 def test_wide_fib_composition_detects_a_bad_row():
     lhs, rhs, _ = _composition_case(16, 5, 8, tamper=True)
     assert lhs != rhs
+
+
+# ---------------------------------------------------------------- full prove -> verify
+def _wf_params(pow_bits=4, last=0, n_queries=3):
+    return {"pow_bits": pow_bits, "log_blowup_factor": 1, "log_last_layer_degree_bound": last,
+            "n_queries": n_queries, "fold_step": 1}
+
+
+@pytest.mark.parametrize("n_cols,log_n", [(4, 4), (12, 6)])
+def test_wide_fib_prove_verify_round_trip(n_cols, log_n):
+    """test_wide_fib_prove_with_blake (crates/examples/src/wide_fibonacci/mod.rs:183-237) on the
+    oracle: the proof verifies, and its felts decode back to the same proof."""
+    from proving_amd.workloads import wide_fib_trace
+    trace = wide_fib_trace(n_cols, log_n, 11)
+    fri = _wf_params()
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    felts, proof = PR.wide_fib_prove(trace, log_n, fri, tw, itw)
+    assert PR.wide_fib_verify(felts, n_cols, log_n, fri)
+    d = PR.deserialize_pcs_proof(felts)
+    assert d["commitments"] == proof["commitments"] and d["queried_values"] == proof["queried_values"]
+
+
+@pytest.mark.parametrize("tamper", ["trace", "queried", "sampled", "pow", "commitment"])
+def test_wide_fib_verify_rejects(tamper):
+    from proving_amd.workloads import wide_fib_trace
+    n_cols, log_n = 6, 5
+    trace = wide_fib_trace(n_cols, log_n, 12)
+    if tamper == "trace":
+        trace[3][5] = (int(trace[3][5]) + 1) % P
+    fri = _wf_params()
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    felts, proof = PR.wide_fib_prove(trace, log_n, fri, tw, itw)
+    if tamper == "queried":
+        proof["queried_values"][1][2][0] = (proof["queried_values"][1][2][0] + 1) % P
+    elif tamper == "sampled":
+        proof["sampled_values"][1][0][0] = PR.qadd(proof["sampled_values"][1][0][0], [1, 0, 0, 0])
+    elif tamper == "pow":
+        proof["oods_proof_of_work"] += 1
+    elif tamper == "commitment":
+        proof["commitments"][1] = bytes(32)
+    felts = PR.serialize_pcs_proof(proof)
+    with pytest.raises(AssertionError):
+        PR.wide_fib_verify(felts, n_cols, log_n, fri)
