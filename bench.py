@@ -103,12 +103,158 @@ def cpu_baseline(work, expected_root):
             "root_matches_gpu": root == expected_root}
 
 
+def _timed(ctx, step_fn, steps):
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_fn()
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def _kernel_table(prof, steps):
+    return {k: {"launches_per_step": v["n"] / steps, "ms_per_step": v["ms"] / steps,
+                "avg_launch_us": 1e3 * v["ms"] / v["n"]} for k, v in prof.items()}
+
+
+def bench_fri_deep(args):
+    """The FRI + DEEP-quotient phase of prove_values (crates/stwo/src/prover/pcs/mod.rs:224-258) on
+    200 columns x 2^21 (LDE 2^22): one step = OOD sampling of every column at 3 points, mixing the
+    samples, drawing the random coefficient, compute_fri_quotients, and FRI commit + decommit with
+    32 queries; the channel starts from the same state each step (the phase is deterministic)."""
+    from proving_amd.backend import (BaseColumn, Channel, Context, FriParams, HashLayer, HipBackend,
+                                     fri_prove, fri_quotients)
+    from proving_amd.workloads import FRI_DEEP, splitmix64_columns
+    w = FRI_DEEP
+    n, N, b = w["log_size"], w["n_cols"], w["fri"]["log_blowup_factor"]
+    L = n + b
+    ctx = Context(0)
+    be = HipBackend(ctx)
+    tree = be.precompute_canonic_twiddles(L)
+    log(f"[bench] fri_deep: {N} columns x 2^{n}, LDE 2^{L}; generating inputs")
+    host = splitmix64_columns(w["seed"], [n] * N)
+    evals = [BaseColumn.from_cpu(ctx, c) for c in host]
+    del host
+    coeffs = [BaseColumn(ctx, n) for _ in range(N)]
+    ldes = [BaseColumn(ctx, L) for _ in range(N)]
+    layers = [HashLayer(ctx, k) for k in range(L + 1)]
+    be.commit_trace(evals, b, tree, coeffs, ldes, layers)  # setup: interpolate + LDE (not timed)
+    for e in evals:
+        e.free()
+    ch0 = Channel(ctx.lib)
+    ch0.mix_u64(w["seed"])
+    points = [[int(x) for x in ch0.draw_random_point()] for _ in range(w["n_points"])]
+    params = FriParams(*[w["fri"][k] for k in ("pow_bits", "log_blowup_factor", "log_last_layer_degree_bound",
+                                               "n_queries", "fold_step")])
+    state = bytes(ch0.s.digest), ch0.s.n_draws
+    result = {}
+
+    def step():
+        ch = Channel(ctx.lib)
+        ch.s.digest[:] = list(state[0])
+        ch.s.n_draws = state[1]
+        vals = be.eval_at_points(coeffs, points)
+        samples = [[(p, v) for p, v in zip(points, vc)] for vc in vals]
+        ch.mix_felts([v for col in samples for _, v in col])
+        rc = [int(x) for x in ch.draw_secure_felt()]
+        quot = fri_quotients(be, tree, ldes, samples, rc, L, b)
+        felts, queries = fri_prove(be, tree, ch, params, quot)
+        quot.free()
+        result["felts"], result["queries"], result["digest"] = felts, queries, ch.digest
+
+    step()
+    first = dict(result)
+    for _ in range(args.warmup):
+        step()
+    ctx.profile(True)
+    sec = _timed(ctx, step, args.steps)
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    assert result["felts"] == first["felts"], "the phase is not deterministic"
+    ms = sec * 1e3 / args.steps
+    # roofline: the fused numerator pass reads the 2^(L-b)-row subdomain of every column once
+    num_bytes = 4 * N * (1 << (L - b)) + 16 * w["n_points"] * (1 << (L - b))
+    k = prof.get("quotients_accumulate_numerators")
+    roof = None
+    if k:
+        avg = k["ms"] / k["n"] / 1e3
+        ach = num_bytes / (k["n"] / args.steps) / avg / 1e9
+        roof = {"kernel": "quotients_accumulate_numerators", "bound": "hbm", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_step": num_bytes, "avg_launch_us": round(avg * 1e6, 2)}
+    out = {"metric": "FRI+DEEP phases/sec (200 x 2^21 columns, 3 OOD points, 32 queries)",
+           "value": round(1e3 / ms, 3), "unit": "phases/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32 (M31/QM31)", "data": "synthetic",
+           "config": {"workload": w["name"], "columns": N, "log_size": n, "lde_log_size": L,
+                      "points": w["n_points"], "fri": w["fri"]},
+           "roofline": roof, "kernels": _kernel_table(prof, args.steps),
+           "proof_felts": len(first["felts"]), "queries": first["queries"][:8],
+           "last_layer_poly_felts": first["felts"][-(4 * 8 + 2):-2]}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_prove(args):
+    """A full proof of the wide-Fibonacci AIR (256 columns x 2^20, FriParams::new(26, 0, 1, 70, 1))
+    on the device: one step = prove() from a resident trace (trace commit, composition,
+    composition commit, OODS grind and sampling, DEEP quotients, FRI with the 26-bit grind,
+    decommitment, proof felts)."""
+    from proving_amd.backend import BaseColumn, Context, HipBackend
+    from proving_amd.prover import prove_wide_fibonacci
+    from proving_amd.workloads import WIDE_FIB, wide_fib_trace
+    w = WIDE_FIB
+    n, N = w["log_size"], w["n_cols"]
+    ctx = Context(0)
+    be = HipBackend(ctx)
+    tree = be.precompute_canonic_twiddles(n + w["fri"]["log_blowup_factor"])
+    log(f"[bench] prove: wide Fibonacci {N} x 2^{n}; generating the trace")
+    trace = [BaseColumn.from_cpu(ctx, c) for c in wide_fib_trace(N, n, w["seed"])]
+    if not args.no_autotune:
+        be.autotune_fft([n] * N, w["fri"]["log_blowup_factor"], tree)
+    res = {}
+
+    def step():
+        felts, proof, _ = prove_wide_fibonacci(be, tree, trace, n, w["fri"])
+        res["felts"] = felts
+
+    step()
+    first = list(res["felts"])
+    for _ in range(args.warmup):
+        step()
+    steps = max(1, args.steps // 4)
+    ctx.profile(True)
+    sec = _timed(ctx, step, steps)
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    assert res["felts"] == first, "proofs differ between runs"
+    ms = sec * 1e3 / steps
+    out = {"metric": "wide-Fibonacci proofs/sec (256 x 2^20, 70 queries, 26-bit grind)",
+           "value": round(1e3 / ms, 3), "unit": "proofs/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u32 (M31/QM31)", "data": "synthetic",
+           "config": {"workload": w["name"], "columns": N, "log_size": n, "fri": w["fri"]},
+           "kernels": _kernel_table(prof, steps), "proof_felts": len(first)}
+    if not args.no_cpu_baseline:
+        import oracle as O
+        from oracle import protocol as PR
+        log("[bench] verifying the proof with the oracle verifier ...")
+        t0 = time.perf_counter()
+        out["oracle_verifies"] = bool(PR.wide_fib_verify(first, N, n, w["fri"]))
+        out["oracle_verify_s"] = round(time.perf_counter() - t0, 2)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["uneven", "plain"], default="uneven")
+    ap.add_argument("--workload", choices=["uneven", "plain", "fri_deep", "prove"], default="uneven",
+                    help="uneven (default, the headline): the 512-column batched commit; plain: 64 x 2^14; "
+                         "fri_deep: the FRI + DEEP-quotient phase on 200 x 2^21; prove: a full proof of the "
+                         "wide-Fibonacci AIR, 256 x 2^20")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fft-tuning", default=None,
@@ -141,6 +287,10 @@ def main():
         ctx.close()
         return
 
+    if args.workload == "fri_deep":
+        return bench_fri_deep(args)
+    if args.workload == "prove":
+        return bench_prove(args)
     work = UNEVEN_COMMIT if args.workload == "uneven" else PLAIN_COMMIT
     logs, b, seed = work["logs"], work["log_blowup"], work["seed"]
     L = max(logs) + b

@@ -62,6 +62,13 @@ int sh_timer_stop(sh_ctx *ctx, float *ms);
  * (crates/stwo/src/prover/backend/mod.rs:45-66). */
 int sh_alloc(sh_ctx *ctx, uint64_t bytes, void **d_out);
 int sh_free(sh_ctx *ctx, void *d);
+/* sh_alloc/sh_free go through a per-context pool (BaseColumnPool,
+ * crates/stwo/src/prover/mempool.rs:14-58): freed blocks are reused by later
+ * allocations of the same rounded size, stream-ordered, without a device sync.
+ * A buffer still read by work enqueued on this context may be freed; one that
+ * another context or the host is using asynchronously must not be.
+ * sh_pool_trim releases every cached block (synchronising the context first). */
+int sh_pool_trim(sh_ctx *ctx);
 int sh_upload(sh_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
 int sh_download(sh_ctx *ctx, void *h_dst, const void *d_src, uint64_t bytes);
 int sh_memset(sh_ctx *ctx, void *d_dst, int value, uint64_t bytes);
@@ -171,6 +178,12 @@ int sh_commit_trace(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d
  * polynomial with 2^log_size coefficients at a QM31 circle point (8 words). */
 int sh_eval_at_point(sh_ctx *ctx, const uint32_t *d_coeffs, uint32_t log_size, const uint32_t point[8],
                      uint32_t out[4]);
+/* eval_at_point for many polynomials at many points in one call (compute_samples,
+ * crates/stwo/src/prover/pcs/mod.rs:176-222, evaluates every column at its sampled points):
+ * column i (2^log_sizes[i] coefficients) at point j (points[8j..8j+8]) goes to
+ * out[4 * (i * n_points + j) ..] (host).  Each column is read once per 4 points. */
+int sh_eval_at_points(sh_ctx *ctx, const uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n_cols,
+                      const uint32_t *points, uint32_t n_points, uint32_t *out);
 /* PolyOps::barycentric_weights_into (crates/stwo/src/prover/poly/circle/ops.rs:50-62;
  * SIMD crates/stwo/src/prover/backend/simd/circle.rs:165-300): weights for
  * CanonicCoset(log_size) and the point, written to 4 coordinate arrays. */

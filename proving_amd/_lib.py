@@ -31,6 +31,7 @@ SIGNATURES = {
     "sh_download": (c_int, [c_vp, c_vp, c_vp, c_u64]),
     "sh_memset": (c_int, [c_vp, c_vp, c_int, c_u64]),
     "sh_copy": (c_int, [c_vp, c_vp, c_vp, c_u64]),
+    "sh_pool_trim": (c_int, [c_vp]),
     "sh_bit_reverse_column": (c_int, [c_vp, c_vp, c_u32]),
     "sh_twiddles_create": (c_int, [c_vp, c_u32, c_u32, ctypes.POINTER(c_vp)]),
     "sh_twiddles_destroy": (c_int, [c_vp, c_vp]),
@@ -40,6 +41,7 @@ SIGNATURES = {
     "sh_evaluate": (c_int, [c_vp, c_vp, c_vp, c_u32, c_vp, c_u32]),
     "sh_evaluate_polynomials": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_split_at_mid": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
+    "sh_eval_at_points": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_u32, c_vp]),
     "sh_wide_fibonacci_composition": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
     "sh_merkle_build_leaves": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_merkle_build_next_layer": (c_int, [c_vp, c_vp, c_u32, c_vp]),

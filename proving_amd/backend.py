@@ -93,6 +93,13 @@ class DeviceBuffer:
             self.ctx.lib.sh_free(self.ctx.h, ctypes.c_void_p(self.ptr))
         self.ptr = None
 
+    def __del__(self):
+        # back to the context's pool (stream-ordered reuse, no device sync)
+        try:
+            self.free()
+        except Exception:
+            pass
+
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
         assert arr.nbytes <= self.nbytes
@@ -314,7 +321,9 @@ class HipBackend:
         self._chk(self.lib.sh_commit_trace_graph_create(
             self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s([e.log_size for e in evals]), len(evals),
             log_blowup, _ptrs(ldes), _ptrs(layers), ctypes.byref(h)), "commit_trace_graph_create")
-        return CommitGraph(self.ctx, h, layers)
+        g = CommitGraph(self.ctx, h, layers)
+        g.buffers = (list(evals), list(coeffs), list(ldes), twiddles)  # the graph reads them on every launch
+        return g
 
     # ---- pass-shape tuning (no reference counterpart) --------------------------------------------
     def autotune_fft(self, log_sizes, log_blowup, twiddles):
@@ -366,6 +375,16 @@ class _Ops:
         pt = _u32s(list(point))
         self._chk(self.lib.sh_eval_at_point(self.h, coeffs.ptr, coeffs.log_size, pt, out), "eval_at_point")
         return np.array(out, np.uint32)
+
+    def eval_at_points(self, coeffs, points):
+        """eval_at_point of every polynomial at every point, one batched call; returns
+        out[i][j] = coeffs[i] at points[j] (QM31 as 4 ints)."""
+        n, k = len(coeffs), len(points)
+        out = (ctypes.c_uint32 * max(1, 4 * n * k))()
+        flat = [int(x) for p in points for x in p]
+        self._chk(self.lib.sh_eval_at_points(self.h, _ptrs(coeffs), _u32s([c.log_size for c in coeffs]), n,
+                                             _u32s(flat), k, out), "eval_at_points")
+        return [[[int(out[4 * (i * k + j) + q]) for q in range(4)] for j in range(k)] for i in range(n)]
 
     def barycentric_weights(self, log_size, point):
         w = SecureColumnByCoords(self.ctx, log_size)

@@ -42,6 +42,13 @@ struct sh_ctx {
   sh_scratch scratch[8];
   hipEvent_t timer_start = nullptr, timer_stop = nullptr;
   std::vector<hipEvent_t> sync_events;  // fork/join events (no timing)
+  // caching device allocator (BaseColumnPool's role, crates/stwo/src/prover/mempool.rs:14-58):
+  // freed blocks are kept per rounded size and handed out again; reuse is safe
+  // without a sync because every entry point orders its work on the main
+  // stream (auxiliary streams are joined back before a call returns)
+  std::unordered_map<uint64_t, std::vector<void*>> pool;
+  std::unordered_map<void*, uint64_t> pool_live;
+  uint64_t pool_bytes = 0;
   // FFT pass shape per transform log size (see fft.hip)
   struct FftTune {
     uint8_t block_log, stride_layers, tile_log, max_stage;
@@ -89,6 +96,9 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev);
 hipEvent_t sh_sync_event(sh_ctx* ctx, size_t i);
 // Ensures scratch buffer `slot` holds at least `bytes` and returns it.
 int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out);
+// Pooled device allocation for buffers that outlive one launch (see sh_ctx::pool).
+int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out);
+void sh_dev_free(sh_ctx* ctx, void* p);
 // Profiling spans around launches.
 void sh_prof_begin(sh_ctx* ctx, const char* cls, sh_prof_rec* rec);
 void sh_prof_end(sh_ctx* ctx, sh_prof_rec* rec);

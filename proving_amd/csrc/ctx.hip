@@ -71,10 +71,13 @@ extern "C" int sh_ctx_destroy(sh_ctx* ctx) {
   }
   for (auto e : ctx->event_pool) hipEventDestroy(e);
   for (auto e : ctx->sync_events) hipEventDestroy(e);
-  for (auto& s : ctx->scratch)
-    if (s.ptr) hipFree(s.ptr);
   for (auto a : ctx->aux)
     if (a) hipStreamSynchronize(a);
+  for (auto& s : ctx->scratch)
+    if (s.ptr) hipFree(s.ptr);
+  for (auto& kv : ctx->pool)
+    for (void* p : kv.second) hipFree(p);
+  for (auto& kv : ctx->pool_live) hipFree(kv.first);
   if (ctx->d_ring) hipFree(ctx->d_ring);
   if (ctx->h_ring) hipHostFree(ctx->h_ring);
   if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
@@ -239,24 +242,73 @@ extern "C" int sh_timer_stop(sh_ctx* ctx, float* ms) {
   return SH_OK;
 }
 
+static constexpr uint64_t kPoolCapBytes = 64ull << 30;  // cached bytes kept for reuse
+
+static int pool_release(sh_ctx* ctx) {
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
+  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
+  for (auto& kv : ctx->pool)
+    for (void* p : kv.second) SH_CHECK_HIP(ctx, hipFree(p));
+  ctx->pool.clear();
+  ctx->pool_bytes = 0;
+  return SH_OK;
+}
+
+int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
+  *out = nullptr;
+  const uint64_t cls = ((bytes ? bytes : 4) + 255) & ~255ull;
+  auto it = ctx->pool.find(cls);
+  if (it != ctx->pool.end() && !it->second.empty()) {
+    *out = it->second.back();
+    it->second.pop_back();
+    ctx->pool_bytes -= cls;
+  } else if (hipMalloc(out, cls) != hipSuccess) {
+    (void)hipGetLastError();
+    SH_TRY(pool_release(ctx));  // cached blocks of other sizes may be what is missing
+    if (hipMalloc(out, cls) != hipSuccess) {
+      *out = nullptr;
+      return sh_fail(ctx, SH_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)cls);
+    }
+  }
+  ctx->pool_live[*out] = cls;
+  return SH_OK;
+}
+
+void sh_dev_free(sh_ctx* ctx, void* p) {
+  if (!p) return;
+  auto it = ctx->pool_live.find(p);
+  if (it == ctx->pool_live.end()) {  // not ours: free it the plain way
+    hipStreamSynchronize(ctx->main_stream);
+    for (auto a : ctx->aux) hipStreamSynchronize(a);
+    hipFree(p);
+    return;
+  }
+  const uint64_t cls = it->second;
+  ctx->pool_live.erase(it);
+  if (ctx->pool_bytes + cls > kPoolCapBytes) {
+    hipStreamSynchronize(ctx->main_stream);
+    for (auto a : ctx->aux) hipStreamSynchronize(a);
+    hipFree(p);
+    return;
+  }
+  ctx->pool[cls].push_back(p);
+  ctx->pool_bytes += cls;
+}
+
 extern "C" int sh_alloc(sh_ctx* ctx, uint64_t bytes, void** d_out) {
   if (!ctx || !d_out) return SH_ERR_INVALID;
-  *d_out = nullptr;
-  if (bytes == 0) bytes = 4;
-  if (hipMalloc(d_out, bytes) != hipSuccess) {
-    *d_out = nullptr;
-    return sh_fail(ctx, SH_ERR_OOM, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
-  }
-  return SH_OK;
+  return sh_dev_alloc(ctx, bytes, d_out);
 }
 
 extern "C" int sh_free(sh_ctx* ctx, void* d) {
   if (!ctx) return SH_ERR_INVALID;
-  if (!d) return SH_OK;
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
-  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
-  SH_CHECK_HIP(ctx, hipFree(d));
+  sh_dev_free(ctx, d);
   return SH_OK;
+}
+
+extern "C" int sh_pool_trim(sh_ctx* ctx) {
+  if (!ctx) return SH_ERR_INVALID;
+  return pool_release(ctx);
 }
 
 extern "C" int sh_upload(sh_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes) {

@@ -102,11 +102,12 @@ int merkle_decommit(sh_ctx* ctx, const std::vector<const uint8_t*>& layers, uint
 // A lifted tree committed on the device, with its layer buffers.
 struct DeviceTree {
   uint32_t L = 0;
-  uint8_t* buf = nullptr;  // all layers, contiguous
+  sh_ctx* owner = nullptr;
+  uint8_t* buf = nullptr;  // all layers, contiguous (pooled)
   std::vector<uint8_t*> layers;
   Hash root{};
   void free_all() {
-    if (buf) hipFree(buf);
+    if (buf) sh_dev_free(owner, buf);
     buf = nullptr;
   }
 };
@@ -114,7 +115,8 @@ struct DeviceTree {
 int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t) {
   t.L = log_size;
   uint64_t total = 32ull * ((2ull << log_size) - 1);
-  if (hipMalloc(&t.buf, total) != hipSuccess) return sh_fail(ctx, SH_ERR_OOM, "tree allocation failed");
+  t.owner = ctx;
+  SH_TRY(sh_dev_alloc(ctx, total, (void**)&t.buf));
   t.layers.resize(log_size + 1);
   uint64_t off = 0;
   for (uint32_t k = 0; k <= log_size; k++) {
@@ -165,11 +167,12 @@ struct LayerProof {
 // A committed FRI layer: its evaluation (4 device arrays) and Merkle tree.
 struct FriLayer {
   uint32_t log_size = 0;
-  uint32_t* buf = nullptr;  // 4 * 2^log_size words
+  sh_ctx* owner = nullptr;
+  uint32_t* buf = nullptr;  // 4 * 2^log_size words (pooled)
   const uint32_t* cols[4] = {nullptr, nullptr, nullptr, nullptr};
   DeviceTree tree;
   void free_all() {
-    if (buf) hipFree(buf);
+    if (buf) sh_dev_free(owner, buf);
     buf = nullptr;
     tree.free_all();
   }
@@ -354,7 +357,9 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   // commit_inner_layers (fri.rs:145-196)
   auto alloc_eval = [&](FriLayer& l, uint32_t lg) -> int {
     l.log_size = lg;
-    if (hipMalloc(&l.buf, 16ull << lg) != hipSuccess) return sh_fail(ctx, SH_ERR_OOM, "FRI layer allocation");
+    l.owner = ctx;
+    int r = sh_dev_alloc(ctx, 16ull << lg, (void**)&l.buf);
+    if (r != SH_OK) return r;
     for (int c = 0; c < 4; c++) l.cols[c] = l.buf + ((uint64_t)c << lg);
     return SH_OK;
   };
@@ -503,7 +508,7 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
   int rc = SH_OK;
   auto free_accs = [&]() {
     for (auto& a : accs)
-      if (a.buf) hipFree(a.buf);
+      if (a.buf) sh_dev_free(ctx, a.buf);
   };
   for (uint32_t lg : sizes) {
     std::vector<const uint32_t*> gcols;
@@ -548,9 +553,9 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
     std::vector<uint32_t*> outs(4 * pts.size());
     for (size_t k = 0; k < pts.size(); k++) {
       uint32_t* buf = nullptr;
-      if (hipMalloc(&buf, 16ull << plog) != hipSuccess) {
+      if ((rc = sh_dev_alloc(ctx, 16ull << plog, (void**)&buf)) != SH_OK) {
         free_accs();
-        return sh_fail(ctx, SH_ERR_OOM, "partial numerator allocation");
+        return rc;
       }
       accs.push_back({pts[k], plog, buf, sh::qzero()});
       for (int c = 0; c < 4; c++) outs[4 * k + c] = buf + ((uint64_t)c << plog);
@@ -593,10 +598,7 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
       for (int c = 0; c < 4; c++) pcols[c] = top.buf + ((uint64_t)c << top.log);
     } else {
       uint32_t* buf = nullptr;
-      if (hipMalloc(&buf, 16ull << plog) != hipSuccess) {
-        rc = sh_fail(ctx, SH_ERR_OOM, "lift allocation");
-        break;
-      }
+      if ((rc = sh_dev_alloc(ctx, 16ull << plog, (void**)&buf)) != SH_OK) break;
       lifted.push_back(buf);
       uint32_t* o[4];
       for (int c = 0; c < 4; c++) o[c] = buf + ((uint64_t)c << plog);
@@ -615,11 +617,8 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
   if (rc == SH_OK)
     rc = sh_compute_quotients_and_combine(ctx, tw, parts.data(), plogs.data(), ppts.data(), pfls.data(), plogs.size(),
                                           L, log_blowup, d_out);
-  if (rc == SH_OK) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) rc = sh_fail(ctx, SH_ERR_HIP, "quotients: %s", hipGetErrorString(e));
-  }
-  for (auto b : lifted) hipFree(b);
+  // the temporaries go back to the context pool: later users are ordered after this work
+  for (auto b : lifted) sh_dev_free(ctx, b);
   free_accs();
   return rc;
 }

@@ -4,38 +4,94 @@
 // eval_at_point (CpuBackend, crates/stwo/src/prover/backend/cpu/circle.rs:81-98):
 // value = sum_i c_i * prod_{bit k of i} f_k with f_0 = p.y, f_1 = p.x,
 // f_k = pi^(k-1)(p.x) (the fold of crates/stwo/src/core/poly/utils.rs:27-38).
-// The device folds 4 index bits per pass: lane t combines 16 consecutive
-// values with the next four factors, so a 2^n polynomial takes ceil(n/4) passes.
+// Batched on the device: the basis value of index i = hi * 2^s + lo factors as
+// B_hi[hi] * B_lo[lo] (products of the factors of the high and the low bits), so
+// with B_lo (2^s words) and B_hi (2^(n-s)) tabulated per point on the host, lane
+// lo of a block sums c[hi * 2^s + lo] * B_hi[hi] over a range of hi (4 lazy
+// 32x32->64 multiply-adds per coefficient and point; B_hi is uniform across the
+// block), multiplies by B_lo[lo] once, and the block's sum is added into 64-bit
+// per-(column, point) accumulators.  One launch per column size covers every
+// column of that size and up to 4 points, reading each coefficient once.
 //
 // Barycentric weights (cpu/circle.rs:100-131), for the bit-reversed domain point q_i:
 //   W_i = V_n(p) / (-2 q_i.y * V'_n(q_i) * (h.y / (1 + h.x))),  h = p - q_i,
 // with V'_n the coset-vanishing derivative (crates/stwo/src/core/constraints.rs:37-55).
 // q_i has M31 coordinates, so V'_n(q_i) is evaluated in M31.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
 
-struct Factors {
-  sh::qm31 f[4];
-};
+constexpr uint32_t kEvalLoLog = 8;   // 256 lanes: the low 8 index bits
+constexpr uint32_t kEvalHiPerBlock = 64;
+constexpr uint32_t kEvalMaxPoints = 4;
 
-// in: M31 (first pass, in_m31 = 1) or QM31 AoS (4 words); out: QM31 AoS.
-__global__ __launch_bounds__(256) void k_fold_pass(const uint32_t* __restrict__ in, int in_m31, uint32_t n_in,
-                                                   uint32_t g, Factors fac, uint32_t* __restrict__ out) {
-  const uint32_t n_out = n_in >> g;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_out) return;
-  sh::qm31 v[16];
-  const uint32_t cnt = 1u << g;
-  for (uint32_t j = 0; j < cnt; j++) {
-    uint32_t idx = (t << g) + j;
-    v[j] = in_m31 ? sh::qfromm(in[idx]) : sh::qload(in + 4 * idx);
+__device__ __forceinline__ void lfold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
+__device__ __forceinline__ uint32_t lfinal(uint64_t v) {
+  lfold(v);
+  lfold(v);
+  return sh::reduce_once((uint32_t)v);
+}
+
+// blockIdx.y = column (table of pointers), blockIdx.x = range of hi; NP points in one pass
+// over the coefficients.  b_lo: [lo][pt] QM31, b_hi: [hi][pt] QM31; acc: [col][pt][4] u64
+// (zeroed by the caller).
+template <int NP>
+__global__ __launch_bounds__(256) void k_eval_points(const uint32_t* const* __restrict__ cols, uint32_t log_n,
+                                                     const uint32_t* __restrict__ b_lo,
+                                                     const uint32_t* __restrict__ b_hi,
+                                                     unsigned long long* __restrict__ acc) {
+  __shared__ uint32_t red[256][4];
+  const uint32_t s = log_n < kEvalLoLog ? log_n : kEvalLoLog;
+  const uint32_t lo = threadIdx.x;
+  const uint32_t n_hi = 1u << (log_n - s);
+  const uint32_t h0 = blockIdx.x * kEvalHiPerBlock;
+  const uint32_t h1 = h0 + kEvalHiPerBlock < n_hi ? h0 + kEvalHiPerBlock : n_hi;
+  const uint32_t* __restrict__ c = cols[blockIdx.y];
+  const bool live = lo < (1u << s);
+  uint64_t a[NP][4];
+#pragma unroll
+  for (int pt = 0; pt < NP; pt++) a[pt][0] = a[pt][1] = a[pt][2] = a[pt][3] = 0;
+  if (live) {
+    uint32_t k = 0;
+    for (uint32_t hi = h0; hi < h1; hi++) {
+      const uint32_t v = c[(hi << s) + lo];
+#pragma unroll
+      for (int pt = 0; pt < NP; pt++) {
+        const uint4 bh = *(const uint4*)&b_hi[4 * (hi * NP + pt)];
+        a[pt][0] += (uint64_t)v * bh.x;
+        a[pt][1] += (uint64_t)v * bh.y;
+        a[pt][2] += (uint64_t)v * bh.z;
+        a[pt][3] += (uint64_t)v * bh.w;
+      }
+      if (++k == 3) {  // three products < 2^62 on a folded value < 2^34 stay below 2^64
+        k = 0;
+#pragma unroll
+        for (int pt = 0; pt < NP; pt++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) lfold(a[pt][q]);
+      }
+    }
   }
-  for (uint32_t k = 0; k < g; k++) {
-    uint32_t half = cnt >> (k + 1);
-    for (uint32_t j = 0; j < half; j++) v[j] = sh::qadd(v[2 * j], sh::qmul(v[2 * j + 1], fac.f[k]));
+#pragma unroll
+  for (int pt = 0; pt < NP; pt++) {
+    sh::qm31 part = {{lfinal(a[pt][0]), lfinal(a[pt][1])}, {lfinal(a[pt][2]), lfinal(a[pt][3])}};
+    part = live ? sh::qmul(part, sh::qload(b_lo + 4 * (lo * NP + pt))) : sh::qzero();
+    red[lo][0] = part.a.a;
+    red[lo][1] = part.a.b;
+    red[lo][2] = part.b.a;
+    red[lo][3] = part.b.b;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+      if (lo < h)
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[lo][q] = sh::madd(red[lo][q], red[lo + h][q]);
+      __syncthreads();
+    }
+    if (lo < 4) atomicAdd(&acc[4 * ((uint64_t)blockIdx.y * NP + pt) + lo], (unsigned long long)red[0][lo]);
+    __syncthreads();
   }
-  sh::qstore(v[0], out + 4 * t);
 }
 
 struct BaryConsts {
@@ -68,13 +124,6 @@ __global__ __launch_bounds__(256) void k_bary_weights(BaryConsts bc, sh_point_ta
   w[1][i] = wv.a.b;
   w[2][i] = wv.b.a;
   w[3][i] = wv.b.b;
-}
-
-__device__ __forceinline__ void lfold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
-__device__ __forceinline__ uint32_t lfinal(uint64_t v) {
-  lfold(v);
-  lfold(v);
-  return sh::reduce_once((uint32_t)v);
 }
 
 // partial[block] = sum_i w_i * e_i over a grid-stride range (lazy 64-bit accumulation).
@@ -127,50 +176,81 @@ sh::qm31 coset_vanishing_host(uint32_t init_index, uint32_t step_index, uint32_t
 
 }  // namespace
 
+// Many columns (any sizes) at many points: out[(col * n_points + pt) * 4 ..] (host).
+extern "C" int sh_eval_at_points(sh_ctx* ctx, const uint32_t* const* d_coeffs, const uint32_t* log_sizes,
+                                 uint64_t n_cols, const uint32_t* points, uint32_t n_points, uint32_t* out) {
+  if (!ctx || (n_cols && (!d_coeffs || !log_sizes || !out)) || (n_points && !points)) return SH_ERR_INVALID;
+  if (n_cols == 0 || n_points == 0) return SH_OK;
+  for (uint64_t i = 0; i < n_cols; i++) SH_REQUIRE(ctx, log_sizes[i] <= 30, "log size %u too large", log_sizes[i]);
+  std::vector<uint32_t> classes(log_sizes, log_sizes + n_cols);
+  std::sort(classes.begin(), classes.end());
+  classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
+  SpanGuard span(ctx, "eval_at_point");
+  for (uint32_t p0 = 0; p0 < n_points; p0 += kEvalMaxPoints) {
+    const uint32_t np = std::min(kEvalMaxPoints, n_points - p0);
+    for (uint32_t n : classes) {
+      // factor of index bit k: bit 0 -> y, bit 1 -> x, bit k -> pi^(k-1)(x)
+      const uint32_t sl = std::min(n, kEvalLoLog), nh = n - sl;
+      std::vector<uint32_t> blo(4ull * np << sl), bhi(4ull * np << nh);
+      for (uint32_t pt = 0; pt < np; pt++) {
+        const uint32_t* P8 = points + 8ull * (p0 + pt);
+        std::vector<sh::qm31> f(n);
+        sh::qm31 x = sh::qload(P8);
+        for (uint32_t k = 0; k < n; k++) {
+          if (k == 0) {
+            f[0] = sh::qload(P8 + 4);
+          } else {
+            f[k] = x;
+            x = sh::qdouble_x(x);
+          }
+        }
+        auto fill = [&](std::vector<uint32_t>& t, uint32_t bits, uint32_t first) {
+          sh::qstore(sh::qone(), &t[4ull * pt]);
+          for (uint32_t j = 0; j < bits; j++)
+            for (uint64_t i = 1ull << j; i < (2ull << j); i++)
+              sh::qstore(sh::qmul(sh::qload(&t[4ull * ((i - (1ull << j)) * np + pt)]), f[first + j]),
+                         &t[4ull * (i * np + pt)]);
+        };
+        fill(blo, sl, 0);
+        fill(bhi, nh, sl);
+      }
+      std::vector<const void*> ptrs;
+      std::vector<uint64_t> idx;
+      for (uint64_t i = 0; i < n_cols; i++)
+        if (log_sizes[i] == n) {
+          ptrs.push_back(d_coeffs[i]);
+          idx.push_back(i);
+        }
+      // columns of this class may not be contiguous in the output: accumulate into a class slice
+      void *d_ptrs, *d_lo, *d_hi, *d_cls;
+      SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_ptrs));
+      SH_TRY(sh_stage(ctx, blo.data(), 4 * blo.size(), &d_lo));
+      SH_TRY(sh_stage(ctx, bhi.data(), 4 * bhi.size(), &d_hi));
+      SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 32ull * ptrs.size() * np, &d_cls));
+      SH_TRY(sh_memset(ctx, d_cls, 0, 32ull * ptrs.size() * np));
+      dim3 grid((unsigned)(((1ull << nh) + kEvalHiPerBlock - 1) / kEvalHiPerBlock), (unsigned)ptrs.size());
+      auto kern = np == 1 ? k_eval_points<1> : np == 2 ? k_eval_points<2> : np == 3 ? k_eval_points<3>
+                                                                              : k_eval_points<4>;
+      kern<<<grid, 256, 0, ctx->stream>>>((const uint32_t* const*)d_ptrs, n, (const uint32_t*)d_lo,
+                                          (const uint32_t*)d_hi, (unsigned long long*)d_cls);
+      SH_TRY(sh_launch_check(ctx, "eval_at_points"));
+      std::vector<unsigned long long> cls(4ull * ptrs.size() * np);
+      SH_CHECK_HIP(ctx, hipMemcpyAsync(cls.data(), d_cls, 8 * cls.size(), hipMemcpyDeviceToHost, ctx->stream));
+      SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      for (size_t c = 0; c < idx.size(); c++)
+        for (uint32_t pt = 0; pt < np; pt++)
+          for (int q = 0; q < 4; q++)
+            out[4 * (idx[c] * n_points + p0 + pt) + q] = (uint32_t)(cls[4 * (c * np + pt) + q] % sh::P);
+    }
+  }
+  return SH_OK;
+}
+
 extern "C" int sh_eval_at_point(sh_ctx* ctx, const uint32_t* d_coeffs, uint32_t log_size, const uint32_t point[8],
                                 uint32_t out[4]) {
   if (!ctx || !d_coeffs || !point || !out) return SH_ERR_INVALID;
-  SH_REQUIRE(ctx, log_size <= 30, "log size %u too large", log_size);
-  sh::qpt p = {sh::qload(point), sh::qload(point + 4)};
-  if (log_size == 0) {
-    uint32_t c0;
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(&c0, d_coeffs, 4, hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    sh::qstore(sh::qfromm(c0), out);
-    return SH_OK;
-  }
-  // factor for index bit k: bit 0 -> y, bit 1 -> x, bit k -> pi^(k-1)(x)
-  std::vector<sh::qm31> f(log_size);
-  f[0] = p.y;
-  sh::qm31 x = p.x;
-  for (uint32_t k = 1; k < log_size; k++) {
-    f[k] = x;
-    x = sh::qdouble_x(x);
-  }
-  uint64_t n = 1ull << log_size;
-  void* scr;
-  SH_TRY(sh_scratch_get(ctx, SCR_MISC, 16ull * ((n + 15) / 16) * 2 + 64, &scr));
-  uint32_t* bufs[2] = {(uint32_t*)scr, (uint32_t*)scr + 4 * ((n + 15) / 16)};
-  const uint32_t* in = d_coeffs;
-  int in_m31 = 1;
-  uint32_t n_in = (uint32_t)n, bit = 0, which = 0;
-  SpanGuard span(ctx, "eval_at_point");
-  while (n_in > 1) {
-    uint32_t g = std::min<uint32_t>(4, log_size - bit);
-    Factors fac;
-    for (uint32_t k = 0; k < g; k++) fac.f[k] = f[bit + k];
-    uint32_t n_out = n_in >> g;
-    k_fold_pass<<<(n_out + 255) / 256, 256, 0, ctx->stream>>>(in, in_m31, n_in, g, fac, bufs[which]);
-    SH_TRY(sh_launch_check(ctx, "eval_at_point fold"));
-    in = bufs[which];
-    in_m31 = 0;
-    which ^= 1;
-    n_in = n_out;
-    bit += g;
-  }
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(out, in, 16, hipMemcpyDeviceToHost, ctx->stream));
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return SH_OK;
+  const uint32_t* c = d_coeffs;
+  return sh_eval_at_points(ctx, &c, &log_size, 1, point, 1, out);
 }
 
 extern "C" int sh_barycentric_weights(sh_ctx* ctx, uint32_t log_size, const uint32_t point[8], uint32_t* const d_w[4]) {

@@ -112,12 +112,24 @@ class CommitmentSchemeProver:
         b = self.fri["log_blowup_factor"]
         # compute_samples (crates/stwo/src/prover/pcs/mod.rs:176-222): coefficient evaluation at the
         # point doubled down to the column's own domain
+        # one batched evaluation per distinct (doubled) point over all columns that sample it
+        jobs = {}
+        for ti, (tree, pts_tree) in enumerate(zip(self.trees, sampled_points)):
+            for ci, (c, e, pts) in enumerate(zip(tree.coeffs, tree.evals, pts_tree)):
+                for pi, p in enumerate(pts):
+                    q = tuple(F.qpoint_repeated_double(p, L - e.log_size))
+                    jobs.setdefault(q, []).append((ti, ci, pi, c))
+        values = {}
+        for q, items in jobs.items():
+            res = self.be.eval_at_points([it[3] for it in items], [list(q)])
+            for it, r in zip(items, res):
+                values[it[:3]] = r[0]
         sampled_values = []
         flat_cols, flat_samples = [], []
-        for tree, pts_tree in zip(self.trees, sampled_points):
+        for ti, (tree, pts_tree) in enumerate(zip(self.trees, sampled_points)):
             vals_tree = []
-            for c, e, pts in zip(tree.coeffs, tree.evals, pts_tree):
-                vals = [list(self.be.eval_at_point(c, F.qpoint_repeated_double(p, L - e.log_size))) for p in pts]
+            for ci, (e, pts) in enumerate(zip(tree.evals, pts_tree)):
+                vals = [values[(ti, ci, pi)] for pi in range(len(pts))]
                 vals_tree.append(vals)
                 flat_cols.append(e)
                 flat_samples.append([(list(p), v) for p, v in zip(pts, vals)])

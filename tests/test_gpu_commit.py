@@ -75,3 +75,24 @@ def test_autotune_keeps_results(ctx, backend):
     backend.autotune_fft(logs, 1, tree)
     _, _, _, r2 = backend.commit_trace(evals, 1, tree, coeffs=coeffs, ldes=ldes, layers=layers)
     assert r1 == root and r2 == root
+
+
+def test_pool_reuses_freed_buffers(ctx, backend):
+    """sh_free returns a block to the context pool; the next allocation of that size gets it back
+    (no device sync), and work through reused blocks is unchanged."""
+    from proving_amd.backend import BaseColumn as BC
+    a = BC(ctx, 16)
+    pa = a.ptr
+    a.free()
+    b = BC(ctx, 16)
+    assert b.ptr == pa
+    logs = [9] * 4 + [12] * 2
+    host, tree, evals, coeffs, ldes, layers, root = run_commit(ctx, backend, logs, 4)
+    for c in coeffs + ldes:
+        c.free()
+    for _ in range(2):
+        _, _, _, r2 = backend.commit_trace(evals, 1, tree)
+        assert r2 == root
+    ctx.check(ctx.lib.sh_pool_trim(ctx.h), "pool_trim")
+    _, _, _, r3 = backend.commit_trace(evals, 1, tree)
+    assert r3 == root

@@ -212,3 +212,19 @@ def test_grind(ctx, backend, golden):
     n20 = backend.grind(d, 20)
     assert O.verify_pow_nonce(d, 20, n20)
     assert backend.grind(d, 0) == 0
+
+
+@pytest.mark.parametrize("n_pts", [1, 3, 4, 6])
+def test_eval_at_points_batched(ctx, backend, n_pts):
+    """Many polynomials of mixed sizes at many points in one call == eval_at_point of each."""
+    rng = np.random.default_rng(70 + n_pts)
+    logs = [0, 1, 3, 8, 9, 12, 9, 16, 3]
+    polys = [rng.integers(0, P, 1 << lg, dtype=np.uint32) for lg in logs]
+    ch = O.Channel()
+    ch.mix_u64(n_pts)
+    pts = [[int(v) for v in ch.random_point()] for _ in range(n_pts)]
+    cols = [BaseColumn.from_cpu(ctx, p) for p in polys]
+    got = backend.eval_at_points(cols, pts)
+    for i, p in enumerate(polys):
+        for j, pt in enumerate(pts):
+            assert got[i][j] == [int(v) for v in O.eval_at_point(p, pt)], (logs[i], j)

@@ -208,3 +208,41 @@ def test_deep_fri_chain_on_device(ctx, backend):
     chv.s.digest[:] = list(ch_before)
     chv.s.n_draws = 0
     assert PR.fri_verify(PR.deserialize_fri_proof(felts), pd, chv, L - b, fle) == qpos
+
+
+def test_fri_deep_phase_full_size_verifies(ctx, backend):
+    """The FRI + DEEP phase at 200 columns x 2^21 (LDE 2^22), 3 OOD points, 32 queries: the
+    device proof is accepted by the oracle FRI verifier with first-layer answers rebuilt from
+    values gathered on the device."""
+    from proving_amd.backend import HashLayer
+    from proving_amd.workloads import FRI_DEEP, splitmix64_columns
+    w = FRI_DEEP
+    n, N, b = w["log_size"], w["n_cols"], w["fri"]["log_blowup_factor"]
+    L = n + b
+    tree = backend.precompute_canonic_twiddles(L)
+    host = splitmix64_columns(w["seed"], [n] * N)
+    evals = [BaseColumn.from_cpu(ctx, c) for c in host]
+    del host
+    coeffs = [BaseColumn(ctx, n) for _ in range(N)]
+    ldes = [BaseColumn(ctx, L) for _ in range(N)]
+    backend.commit_trace(evals, b, tree, coeffs, ldes, [HashLayer(ctx, k) for k in range(L + 1)])
+    ch = Channel()
+    ch.mix_u64(w["seed"])
+    points = [[int(x) for x in ch.draw_random_point()] for _ in range(w["n_points"])]
+    samples = [[(p, [int(v) for v in backend.eval_at_point(c, p)]) for p in points] for c in coeffs]
+    ch.mix_felts([v for col in samples for _, v in col])
+    before = ch.digest
+    rc = [int(x) for x in ch.draw_secure_felt()]
+    quot = fri_quotients(backend, tree, ldes, samples, rc, L, b)
+    felts, qpos = fri_prove(backend, tree, ch, params_of(w["fri"]), quot)
+    assert len(qpos) <= 32
+
+    def fle(qs):
+        rows = [gather(backend, [e] * len(qs), qs) for e in ldes]
+        return PR.fri_answers([L] * N, samples, rc, qs, rows, L)
+
+    chv = O.Channel()
+    chv.s.digest[:] = list(before)
+    chv.s.n_draws = 0
+    assert [int(v) for v in chv.draw_secure_felt()] == rc
+    assert PR.fri_verify(PR.deserialize_fri_proof(felts), w["fri"], chv, L - b, fle) == qpos
