@@ -246,15 +246,69 @@ def bench_prove(args):
     ctx.close()
 
 
+def bench_serve(args):
+    """Proof serving on one GPU: requests with 32/64/128/256 columns of 2^14..2^20 rows (seed 4),
+    Poisson arrivals at 80% of the capacity measured by back-to-back plain proving; the server
+    (continuous batching: admission by HBM budget, cross-request trace commits, 3 worker
+    contexts) against the plain sequential drop-in on the same arrivals."""
+    import numpy as np
+    from proving_amd.backend import Context, HipBackend
+    from proving_amd.serving import ProofServer, make_requests, prove_sequential
+    fri = {"pow_bits": 10, "log_blowup_factor": 1, "log_last_layer_degree_bound": 0, "n_queries": 70,
+           "fold_step": 1}
+    n = args.requests
+    ctx = Context(0)
+    be = HipBackend(ctx)
+    tw = be.precompute_canonic_twiddles(21)
+    log(f"[bench] serve: {n} requests; warm-up and capacity (back-to-back plain proving)")
+    prove_sequential(be, tw, make_requests(4, 99), fri, realtime=False)  # warm-up
+    reqs = make_requests(n, 4)
+    t0 = time.perf_counter()
+    prove_sequential(be, tw, reqs, fri, realtime=False)
+    capacity = n / (time.perf_counter() - t0)
+    rate = 0.8 * capacity
+    log(f"[bench] capacity {capacity:.1f} proofs/s; arrivals at {rate:.1f}/s")
+    seq = make_requests(n, 4, rate=rate)
+    prove_sequential(be, tw, seq, fri, realtime=True)
+    srv_reqs = make_requests(n, 4, rate=rate)
+    server = ProofServer(21, fri, n_workers=3, hbm_budget=128 << 30)
+    try:
+        server.serve(srv_reqs, realtime=True)
+    finally:
+        server.close()
+    same = all(a.felts == b.felts for a, b in zip(seq, srv_reqs))
+
+    def stats(rs):
+        lat = np.array([r.done - r.arrival for r in rs])
+        span = max(r.done for r in rs) - min(r.arrival for r in rs)
+        return {"proofs_per_s": round(len(rs) / span, 2), "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 1),
+                "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 1)}
+
+    s_srv, s_seq = stats(srv_reqs), stats(seq)
+    out = {"metric": "proof-serving throughput (wide-Fibonacci requests, Poisson arrivals at 80% capacity)",
+           "value": s_srv["proofs_per_s"], "unit": "proofs/s", "n_gpus": 1, "steps": n, "warmup": 4,
+           "ms_per_step": round(1e3 / s_srv["proofs_per_s"], 2), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32 (M31/QM31)", "data": "synthetic",
+           "config": {"workload": "serve_wide_fib", "requests": n, "columns": [32, 64, 128, 256],
+                      "log_rows": [14, 20], "seed": 4, "arrival_rate": round(rate, 2), "fri": fri,
+                      "workers": 3},
+           "server": s_srv, "plain_sequential": s_seq, "capacity_plain_back_to_back": round(capacity, 2),
+           "proofs_identical_to_plain": same}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["uneven", "plain", "fri_deep", "prove"], default="uneven",
+    ap.add_argument("--workload", choices=["uneven", "plain", "fri_deep", "prove", "serve"], default="uneven",
                     help="uneven (default, the headline): the 512-column batched commit; plain: 64 x 2^14; "
                          "fri_deep: the FRI + DEEP-quotient phase on 200 x 2^21; prove: a full proof of the "
-                         "wide-Fibonacci AIR, 256 x 2^20")
+                         "wide-Fibonacci AIR, 256 x 2^20; serve: a Poisson stream of wide-Fibonacci prove "
+                         "requests through the continuous-batching server")
+    ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fft-tuning", default=None,
@@ -291,6 +345,8 @@ def main():
         return bench_fri_deep(args)
     if args.workload == "prove":
         return bench_prove(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     work = UNEVEN_COMMIT if args.workload == "uneven" else PLAIN_COMMIT
     logs, b, seed = work["logs"], work["log_blowup"], work["seed"]
     L = max(logs) + b

@@ -266,6 +266,13 @@ int sh_wide_fibonacci_composition(sh_ctx *ctx, const uint32_t *const *d_cols, ui
                                   uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
                                   const uint32_t *powers, uint32_t *const d_accum[4]);
 
+/* generate_trace of the wide-Fibonacci AIR (crates/examples/src/wide_fibonacci/mod.rs:27-48)
+ * on the device from its two input columns d_c0, d_c1 (2^log_size rows): d_cols[0..n_cols)
+ * receive c0, c1 and c_k = c_{k-2}^2 + c_{k-1}^2 (bit-reversed evaluations on
+ * CanonicCoset(log_size), as the reference's generator lays them out). */
+int sh_wide_fibonacci_trace(sh_ctx *ctx, const uint32_t *d_c0, const uint32_t *d_c1, uint32_t log_size,
+                            uint64_t n_cols, uint32_t *const *d_cols);
+
 /* ---------------------------------------------------------------- graph capture */
 /* hipGraph of sh_commit_trace with these exact arguments (no counterpart in the
  * reference): one graph launch replays interpolate + LDE + Merkle on the same

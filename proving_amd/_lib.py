@@ -43,6 +43,7 @@ SIGNATURES = {
     "sh_split_at_mid": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
     "sh_eval_at_points": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_u32, c_vp]),
     "sh_wide_fibonacci_composition": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
+    "sh_wide_fibonacci_trace": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u64, c_vp]),
     "sh_merkle_build_leaves": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_merkle_build_next_layer": (c_int, [c_vp, c_vp, c_u32, c_vp]),
     "sh_merkle_commit_lifted": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp]),

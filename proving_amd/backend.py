@@ -491,6 +491,14 @@ class _Ops:
                   "wide_fibonacci_composition")
         return accum
 
+    def wide_fibonacci_trace(self, c0, c1, n_cols):
+        """generate_trace (crates/examples/src/wide_fibonacci/mod.rs:27-48) on the device from the
+        two input columns; returns n_cols BaseColumns."""
+        cols = [BaseColumn(self.ctx, c0.log_size) for _ in range(n_cols)]
+        self._chk(self.lib.sh_wide_fibonacci_trace(self.h, c0.ptr, c1.ptr, c0.log_size, n_cols, _ptrs(cols)),
+                  "wide_fibonacci_trace")
+        return cols
+
     # ---- GrindOps (crates/stwo/src/core/proof_of_work.rs:10-14) -------------------------------
     def grind(self, digest: bytes, pow_bits):
         d = (ctypes.c_uint8 * 32)(*digest)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -46,6 +47,7 @@ struct sh_ctx {
   // freed blocks are kept per rounded size and handed out again; reuse is safe
   // without a sync because every entry point orders its work on the main
   // stream (auxiliary streams are joined back before a call returns)
+  std::mutex pool_mu;  // sh_free may come from another host thread (a buffer handed between contexts)
   std::unordered_map<uint64_t, std::vector<void*>> pool;
   std::unordered_map<void*, uint64_t> pool_live;
   uint64_t pool_bytes = 0;

@@ -68,7 +68,39 @@ __global__ __launch_bounds__(kThreads) void k_wide_fib_composition(const uint32_
   finish(s3, acc[3]);
 }
 
+// generate_trace (crates/examples/src/wide_fibonacci/mod.rs:27-48) from its two input
+// columns: one lane per row runs the recurrence c_k = c_{k-2}^2 + c_{k-1}^2 down the row.
+__global__ __launch_bounds__(kThreads) void k_wide_fib_trace(const uint32_t* __restrict__ c0,
+                                                             const uint32_t* __restrict__ c1, uint32_t n_rows,
+                                                             uint32_t n_cols, uint32_t* const* __restrict__ out) {
+  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+  if (r >= n_rows) return;
+  uint32_t a = c0[r], b = c1[r];
+  out[0][r] = a;
+  out[1][r] = b;
+  for (uint32_t k = 2; k < n_cols; k++) {
+    const uint32_t c = sh::madd(sh::mmul(a, a), sh::mmul(b, b));
+    out[k][r] = c;
+    a = b;
+    b = c;
+  }
+}
+
 }  // namespace
+
+extern "C" int sh_wide_fibonacci_trace(sh_ctx* ctx, const uint32_t* d_c0, const uint32_t* d_c1, uint32_t log_size,
+                                       uint64_t n_cols, uint32_t* const* d_cols) {
+  if (!ctx || !d_c0 || !d_c1 || !d_cols) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, n_cols >= 2 && n_cols <= 4096 && log_size <= 30, "invalid trace shape (%llu columns, log %u)",
+             (unsigned long long)n_cols, log_size);
+  void* d_ptrs;
+  SH_TRY(sh_stage(ctx, d_cols, n_cols * sizeof(void*), &d_ptrs));
+  SpanGuard span(ctx, "trace_generation");
+  const uint32_t rows = 1u << log_size;
+  k_wide_fib_trace<<<(rows + kThreads - 1) / kThreads, kThreads, 0, ctx->stream>>>(d_c0, d_c1, rows, (uint32_t)n_cols,
+                                                                                  (uint32_t* const*)d_ptrs);
+  return sh_launch_check(ctx, "wide_fib_trace");
+}
 
 // coset_vanishing (crates/stwo/src/core/constraints.rs:12-35) of a coset given by
 // index (initial, log) at the M31 point of index p: the x coordinate of

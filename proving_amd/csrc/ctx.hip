@@ -244,7 +244,7 @@ extern "C" int sh_timer_stop(sh_ctx* ctx, float* ms) {
 
 static constexpr uint64_t kPoolCapBytes = 64ull << 30;  // cached bytes kept for reuse
 
-static int pool_release(sh_ctx* ctx) {
+static int pool_release(sh_ctx* ctx) {  // caller holds pool_mu
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
   for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
   for (auto& kv : ctx->pool)
@@ -255,6 +255,7 @@ static int pool_release(sh_ctx* ctx) {
 }
 
 int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
   *out = nullptr;
   const uint64_t cls = ((bytes ? bytes : 4) + 255) & ~255ull;
   auto it = ctx->pool.find(cls);
@@ -276,6 +277,7 @@ int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
 
 void sh_dev_free(sh_ctx* ctx, void* p) {
   if (!p) return;
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
   auto it = ctx->pool_live.find(p);
   if (it == ctx->pool_live.end()) {  // not ours: free it the plain way
     hipStreamSynchronize(ctx->main_stream);
@@ -308,6 +310,7 @@ extern "C" int sh_free(sh_ctx* ctx, void* d) {
 
 extern "C" int sh_pool_trim(sh_ctx* ctx) {
   if (!ctx) return SH_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
   return pool_release(ctx);
 }
 

@@ -197,26 +197,38 @@ def serialize_commitment_scheme_proof(proof):
     return out
 
 
-def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None):
-    """test_wide_fib_prove_with_blake's prover side (crates/examples/src/wide_fibonacci/mod.rs:
-    183-220) through prove_ex (crates/stwo/src/prover/mod.rs:38-135): empty preprocessed tree,
-    the trace tree, the composition polynomial (evaluated on the committed evaluations,
-    interpolated, split at the middle, committed as 8 coordinate polynomials), the OODS
-    proof of work, the OODS point and prove_values.  trace: device columns (BaseColumn, bit-
-    reversed evaluations on CanonicCoset(log_n)).  Returns (felts, proof dict, the channel)."""
-    ctx = backend.ctx
+def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None):
+    """First phase of test_wide_fib_prove_with_blake's prover (crates/examples/src/wide_fibonacci/
+    mod.rs:183-210): config with an empty, unlifted preprocessed tree; commit it and the trace.
+    committed_trace may carry a CommitmentTree built elsewhere (batched across proofs); its root is
+    mixed exactly as a fresh commit would be.  Returns (scheme, channel)."""
     b = fri["log_blowup_factor"]
-    n_cols = len(trace)
     channel = channel or Channel(backend.lib)
     scheme = CommitmentSchemeProver(backend, twiddles, fri, log_n + b, 0)
     scheme.commit_evals([], channel)                       # preprocessed trace (empty)
-    trace_tree = scheme.commit_evals(trace, channel)       # trace
+    if committed_trace is None:
+        scheme.commit_evals(trace, channel)                # trace
+    else:
+        assert committed_trace.lifting_log_size == log_n + b
+        channel.mix_root(committed_trace.root)
+        scheme.trees.append(committed_trace)
+    return scheme, channel
+
+
+def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
+    """prove_ex (crates/stwo/src/prover/mod.rs:38-135) after the trace commitment: the composition
+    polynomial (evaluated on the committed evaluations, interpolated, split at the middle,
+    committed as 8 coordinate polynomials), the OODS proof of work and point, prove_values.
+    Runs on scheme.be (which may be another context than the one that committed the trace)."""
+    backend, twiddles, fri = scheme.be, scheme.tw, scheme.fri
+    ctx = backend.ctx
+    b = fri["log_blowup_factor"]
+    trace_tree = scheme.trees[1]
     # compute_composition_polynomial (crates/stwo/src/prover/air/component_prover.rs:103-131)
     alpha = [int(x) for x in channel.draw_secure_felt()]
     m = n_cols - 2
     powers = F.secure_powers(alpha, m)[::-1]               # ColumnAccumulator powers, reversed
     eval_log = log_n + 1                                   # max_constraint_log_degree_bound
-    assert 1 <= b, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
     log_expansion = b - 1                                  # EvaluationMode::infer: SubDomain
     assert log_expansion == 0, "only log_blowup_factor 1 (SubDomain {log_expansion: 0}) is driven here"
     acc = SecureColumnByCoords(ctx, eval_log)
@@ -235,4 +247,33 @@ def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None):
     felts, proof = scheme.prove_values(sampled_points, channel, oods_pow)
     proof["composition_random_coeff"] = alpha
     proof["oods_point"] = z
+    return felts, proof
+
+
+def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None):
+    """test_wide_fib_prove_with_blake's prover side (crates/examples/src/wide_fibonacci/mod.rs:
+    183-220) through prove_ex: empty preprocessed tree, the trace tree, then
+    finish_wide_fibonacci.  trace: device columns (BaseColumn, bit-reversed evaluations on
+    CanonicCoset(log_n)).  Returns (felts, proof dict, the channel)."""
+    assert fri["log_blowup_factor"] >= 1, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
+    scheme, channel = start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel)
+    felts, proof = finish_wide_fibonacci(scheme, channel, len(trace), log_n)
     return felts, proof, channel
+
+
+def commit_traces_batched(backend, twiddles, traces, log_sizes, log_blowup):
+    """Trace commitments of several proofs at once: one interpolation and one evaluation call
+    over every column of every proof (one launch chain per column size, whichever proof a column
+    belongs to), then each proof's own lifted Merkle tree.  The coefficient columns are the
+    interpolated trace buffers themselves.  Returns one CommitmentTree per proof, identical to
+    what commit_evals would build for it alone."""
+    cols = [c for t in traces for c in t]
+    backend.interpolate_columns(cols, twiddles)
+    ldes = backend.evaluate_polynomials(cols, log_blowup, twiddles)
+    trees, k = [], 0
+    for t, lg in zip(traces, log_sizes):
+        mine = ldes[k:k + len(t)]
+        k += len(t)
+        layers, root = backend.commit_lifted(mine, lg + log_blowup)
+        trees.append(CommitmentTree(list(t), mine, layers, root, lg + log_blowup))
+    return trees

@@ -46,3 +46,12 @@ def test_composition_rejects_bad_shapes(ctx, backend):
     acc = SecureColumnByCoords(ctx, 4)
     with pytest.raises(HipError, match="3..4096"):
         backend.wide_fibonacci_composition(cols, 3, 4, [[1, 0, 0, 0]], acc)
+
+
+@pytest.mark.parametrize("n_cols,log_n", [(2, 3), (3, 0), (17, 9), (256, 12)])
+def test_trace_generation_matches_host(ctx, backend, n_cols, log_n):
+    exp = wide_fib_trace(n_cols, log_n, 9)
+    c0, c1 = BaseColumn.from_cpu(ctx, exp[0]), BaseColumn.from_cpu(ctx, exp[1])
+    cols = backend.wide_fibonacci_trace(c0, c1, n_cols)
+    for k in (0, 1, n_cols // 2, n_cols - 1):
+        np.testing.assert_array_equal(cols[k].to_cpu(), exp[k])
