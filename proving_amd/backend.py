@@ -23,7 +23,10 @@ def _ptrs(bufs):
 
 
 def _u32s(vals):
-    return (ctypes.c_uint32 * max(1, len(vals)))(*[int(v) for v in vals])
+    a = np.asarray(vals, dtype=np.uint32).ravel()
+    if a.size == 0:
+        a = np.zeros(1, np.uint32)
+    return a.ctypes.data_as(ctypes.c_void_p)  # the pointer object keeps the array alive
 
 
 class Context:
@@ -380,11 +383,10 @@ class _Ops:
         """eval_at_point of every polynomial at every point, one batched call; returns
         out[i][j] = coeffs[i] at points[j] (QM31 as 4 ints)."""
         n, k = len(coeffs), len(points)
-        out = (ctypes.c_uint32 * max(1, 4 * n * k))()
-        flat = [int(x) for p in points for x in p]
+        out = np.empty(max(1, 4 * n * k), np.uint32)
         self._chk(self.lib.sh_eval_at_points(self.h, _ptrs(coeffs), _u32s([c.log_size for c in coeffs]), n,
-                                             _u32s(flat), k, out), "eval_at_points")
-        return [[[int(out[4 * (i * k + j) + q]) for q in range(4)] for j in range(k)] for i in range(n)]
+                                             _u32s(points), k, out.ctypes.data), "eval_at_points")
+        return out[:4 * n * k].reshape(n, k, 4).tolist()
 
     def barycentric_weights(self, log_size, point):
         w = SecureColumnByCoords(self.ctx, log_size)
@@ -570,32 +572,42 @@ def merkle_decommit(backend, layers, positions):
     97-162); layers root-first as returned by commit_lifted; positions sorted and deduplicated."""
     L = len(layers) - 1
     cap = max(1, len(positions) * max(1, L))
-    wit = (ctypes.c_uint8 * (32 * cap))()
+    wit = np.empty(32 * cap, np.uint8)
     nw = ctypes.c_uint64()
     backend._chk(backend.lib.sh_merkle_decommit(backend.h, _ptrs(layers), L, _u32s(positions), len(positions),
-                                                wit, cap, ctypes.byref(nw)), "merkle_decommit")
-    raw = bytes(wit)
+                                                wit.ctypes.data, cap, ctypes.byref(nw)), "merkle_decommit")
+    raw = wit[:32 * nw.value].tobytes()
     return [raw[32 * i:32 * i + 32] for i in range(nw.value)]
+
+
+def gather_ptrs(backend, ptrs, rows):
+    """out[i] = word rows[i] of the device column at address ptrs[i] (numpy arrays), one launch."""
+    ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64)
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    out = np.empty(max(1, len(rows)), np.uint32)
+    backend._chk(backend.lib.sh_gather(backend.h, ptrs.ctypes.data, rows.ctypes.data, len(rows), out.ctypes.data),
+                 "gather")
+    return out[:len(rows)]
 
 
 def gather(backend, cols, rows):
     """out[i] = cols[i].at(rows[i]) in one launch."""
-    out = (ctypes.c_uint32 * max(1, len(rows)))()
-    backend._chk(backend.lib.sh_gather(backend.h, _ptrs(cols), _u32s(rows), len(rows), out), "gather")
-    return [int(x) for x in out[:len(rows)]]
+    out = np.empty(max(1, len(rows)), np.uint32)
+    backend._chk(backend.lib.sh_gather(backend.h, _ptrs(cols), _u32s(rows), len(rows), out.ctypes.data), "gather")
+    return out[:len(rows)].tolist()
 
 
 def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22):
     """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284) on a device QM31
     column; returns (proof felts, sorted query positions)."""
-    felts = (ctypes.c_uint64 * cap)()
+    felts = np.empty(cap, np.uint64)  # not zero-filled: only the written prefix is read
     nf = ctypes.c_uint64()
-    qp = (ctypes.c_uint32 * max(1, params.n_queries))()
+    qp = np.empty(max(1, params.n_queries), np.uint32)
     nq = ctypes.c_uint64()
     backend._chk(backend.lib.sh_fri_prove(backend.h, twiddles.h, ctypes.byref(channel.s), ctypes.byref(params),
-                                          _ptrs(column.columns), column.log_size, felts, cap, ctypes.byref(nf),
-                                          qp, ctypes.byref(nq)), "fri_prove")
-    return [int(x) for x in felts[:nf.value]], list(qp)[:nq.value]
+                                          _ptrs(column.columns), column.log_size, felts.ctypes.data, cap,
+                                          ctypes.byref(nf), qp.ctypes.data, ctypes.byref(nq)), "fri_prove")
+    return felts[:nf.value].tolist(), qp[:nq.value].tolist()
 
 
 def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log_size, log_blowup, out=None):

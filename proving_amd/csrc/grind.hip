@@ -7,10 +7,12 @@
 // (crates/stwo/src/prover/backend/cpu/grind.rs:5-15) and the SIMD backend's
 // deterministic chunked search (crates/stwo/src/prover/backend/simd/grind.rs:78-122).
 //
-// Device search: rounds of 2^26 consecutive nonces; each lane hashes 16 nonces
-// (one 40-byte single-block compression each) and the round's smallest hit is
-// kept with atomicMin.  Rounds run in increasing order and stop at the first
-// round with a hit, so the answer is the global minimum.
+// Device search: rounds of consecutive nonces; each lane hashes 16 nonces (one
+// 40-byte single-block compression each) and the round's smallest hit is kept
+// with atomicMin.  Rounds run in increasing order and stop at the first round
+// with a hit, so the answer is the global minimum.  The first round covers
+// 2^(pow_bits + 5) nonces (a hit with probability 1 - e^-32), later rounds
+// double up to 2^26, so a small grind costs a small launch.
 #include <string.h>
 
 #include "blake2s.h"
@@ -19,7 +21,8 @@
 namespace {
 
 constexpr uint32_t kPerLane = 16;
-constexpr uint32_t kRoundLog = 26;
+constexpr uint32_t kRoundLog = 26;     // largest round
+constexpr uint32_t kMinRoundLog = 12;  // kPerLane * 256 nonces: one workgroup
 
 __global__ __launch_bounds__(256) void k_grind(const uint32_t* __restrict__ pre_w, uint32_t pow_bits,
                                                uint64_t base, unsigned long long* best) {
@@ -71,10 +74,11 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
   unsigned long long* d_best = (unsigned long long*)((uint8_t*)scr + 32);
   SH_CHECK_HIP(ctx, hipMemcpyAsync(d_pre, pre, 32, hipMemcpyHostToDevice, ctx->stream));
   const unsigned long long none = ~0ull;
-  const uint64_t round = 1ull << kRoundLog;
-  const unsigned grid = (unsigned)(round / kPerLane / 256);
+  uint32_t round_log = pow_bits + 5 < kMinRoundLog ? kMinRoundLog : (pow_bits + 5 > kRoundLog ? kRoundLog : pow_bits + 5);
   SpanGuard span(ctx, "grind");
-  for (uint64_t base = 0;; base += round) {
+  for (uint64_t base = 0;;) {
+    const uint64_t round = 1ull << round_log;
+    const unsigned grid = (unsigned)(round / kPerLane / 256);
     SH_CHECK_HIP(ctx, hipMemcpyAsync(d_best, &none, 8, hipMemcpyHostToDevice, ctx->stream));
     k_grind<<<grid, 256, 0, ctx->stream>>>(d_pre, pow_bits, base, d_best);
     SH_TRY(sh_launch_check(ctx, "grind"));
@@ -86,5 +90,7 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
       return SH_OK;
     }
     if (base > (~0ull) - 2 * round) return sh_fail(ctx, SH_ERR_NOT_FOUND, "no nonce found");
+    base += round;
+    if (round_log < kRoundLog) round_log++;
   }
 }

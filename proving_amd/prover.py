@@ -16,15 +16,11 @@ import ctypes
 import numpy as np
 
 from . import field as F
-from .backend import (BaseColumn, Channel, FriParams, HashLayer, SecureColumnByCoords, canonic_half_initial, fri_prove,
-                      fri_quotients, gather, merkle_decommit)
+from .backend import (BaseColumn, Channel, FriParams, HashLayer, SecureColumnByCoords, fri_prove, fri_quotients,
+                      gather_ptrs, merkle_decommit)
 
 OODS_POW_BITS = 16  # crates/stwo/src/core/proof_of_work.rs:8
 COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
-
-
-def _hash_words(h):
-    return [int.from_bytes(h[4 * i:4 * i + 4], "little") for i in range(8)]
 
 
 class CommitmentTree:
@@ -40,14 +36,14 @@ class CommitmentTree:
         MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162):
         queried values per column at every position (duplicates kept), and the hash witness."""
         L = self.lifting_log_size
-        cols, rows = [], []
-        for e in self.evals:
-            shift = L - e.log_size
-            cols.extend([e] * len(positions))
-            rows.extend(((p >> (shift + 1)) << 1) + (p & 1) for p in positions)
-        flat = gather(backend, cols, rows) if rows else []  # one launch for every (column, row)
         k = len(positions)
-        queried = [flat[i * k:(i + 1) * k] for i in range(len(self.evals))]
+        pos = np.asarray(positions, dtype=np.int64)
+        if k and self.evals:
+            rows = np.concatenate([((pos >> (L - e.log_size + 1)) << 1) + (pos & 1) for e in self.evals])
+            ptrs = np.repeat(np.array([e.ptr for e in self.evals], np.uint64), k)
+            queried = gather_ptrs(backend, ptrs, rows).reshape(len(self.evals), k)  # one launch
+        else:
+            queried = np.zeros((len(self.evals), k), np.uint32)
         witness = merkle_decommit(backend, self.layers, sorted(set(positions))) if L > 0 else []
         return queried, witness
 
@@ -169,32 +165,40 @@ def serialize_commitment_scheme_proof(proof):
     PcsConfig (its FriParams only), commitments, oods_proof_of_work, sampled_values,
     decommitments, queried_values, fri_proof; vectors are length-prefixed, a hash is 8 LE u32,
     a QM31 4 felts."""
+    parts = []
+
+    def scalars(*v):
+        parts.append(np.asarray(v, dtype=np.uint64))
+
+    def hashes(hs):
+        scalars(len(hs))
+        if hs:
+            parts.append(np.frombuffer(b"".join(hs), dtype="<u4").astype(np.uint64))
+
     f = proof["fri_params"]
-    out = [f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"], f["fold_step"]]
-    out.append(len(proof["commitments"]))
-    for h in proof["commitments"]:
-        out.extend(_hash_words(h))
-    out.append(proof["oods_proof_of_work"])
-    out.append(len(proof["sampled_values"]))
+    scalars(f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"], f["fold_step"])
+    hashes(proof["commitments"])
+    scalars(proof["oods_proof_of_work"], len(proof["sampled_values"]))
     for tree in proof["sampled_values"]:
-        out.append(len(tree))
+        scalars(len(tree))
         for col in tree:
-            out.append(len(col))
-            for v in col:
-                out.extend(int(x) for x in v)
-    out.append(len(proof["decommitments"]))
+            scalars(len(col))
+            if col:
+                parts.append(np.asarray(col, dtype=np.uint64).ravel())
+    scalars(len(proof["decommitments"]))
     for wit in proof["decommitments"]:
-        out.append(len(wit))
-        for h in wit:
-            out.extend(_hash_words(h))
-    out.append(len(proof["queried_values"]))
+        hashes(wit)
+    scalars(len(proof["queried_values"]))
     for tree in proof["queried_values"]:
-        out.append(len(tree))
-        for col in tree:
-            out.append(len(col))
-            out.extend(int(x) for x in col)
-    out.extend(proof["fri_felts"])
-    return out
+        tree = np.asarray(tree, dtype=np.uint64).reshape(len(tree), -1) if len(tree) else np.zeros((0, 0), np.uint64)
+        scalars(tree.shape[0])
+        if tree.shape[0]:
+            block = np.empty((tree.shape[0], tree.shape[1] + 1), np.uint64)
+            block[:, 0] = tree.shape[1]
+            block[:, 1:] = tree
+            parts.append(block.ravel())
+    parts.append(np.asarray(proof["fri_felts"], dtype=np.uint64))
+    return np.concatenate(parts).tolist()
 
 
 def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None):

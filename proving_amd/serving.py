@@ -1,7 +1,8 @@
 """A proof-serving loop on one GPU: continuous batching of wide-Fibonacci prove requests.
 
-Requests arrive over time (a Poisson stream in the benchmark). The scheduler admits a request
-when its estimated device footprint fits the HBM budget and a slot is free; every scheduling
+Requests arrive over time (a Poisson stream in the benchmark). The scheduler admits waiting
+requests shortest first (by trace cells) when the estimated device footprint fits the HBM budget
+and a slot is free; every scheduling
 tick, the traces of all newly admitted requests are generated on the device and committed
 together (one interpolation and one evaluation launch chain per column size across all of
 them, then each request's own Merkle tree), and each request then finishes its proof
@@ -52,7 +53,7 @@ def device_bytes(n_cols, log_n, log_blowup=1):
 
 
 class ProofServer:
-    def __init__(self, twiddle_log, fri, n_workers=3, hbm_budget=64 << 30, max_in_flight=None):
+    def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05):
         self.fri = dict(fri)
         self.ctx = Context(0)
         self.be = HipBackend(self.ctx)
@@ -60,6 +61,7 @@ class ProofServer:
         self.budget = hbm_budget
         self.max_in_flight = max_in_flight or 2 * n_workers
         self.n_workers = n_workers
+        self.max_wait = max_wait
         self._work = queue.Queue()
         self._lock = threading.Lock()
         self._reserved = 0
@@ -120,23 +122,30 @@ class ProofServer:
         """Runs the request stream; arrivals are honoured in wall-clock time when realtime.
         Returns the requests with their timestamps and proof felts."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        waiting = []  # arrived, not admitted
         self._t0 = time.perf_counter()
         i = 0
-        while i < len(pending):
+        b = self.fri["log_blowup_factor"]
+        while i < len(pending) or waiting:
             now = time.perf_counter() - self._t0
+            while i < len(pending) and (pending[i].arrival <= now or not realtime):
+                waiting.append(pending[i])
+                i += 1
+            # shortest job first (by trace cells), so a small request does not queue behind a
+            # large one; a request waiting longer than max_wait goes first (no starvation)
+            waiting.sort(key=lambda r: (now - r.arrival < self.max_wait, r.n_cols << r.log_n, r.arrival))
             batch = []
             with self._lock:
-                while i < len(pending) and (pending[i].arrival <= now or not realtime):
-                    r = pending[i]
-                    cost = device_bytes(r.n_cols, r.log_n, self.fri["log_blowup_factor"])
+                while waiting:
+                    r = waiting[0]
+                    cost = device_bytes(r.n_cols, r.log_n, b)
                     if self._in_flight >= self.max_in_flight or (self._reserved + cost > self.budget
                                                                    and self._in_flight > 0):
                         break
                     self._reserved += cost
                     self._in_flight += 1
                     r.admitted = now
-                    batch.append(r)
-                    i += 1
+                    batch.append(waiting.pop(0))
             if batch:
                 self._admit(batch)
             else:
