@@ -20,6 +20,8 @@
 // groups of 2^(hi-lo) words at stride 2^lo, 32 consecutive groups per
 // workgroup so that every global access is a 128-byte coalesced row.  One
 // launch covers every column of one size class (blockIdx.y = column).
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -154,7 +156,24 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
 // a subdomain uses prefixes of its parent's layers).  Global words move as
 // 128-bit rows whenever four consecutive LDS words are four consecutive
 // positions (tile of >= 4 groups, or a contiguous block).
-template <bool INVERSE>
+// One register stage of r layers, r <= MAXR (a kernel compiled for a smaller MAXR holds fewer
+// values in registers and runs at a higher occupancy).
+template <bool INVERSE, bool CIRCLE, int MAXR>
+__device__ __forceinline__ void run_stage(uint32_t r, uint32_t* lds, uint32_t G, uint32_t tl, uint32_t s,
+                                          uint32_t base, uint32_t lo, const uint32_t* __restrict__ tw,
+                                          uint32_t tw_end, uint32_t Pl) {
+  if constexpr (MAXR >= 5) {
+    if (r == 5) return fft_stage<INVERSE, CIRCLE, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl);
+  }
+  if constexpr (MAXR >= 4) {
+    if (r == 4) return fft_stage<INVERSE, CIRCLE, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl);
+  }
+  if (r == 3) return fft_stage<INVERSE, CIRCLE, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl);
+  if (r == 2) return fft_stage<INVERSE, CIRCLE, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl);
+  fft_stage<INVERSE, CIRCLE, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl);
+}
+
+template <bool INVERSE, int MAXR>
 __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* src_cols,
                                                        uint32_t* const* dst_cols, uint32_t src_mask,
                                                        const uint32_t* __restrict__ tw, uint32_t K,
@@ -197,23 +216,10 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
     const uint32_t si = INVERSE ? st : n_stages - 1 - st;
     const uint32_t s = si * st_base + (si < st_extra ? si : st_extra);
     const uint32_t r = st_base + (si < st_extra ? 1 : 0);
-    if (lo + s == 0) {
-      switch (r) {
-        case 5: fft_stage<INVERSE, true, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 4: fft_stage<INVERSE, true, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 3: fft_stage<INVERSE, true, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 2: fft_stage<INVERSE, true, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        default: fft_stage<INVERSE, true, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      }
-    } else {
-      switch (r) {
-        case 5: fft_stage<INVERSE, false, 5>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 4: fft_stage<INVERSE, false, 4>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 3: fft_stage<INVERSE, false, 3>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        case 2: fft_stage<INVERSE, false, 2>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-        default: fft_stage<INVERSE, false, 1>(lds, G, tl, s, base, lo, tw, tw_end, Pl); break;
-      }
-    }
+    if (lo + s == 0)
+      run_stage<INVERSE, true, MAXR>(r, lds, G, tl, s, base, lo, tw, tw_end, Pl);
+    else
+      run_stage<INVERSE, false, MAXR>(r, lds, G, tl, s, base, lo, tw, tw_end, Pl);
     __syncthreads();
   }
 
@@ -351,9 +357,11 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
     size_t lds = 4u << (G + p.tl);
     const uint32_t* const* s = k == 0 ? d_src : (const uint32_t* const*)d_dst;
     SpanGuard span(ctx, "fft_ifft_pass");
-    k_fft_pass<true><<<grid, kThreads, lds, ctx->stream>>>(
-        s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi, p.tl,
-        k + 1 == passes.size() ? scale : 0u, tune.max_stage);
+    auto kern = tune.max_stage >= 5 ? k_fft_pass<true, 5> : tune.max_stage == 4 ? k_fft_pass<true, 4>
+                                                                                 : k_fft_pass<true, 3>;
+    kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi,
+                                               p.tl, k + 1 == passes.size() ? scale : 0u,
+                                               std::min<uint32_t>(tune.max_stage, 5));
     SH_TRY(sh_launch_check(ctx, "ifft_pass"));
   }
   return SH_OK;
@@ -389,9 +397,11 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
     bool first = k == (int)passes.size() - 1;
     const uint32_t* const* s = first ? d_src : (const uint32_t* const*)d_dst;
     SpanGuard span(ctx, "fft_rfft_pass");
-    k_fft_pass<false><<<grid, kThreads, lds, ctx->stream>>>(
-        s, d_dst, first ? src_mask : 0xffffffffu, tw->tw, tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u,
-        tune.max_stage);
+    auto kern = tune.max_stage >= 5 ? k_fft_pass<false, 5> : tune.max_stage == 4 ? k_fft_pass<false, 4>
+                                                                                  : k_fft_pass<false, 3>;
+    kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, first ? src_mask : 0xffffffffu, tw->tw,
+                                               tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u,
+                                               std::min<uint32_t>(tune.max_stage, 5));
     SH_TRY(sh_launch_check(ctx, "rfft_pass"));
   }
   return SH_OK;
