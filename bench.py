@@ -477,11 +477,20 @@ def main():
     bytes_per_launch = work_per[dom]["bytes"] / n_launch
     avg_s = prof[dom]["ms"] / prof[dom]["n"] / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
+    # the FFT passes are bound by integer VALU issue, not HBM: 12 VALU instructions per M31
+    # butterfly (multiply 6, add 3, subtract 3) against the 32-bit VALU peak
+    bfly = {"fft_ifft_pass": sum(lg << (lg - 1) for lg in logs),
+            "fft_rfft_pass": sum(lg << (lg + b - 1) for lg in logs)}
+    valu_ops = 12 * bfly[dom] / n_launch
+    valu_tops = valu_ops / avg_s / 1e12
     tuning_now = {str(m): list(be.fft_tuning(m)) for m in sorted(set(logs) | {lg + b for lg in logs})}
     traffic, same_shapes = profiled_traffic(dom, tuning_now) if args.workload == "uneven" else (None, None)
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
+    roofline["valu"] = {"ops_per_butterfly": 12, "butterflies_per_launch": bfly[dom] / n_launch,
+                        "achieved_tops": round(valu_tops, 2), "peak_tops": VALU_PEAK_TOPS,
+                        "frac": round(valu_tops / VALU_PEAK_TOPS, 4)}
     if traffic is not None:
         roofline["traffic_source"] = ("profiles/commit_uneven512_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + "
                                       "WRITE_SIZE x 1.0 per dispatch)")

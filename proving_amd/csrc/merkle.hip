@@ -321,7 +321,11 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers) {
   uint32_t k = L;
   while (k > 0) {
-    uint32_t levels = k < 8 ? k : 8;
+    // Level l of a launch keeps 256 >> l lanes of each workgroup busy, so wide layers take three
+    // levels per launch (every wave full); once the next launch would be small (< 2^12 nodes),
+    // the rest goes up to eight levels at a time (launch latency dominates there).
+    const uint64_t n_in = 1ull << (k - 1);
+    uint32_t levels = (n_in >> 3) >= (1ull << 12) ? 3 : (k < 8 ? k : 8);
     LayerOuts outs;
     for (uint32_t l = 0; l < levels; l++) outs.p[l] = layers[k - 1 - l];
     uint64_t n0 = 1ull << (k - 1);
