@@ -71,10 +71,14 @@ def merkle_decommit(layers, positions):
 def merkle_verify(root, height, positions, rows, witness):
     """MerkleVerifierLifted::verify for columns of one size (crates/stwo/src/core/vcs_lifted/
     verifier.rs:115-212): rows[k] holds every column's value at positions[k]."""
+    hashes = [O.blake2s(b"".join(int(v).to_bytes(4, "little") for v in row)) for row in rows]
+    return merkle_verify_leaves(root, height, positions, hashes, witness)
+
+
+def merkle_verify_leaves(root, height, positions, leaf_hashes, witness):
+    """The same check from the leaf hashes of the queried rows (leaf_hashes[k] at positions[k])."""
     leaves = {}
-    for pos, row in zip(positions, rows):
-        data = b"".join(int(v).to_bytes(4, "little") for v in row)
-        h = O.blake2s(data)
+    for pos, h in zip(positions, leaf_hashes):
         assert leaves.setdefault(pos, h) == h
     prev = sorted(leaves.items())
     wit = iter(witness)
