@@ -121,43 +121,87 @@ def line_interpolate(values_bitrev, init, log):
     return [qmulm(x, inv) for x in v]
 
 
-def _decommit_layer(col, layers, queries):
-    """compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438),
-    fold_step 1, then the layer's Merkle decommitment."""
+def squared_alpha_powers(alpha, n):
+    """squared_alpha_powers (crates/stwo/src/prover/fri.rs:79-87): [a, a^2, a^4, ...], n terms."""
+    out = []
+    for _ in range(n):
+        out.append(list(alpha))
+        alpha = qmul(alpha, alpha)
+    return out
+
+
+def _pack_leaves_input(cols4):
+    """PackLeavesOps::pack_leaves_input (crates/stwo/src/prover/backend/cpu/merkle_lifted.rs:101-124):
+    column coord + 4*offset holds coordinate coord of the values at rows 4i + offset."""
+    return [np.ascontiguousarray(np.asarray(cols4[c])[off::4]) for off in range(4) for c in range(4)]
+
+
+def _commit_fri_layer(cols4, log, pack):
+    """FriFirstLayerProver::new / FriInnerLayerProver::new (crates/stwo/src/prover/fri.rs:294-306,
+    357-368): one leaf per value, or per four consecutive values when the leaves are packed."""
+    if pack:
+        return O.merkle_commit_lifted(_pack_leaves_input(cols4), log - 2)
+    return O.merkle_commit_lifted([np.ascontiguousarray(c) for c in cols4], log)
+
+
+def _fold_queries(queries, k):
+    """Queries::fold (crates/stwo/src/core/queries.rs:48-55)."""
+    return sorted(set(q >> k for q in queries))
+
+
+def _decommit_layer(col, layers, queries, fold_step=1, pack=False):
+    """compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438):
+    every coset of 2^fold_step positions holding a query is opened, the values the verifier cannot
+    rebuild go to the witness; then the layer's Merkle decommitment (fri.rs:308-341), at positions
+    divided by 4 when the leaves are packed."""
     positions, witness_evals = [], []
     i = 0
     while i < len(queries):
         j = i
-        while j < len(queries) and queries[j] >> 1 == queries[i] >> 1:
+        while j < len(queries) and queries[j] >> fold_step == queries[i] >> fold_step:
             j += 1
-        start = (queries[i] >> 1) << 1
-        for p in (start, start + 1):
+        start = (queries[i] >> fold_step) << fold_step
+        for p in range(start, start + (1 << fold_step)):
             positions.append(p)
             if p not in queries[i:j]:
                 witness_evals.append([int(col[c][p]) for c in range(4)])
         i = j
-    return {"fri_witness": witness_evals, "hash_witness": merkle_decommit(layers, positions),
+    mpos = sorted(set(p >> 2 for p in positions)) if pack else positions
+    return {"fri_witness": witness_evals, "hash_witness": merkle_decommit(layers, mpos),
             "commitment": layers[0][0].tobytes()}
 
 
 def fri_prove(column, log_size, params, ch):
-    """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284), fold_step 1.
+    """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284).
     column: (4, 2^log_size) QM31 evaluation on CanonicCoset(log_size).  Returns (proof, queries)."""
-    assert params["fold_step"] == 1
+    fs = params["fold_step"]
+    assert fs >= 1
     last_log = params["log_last_layer_degree_bound"] + params["log_blowup_factor"]
-    first_layers = O.merkle_commit_lifted(list(column), log_size)
+    pack0 = log_size >= 2 and fs > 1
+    first_layers = _commit_fri_layer(column, log_size, pack0)
     ch.mix_hash(first_layers[0][0].tobytes())
     alpha = ch.draw_secure_felt()
     cur = O.fold_circle_into_line(column, log_size, alpha)
     line_log, line_init = log_size - 1, O.canonic_half_initial(log_size)
+
+    def fold(cur, line_init, line_log, alphas):
+        for a in alphas:
+            cur = O.fold_line(cur, line_init, line_log, a)
+            line_log, line_init = line_log - 1, (line_init * 2) % (1 << 31)
+        return cur, line_init, line_log
+
+    # the circle-to-line fold is followed by fold_step - 1 line folds with alpha^2, alpha^4, ...
+    cur, line_init, line_log = fold(cur, line_init, line_log, squared_alpha_powers(qmul(alpha, alpha), fs - 1))
+    assert line_log >= last_log, "The circle-to-line fold results in a smaller line domain than the last layer."
     inner = []
     while line_log > last_log:
-        layers = O.merkle_commit_lifted(list(cur), line_log)
+        k = fs if line_log > last_log + fs else line_log - last_log
+        pack = line_log >= 2 and k > 1
+        layers = _commit_fri_layer(cur, line_log, pack)
         ch.mix_hash(layers[0][0].tobytes())
         alpha = ch.draw_secure_felt()
-        inner.append((cur, layers))
-        cur = O.fold_line(cur, line_init, line_log, alpha)
-        line_log, line_init = line_log - 1, (line_init * 2) % (1 << 31)
+        inner.append((cur, layers, k, pack))
+        cur, line_init, line_log = fold(cur, line_init, line_log, squared_alpha_powers(alpha, k))
     coeffs = line_interpolate([cur[:, i] for i in range(cur.shape[1])], line_init, line_log)
     ordered = [coeffs[bitrev(i, line_log)] for i in range(len(coeffs))]
     bound = 1 << params["log_last_layer_degree_bound"]
@@ -174,12 +218,12 @@ def fri_prove(column, log_size, params, ch):
             if len(raw) < params["n_queries"]:
                 raw.append(int(w) & ((1 << log_size) - 1))
     queries = sorted(set(raw))
-    first = _decommit_layer(column, first_layers, queries)
+    first = _decommit_layer(column, first_layers, queries, fs, pack0)
     inner_proofs = []
-    lq = queries
-    for col, layers in inner:
-        lq = sorted(set(q >> 1 for q in lq))
-        inner_proofs.append(_decommit_layer(col, layers, lq))
+    lq = _fold_queries(queries, fs)
+    for col, layers, k, pack in inner:
+        inner_proofs.append(_decommit_layer(col, layers, lq, k, pack))
+        lq = _fold_queries(lq, k)
     proof = {"first_layer": first, "inner_layers": inner_proofs, "last_layer_poly": last_poly,
              "log_last_layer_degree_bound": params["log_last_layer_degree_bound"], "proof_of_work": nonce}
     return proof, queries
@@ -245,25 +289,68 @@ def _line_eval_at_point(coeffs_bitrev, x):
     return vals[0]
 
 
+def _circle_domain_index(half_init, half_log, i):
+    """CircleDomain::index_at (crates/stwo/src/core/poly/circle/domain.rs:50-60): the half coset,
+    then its negation."""
+    if i < 1 << half_log:
+        return (half_init + (1 << (31 - half_log)) * i) % (1 << 31)
+    return ((1 << 31) - (half_init + (1 << (31 - half_log)) * (i - (1 << half_log)))) % (1 << 31)
+
+
+def fold_coset(evals, init_idx, log, alpha):
+    """fold_coset (crates/stwo/src/core/fri.rs:766-786): log line folds of the values on
+    Coset(init_idx, log) (bit-reversed), alpha squared after each fold."""
+    ev = [list(e) for e in evals]
+    a = list(alpha)
+    for i in range(log):
+        dl, di = log - i, (init_idx << i) % (1 << 31)
+        for j in range(0, 1 << dl, 2):
+            xi = O.m31_inv(int(coset_at(di, dl, bitrev(j, dl))[0]))
+            f0, f1 = qadd(ev[j], ev[j + 1]), qmulm(qsub(ev[j], ev[j + 1]), xi)
+            ev[j >> 1] = qadd(f0, qmul(a, f1))
+        a = qmul(a, a)
+    return ev[0]
+
+
+def fold_circle(evals, init_idx, fold_step, alpha):
+    """SparseEvaluation::fold_circle for one subset (crates/stwo/src/core/fri.rs:725-756): the
+    circle-to-line fold on CircleDomain(Coset(init_idx, fold_step - 1)), then fold_step - 1 line
+    folds with alpha^2, alpha^4, ..."""
+    half = fold_step - 1
+    buf = []
+    for i in range(1 << half):
+        k = bitrev(2 * i, fold_step)
+        y = int(O.index_to_point(_circle_domain_index(init_idx, half, k))[1])
+        yi = O.m31_inv(y)
+        f0, f1 = qadd(evals[2 * i], evals[2 * i + 1]), qmulm(qsub(evals[2 * i], evals[2 * i + 1]), yi)
+        buf.append(qadd(qmul(alpha, f1), f0))
+    return buf[0] if fold_step == 1 else fold_coset(buf, init_idx, half, qmul(alpha, alpha))
+
+
 def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn):
     """FriVerifier::commit / verify_proof_of_work / sample_query_positions / decommit
-    (crates/stwo/src/core/fri.rs:156-378), fold_step 1.  first_layer_query_evals_fn(queries)
-    returns the column's QM31 values at the sorted queries.  Returns the queries, raises on
-    failure."""
-    b = params["log_blowup_factor"]
+    (crates/stwo/src/core/fri.rs:156-378, 480-700).  first_layer_query_evals_fn(queries) returns the
+    column's QM31 values at the sorted queries.  Returns the queries, raises on failure."""
+    b, fs = params["log_blowup_factor"], params["fold_step"]
     ch.mix_hash(proof["first_layer"]["commitment"])
     col_log = log_degree_bound + b
     alpha0 = ch.draw_secure_felt()
-    layer_bound = log_degree_bound - 1
+    pack0 = col_log >= 2 and fs > 1
+    assert log_degree_bound >= fs, "InvalidNumFriLayers"
+    layer_bound = log_degree_bound - fs
     line_init, line_log = O.canonic_half_initial(layer_bound + b + 1), layer_bound + b
     inner = []
+    n_inner = len(proof["inner_layers"])
     for k, lp in enumerate(proof["inner_layers"]):
         ch.mix_hash(lp["commitment"])
-        if k == len(proof["inner_layers"]) - 1:
-            assert layer_bound - params["log_last_layer_degree_bound"] == 1, "InvalidNumFriLayers"
-        inner.append((line_init, line_log, ch.draw_secure_felt(), lp))
-        layer_bound -= 1
-        line_init, line_log = (line_init * 2) % (1 << 31), line_log - 1
+        step = fs
+        if k == n_inner - 1:
+            step = layer_bound - params["log_last_layer_degree_bound"]
+            assert 1 <= step <= fs, "InvalidNumFriLayers"
+        assert layer_bound >= step, "InvalidNumFriLayers"
+        inner.append((line_init, line_log, ch.draw_secure_felt(), lp, step, line_log >= 2 and step > 1))
+        layer_bound -= step
+        line_init, line_log = (line_init << step) % (1 << 31), line_log - step
     assert layer_bound == params["log_last_layer_degree_bound"], "InvalidNumFriLayers"
     assert len(proof["last_layer_poly"]) <= 1 << params["log_last_layer_degree_bound"]
     ch.mix_felts(np.array(proof["last_layer_poly"], np.uint32).reshape(-1))
@@ -276,49 +363,51 @@ def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn):
                 raw.append(int(w) & ((1 << col_log) - 1))
     queries = sorted(set(raw))
 
-    def rebuild(qs, evals, lp, log):
-        """compute_decommitment_positions_and_rebuild_evals (fri.rs:629-671)."""
+    def rebuild(qs, evals, lp, log, step, pack):
+        """compute_decommitment_positions_and_rebuild_evals + build_merkle_verification_inputs
+        (fri.rs:629-700), then the layer's Merkle check."""
         wit = iter(lp["fri_witness"])
         ev = iter(evals)
         positions, subsets = [], []
         i = 0
         while i < len(qs):
             j = i
-            while j < len(qs) and qs[j] >> 1 == qs[i] >> 1:
+            while j < len(qs) and qs[j] >> step == qs[i] >> step:
                 j += 1
-            start = (qs[i] >> 1) << 1
+            start = (qs[i] >> step) << step
             sub = []
-            for p in (start, start + 1):
+            for p in range(start, start + (1 << step)):
                 positions.append(p)
-                sub.append(list(next(ev)) if p in qs[i:j] else list(next(wit)))
+                if p in qs[i:j]:
+                    sub.append(list(next(ev)))
+                else:
+                    w = next(wit, None)
+                    assert w is not None, "EvaluationsInvalid"
+                    sub.append(list(w))
             subsets.append((start, sub))
             i = j
         assert next(wit, None) is None, "EvaluationsInvalid"
-        rows = [s for _, sub in subsets for s in sub]
-        assert merkle_verify(lp["commitment"], log, positions, rows, lp["hash_witness"]), "CommitmentInvalid"
+        values = [v for _, sub in subsets for v in sub]
+        if pack:
+            mpos = sorted(set(p >> 2 for p in positions))
+            rows = [[x for v in values[4 * k:4 * k + 4] for x in v] for k in range(len(mpos))]
+            ok = merkle_verify(lp["commitment"], log - 2, mpos, rows, lp["hash_witness"])
+        else:
+            ok = merkle_verify(lp["commitment"], log, positions, values, lp["hash_witness"])
+        assert ok, "CommitmentInvalid"
         return subsets
 
-    # first layer: fold_circle on each subset (fri.rs:727-756)
-    subsets = rebuild(queries, first_layer_query_evals_fn(queries), proof["first_layer"], col_log)
+    subsets = rebuild(queries, first_layer_query_evals_fn(queries), proof["first_layer"], col_log, fs, pack0)
     half_init = O.canonic_half_initial(col_log)
-    evals = []
-    for start, sub in subsets:
-        p = circle_domain_at(half_init, col_log - 1, bitrev(start, col_log))
-        yi = O.m31_inv(int(p[1]))
-        f0, f1 = qadd(sub[0], sub[1]), qmulm(qsub(sub[0], sub[1]), yi)
-        evals.append(qadd(qmul(alpha0, f1), f0))
-    qs = sorted(set(q >> 1 for q in queries))
-    # inner layers: verify_and_fold (fri.rs:561-621), fold_coset of size 2 (fri.rs:785-804)
-    for li, llog, alpha, lp in inner:
-        subsets = rebuild(qs, evals, lp, llog)
-        evals = []
-        for start, sub in subsets:
-            x = coset_at(li, llog, bitrev(start, llog))[0]
-            xi = O.m31_inv(int(x))
-            f0, f1 = qadd(sub[0], sub[1]), qmulm(qsub(sub[0], sub[1]), xi)
-            evals.append(qadd(f0, qmul(alpha, f1)))
-        qs = sorted(set(q >> 1 for q in qs))
-    # last layer (fri.rs:347-362)
+    evals = [fold_circle(sub, _circle_domain_index(half_init, col_log - 1, bitrev(start, col_log)), fs, alpha0)
+             for start, sub in subsets]
+    qs = _fold_queries(queries, fs)
+    for li, llog, alpha, lp, step, pack in inner:
+        subsets = rebuild(qs, evals, lp, llog, step, pack)
+        evals = [fold_coset(sub, (li + (1 << (31 - llog)) * bitrev(start, llog)) % (1 << 31), step, alpha)
+                 for start, sub in subsets]
+        qs = _fold_queries(qs, step)
+    # last layer (fri.rs:320-334)
     for q, e in zip(qs, evals):
         x = coset_at(line_init, line_log, bitrev(q, line_log))[0]
         assert e == _line_eval_at_point(proof["last_layer_poly"], [int(x), 0, 0, 0]), "LastLayerEvaluationsInvalid"

@@ -13,7 +13,8 @@
 //
 // Streaming rule (the `blake2` crate): a full block is compressed only when
 // more data follows; the last block, full or not, is compressed with the final
-// flag.  Pending words therefore range over 1..16 between groups.
+// flag, except that a group followed by another compresses its own last full block: between
+// groups 0..15 words are pending.
 #include "../../include/stwo_hip.h"
 #include "blake2s.h"
 #include "common.h"
@@ -40,11 +41,14 @@ __device__ __forceinline__ void load_msg(const uint32_t* lm, uint32_t m[16]) {
   for (int j = 0; j < 16; j++) m[j] = lm[j];
 }
 
+// One lane per row.  Whole 16-column blocks are loaded straight into registers (16 independent
+// loads in flight) while the previous block is compressed; only a partial block at either end of
+// the group goes through the lane's LDS slot.
 __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
   __shared__ uint32_t msg_lds[kThreads * kMsgStride];
   const uint32_t rows = 1u << a.log_n;
   const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
-  const bool active = r < rows;
+  if (r >= rows) return;  // no barriers below: each lane only touches its own LDS slot
   uint32_t* lm = msg_lds + threadIdx.x * kMsgStride;
   uint32_t h[8];
   uint32_t pend = a.pend_in, t = a.t_in;
@@ -52,49 +56,61 @@ __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
     const uint32_t in_rows = 1u << a.in_log;
     const uint32_t ratio = a.log_n - a.in_log;
     const uint32_t parent = ((r >> (ratio + 1)) << 1) + (r & 1);
-    if (active) {
 #pragma unroll
-      for (int w = 0; w < 8; w++) h[w] = a.state_in[w * in_rows + parent];
-      for (uint32_t j = 0; j < pend; j++) lm[j] = a.state_in[(8 + j) * in_rows + parent];
-    }
+    for (int w = 0; w < 8; w++) h[w] = a.state_in[w * in_rows + parent];
+    for (uint32_t j = 0; j < pend; j++) lm[j] = a.state_in[(8 + j) * in_rows + parent];
   } else {
     sh::b2_init(h);
   }
-  for (uint32_t k = 0; k < a.cnt; k++) {
+  const uint32_t cnt = a.cnt;
+  uint32_t k = 0;
+  uint32_t m[16];
+  bool full = false;  // m holds a full block that is compressed once more data is known to follow
+  if (pend) {         // complete the block carried in from the previous group
+    while (pend < 16 && k < cnt) lm[pend++] = a.cols[k++][r];
     if (pend == 16) {
-      t += 64;
-      if (active) {
-        uint32_t m[16];
-        load_msg(lm, m);
-        sh::b2_compress(h, m, t, false);
-      }
+      load_msg(lm, m);
+      full = true;
       pend = 0;
     }
-    const uint32_t* col = a.cols[k];
-    if (active) lm[pend] = col[r];
-    pend++;
   }
-  if (a.state_out && pend == 16) {
-    // More data follows this group, so a full pending block can be compressed
-    // here, at the smaller size, instead of once per lifted row later.
-    t += 64;
-    if (active) {
-      uint32_t m[16];
-      load_msg(lm, m);
+  while (k + 16 <= cnt) {  // (pend is 0 whenever this runs)
+    uint32_t n[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) n[j] = a.cols[k + j][r];
+    if (full) {
+      t += 64;
       sh::b2_compress(h, m, t, false);
     }
-    pend = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) m[j] = n[j];
+    full = true;
+    k += 16;
   }
-  if (!active) return;
+  if (k < cnt) {  // a tail of fewer than 16 words
+    if (full) {
+      t += 64;
+      sh::b2_compress(h, m, t, false);
+      full = false;
+    }
+    while (k < cnt) lm[pend++] = a.cols[k++][r];
+  }
   if (a.state_out) {
+    if (full) {  // more data follows in a later group: compress here, at the smaller size
+      t += 64;
+      sh::b2_compress(h, m, t, false);
+    }
 #pragma unroll
     for (int w = 0; w < 8; w++) a.state_out[w * rows + r] = h[w];
     for (uint32_t j = 0; j < pend; j++) a.state_out[(8 + j) * rows + r] = lm[j];
   } else {
-    t += 4 * pend;
-    for (uint32_t j = pend; j < 16; j++) lm[j] = 0;
-    uint32_t m[16];
-    load_msg(lm, m);
+    if (full) {
+      t += 64;
+    } else {
+      t += 4 * pend;
+      for (uint32_t j = pend; j < 16; j++) lm[j] = 0;
+      load_msg(lm, m);
+    }
     sh::b2_compress(h, m, t, true);
     uint4* o = (uint4*)(a.hash_out + 8 * r);
     o[0] = make_uint4(h[0], h[1], h[2], h[3]);

@@ -132,6 +132,38 @@ int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32
   return SH_OK;
 }
 
+// PackLeavesOps::pack_leaves_input (crates/stwo/src/prover/backend/cpu/merkle_lifted.rs:101-124):
+// output column coord + 4*offset, row i, holds coordinate coord of value 4i + offset.
+__global__ void k_pack_leaves(const uint32_t* c0, const uint32_t* c1, const uint32_t* c2, const uint32_t* c3,
+                              uint32_t n_out, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const uint32_t* cols[4] = {c0, c1, c2, c3};
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int off = 0; off < 4; off++) out[(uint64_t)(c + 4 * off) * n_out + i] = cols[c][4ull * i + off];
+}
+
+// FriFirstLayerProver::new / FriInnerLayerProver::new (crates/stwo/src/prover/fri.rs:294-306,
+// 357-368): a tree over the 4 coordinate columns, one leaf per value, or with packed leaves one
+// leaf per 4 consecutive values (16 words: one Blake2s block) and a tree 2 levels shorter.
+int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_size, bool pack, DeviceTree& t) {
+  if (!pack) return commit_columns(ctx, {cols[0], cols[1], cols[2], cols[3]}, log_size, t);
+  const uint32_t n_out = 1u << (log_size - 2);
+  uint32_t* packed;
+  SH_TRY(sh_dev_alloc(ctx, 64ull * n_out, (void**)&packed));
+  k_pack_leaves<<<(n_out + 255) / 256, 256, 0, ctx->stream>>>(cols[0], cols[1], cols[2], cols[3], n_out, packed);
+  int rc = sh_launch_check(ctx, "pack_leaves");
+  if (rc == SH_OK) {
+    std::vector<const uint32_t*> pc(16);
+    for (int k = 0; k < 16; k++) pc[k] = packed + (uint64_t)k * n_out;
+    rc = commit_columns(ctx, pc, log_size - 2, t);  // synchronises the stream
+  }
+  sh_dev_free(ctx, packed);
+  return rc;
+}
+
 // Serialisation helpers (crates/cairo-serialize/src/serialize.rs).
 struct Felts {
   std::vector<uint64_t> v;
@@ -171,6 +203,8 @@ struct FriLayer {
   uint32_t* buf = nullptr;  // 4 * 2^log_size words (pooled)
   const uint32_t* cols[4] = {nullptr, nullptr, nullptr, nullptr};
   DeviceTree tree;
+  uint32_t fold_step = 1;  // folds from this layer to the next
+  bool pack = false;       // leaves of 4 consecutive values
   void free_all() {
     if (buf) sh_dev_free(owner, buf);
     buf = nullptr;
@@ -178,19 +212,22 @@ struct FriLayer {
   }
 };
 
-// compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438) for
-// fold_step 1, followed by the layer's Merkle decommitment.
-int decommit_layer(sh_ctx* ctx, const uint32_t* const cols[4], const DeviceTree& tree,
-                   const std::vector<uint32_t>& queries, LayerProof& out) {
+// compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438): every
+// coset of 2^fold_step positions holding a query is opened and the values the verifier cannot
+// rebuild go to the witness; then the layer's Merkle decommitment (fri.rs:308-341, 370-395), at
+// the positions divided by 4 when the leaves are packed.
+int decommit_layer(sh_ctx* ctx, const uint32_t* const cols[4], const DeviceTree& tree, const std::vector<uint32_t>& queries,
+                   uint32_t fold_step, bool pack, LayerProof& out) {
   std::vector<uint32_t> positions;
   std::vector<bool> is_query;
   for (size_t i = 0; i < queries.size();) {
-    uint32_t start = (queries[i] >> 1) << 1;
+    const uint32_t start = (queries[i] >> fold_step) << fold_step;
     size_t j = i;
-    while (j < queries.size() && (queries[j] >> 1) == (queries[i] >> 1)) j++;
-    for (uint32_t p = start; p < start + 2; p++) {
-      bool q = false;
-      for (size_t k = i; k < j; k++) q |= queries[k] == p;
+    while (j < queries.size() && (queries[j] >> fold_step) == (queries[i] >> fold_step)) j++;
+    size_t k = i;
+    for (uint32_t p = start; p < start + (1u << fold_step); p++) {
+      const bool q = k < j && queries[k] == p;
+      if (q) k++;
       positions.push_back(p);
       is_query.push_back(q);
     }
@@ -208,8 +245,13 @@ int decommit_layer(sh_ctx* ctx, const uint32_t* const cols[4], const DeviceTree&
   SH_TRY(gather_words(ctx, gc, gr, words));
   out.fri_witness.clear();
   for (size_t k = 0; k < words.size(); k += 4) out.fri_witness.push_back(sh::qload(&words[k]));
+  std::vector<uint32_t> mpos;
+  for (uint32_t p : positions) {
+    const uint32_t m = pack ? p >> 2 : p;
+    if (mpos.empty() || mpos.back() != m) mpos.push_back(m);
+  }
   std::vector<const uint8_t*> layers(tree.layers.begin(), tree.layers.end());
-  SH_TRY(merkle_decommit(ctx, layers, tree.L, positions, out.hash_witness));
+  SH_TRY(merkle_decommit(ctx, layers, tree.L, mpos, out.hash_witness));
   out.commitment = tree.root;
   return SH_OK;
 }
@@ -324,22 +366,24 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
                             const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
                             uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
   if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions) return SH_ERR_INVALID;
-  SH_REQUIRE(ctx, p->fold_step == 1, "fold_step %u: only fold_step 1 (unpacked leaves) is implemented",
-             p->fold_step);
+  SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 16, "fold_step %u out of range", p->fold_step);
   SH_REQUIRE(ctx, p->log_blowup_factor >= 1 && p->log_blowup_factor <= 16 && p->log_last_layer_degree_bound <= 10,
              "invalid FRI parameters");
+  const uint32_t fs = p->fold_step;
   const uint32_t last_log = p->log_last_layer_degree_bound + p->log_blowup_factor;
-  SH_REQUIRE(ctx, log_size >= 2 && log_size - 1 >= last_log,
+  SH_REQUIRE(ctx, log_size >= 2 && log_size <= 30 && log_size >= fs + last_log,
              "The circle-to-line fold results in a smaller line domain than the last layer.");
   int rc = SH_OK;
   FriLayer first;
   std::vector<FriLayer> inner;
   FriLayer cur;  // the evaluation being folded (not committed yet)
+  FriLayer tmp;  // ping-pong buffer of a multi-step fold
   std::vector<sh::qm31> last_coeffs;
   auto cleanup = [&]() {
     first.tree.free_all();
     for (auto& l : inner) l.free_all();
     cur.free_all();
+    tmp.free_all();
   };
 #define TRY_CLEAN(expr)    \
   do {                     \
@@ -349,10 +393,12 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
       return rc;           \
     }                      \
   } while (0)
-  // commit_first_layer (fri.rs:130-140)
+  // commit_first_layer (fri.rs:130-140): packed leaves when fold_step > 1
   for (int c = 0; c < 4; c++) first.cols[c] = column[c];
   first.log_size = log_size;
-  TRY_CLEAN(commit_columns(ctx, {column[0], column[1], column[2], column[3]}, log_size, first.tree));
+  first.fold_step = fs;
+  first.pack = log_size >= 2 && fs > 1;
+  TRY_CLEAN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree));
   sh_channel_mix_root(ch, first.tree.root.data());
   // commit_inner_layers (fri.rs:145-196)
   auto alloc_eval = [&](FriLayer& l, uint32_t lg) -> int {
@@ -363,6 +409,34 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
     for (int c = 0; c < 4; c++) l.cols[c] = l.buf + ((uint64_t)c << lg);
     return SH_OK;
   };
+  // k line folds of cur (log line_log) with alphas[0..k) (FriOps::fold_line with
+  // squared_alpha_powers, fri.rs:79-87); the result replaces cur
+  uint32_t line_log = log_size - 1;
+  uint32_t line_init = sh::subgroup_gen(line_log + 2);  // half_odds(line_log)
+  auto fold_lines = [&](const std::vector<uint32_t>& alphas) -> int {
+    for (size_t k = 0; k < alphas.size() / 4; k++) {
+      int r = alloc_eval(tmp, line_log - 1);
+      if (r != SH_OK) return r;
+      uint32_t* dst[4] = {tmp.buf, tmp.buf + (1ull << (line_log - 1)), tmp.buf + (2ull << (line_log - 1)),
+                          tmp.buf + (3ull << (line_log - 1))};
+      r = sh_fold_line(ctx, tw, cur.cols, line_init, line_log, &alphas[4 * k], 1, dst);
+      if (r != SH_OK) return r;
+      cur.free_all();  // stream-ordered: the fold that reads it is queued first
+      std::swap(cur, tmp);
+      line_log--;
+      line_init = sh::idx_reduce((uint64_t)line_init * 2);
+    }
+    return SH_OK;
+  };
+  auto alpha_powers = [](uint32_t a[4], uint32_t n) {  // squared_alpha_powers (fri.rs:79-87)
+    std::vector<uint32_t> v(4 * n);
+    sh::qm31 x = sh::qload(a);
+    for (uint32_t k = 0; k < n; k++) {
+      sh::qstore(x, &v[4 * k]);
+      x = sh::qmul(x, x);
+    }
+    return v;
+  };
   uint32_t alpha[4];
   sh_channel_draw_secure_felt(ch, alpha);
   TRY_CLEAN(alloc_eval(cur, log_size - 1));
@@ -371,32 +445,42 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
                         cur.buf + (3ull << (log_size - 1))};
     TRY_CLEAN(sh_fold_circle_into_line(ctx, tw, column, log_size, alpha, dst));
   }
-  uint32_t line_log = log_size - 1;
-  uint32_t line_init = sh::subgroup_gen(line_log + 2);  // half_odds(line_log)
+  {  // the circle-to-line fold counts as one of the first layer's fold_step folds (fri.rs:157-163)
+    uint32_t a2[4];
+    sh::qstore(sh::qmul(sh::qload(alpha), sh::qload(alpha)), a2);
+    TRY_CLEAN(fold_lines(alpha_powers(a2, fs - 1)));
+  }
   while (line_log > last_log) {
-    // FriInnerLayerProver::new + mix root + draw alpha + fold_line (one step: fold_step is 1)
-    TRY_CLEAN(commit_columns(ctx, {cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3]}, line_log, cur.tree));
+    // FriInnerLayerProver::new + mix root + draw alpha + fold_line; the last inner layer folds
+    // exactly down to the last layer's domain (fri.rs:175-194)
+    const uint32_t k = line_log > last_log + fs ? fs : line_log - last_log;
+    cur.fold_step = k;
+    cur.pack = line_log >= 2 && k > 1;
+    TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree));
     sh_channel_mix_root(ch, cur.tree.root.data());
     sh_channel_draw_secure_felt(ch, alpha);
-    FriLayer next;
-    TRY_CLEAN(alloc_eval(next, line_log - 1));
-    uint32_t* dst[4] = {next.buf, next.buf + (1ull << (line_log - 1)), next.buf + (2ull << (line_log - 1)),
-                        next.buf + (3ull << (line_log - 1))};
-    rc = sh_fold_line(ctx, tw, cur.cols, line_init, line_log, alpha, 1, dst);
+    // keep this layer's evaluation (the decommitment reads it): fold from a copy of the handle
+    FriLayer kept = cur;
+    cur.buf = nullptr;
+    cur.tree = DeviceTree();
+    for (int c = 0; c < 4; c++) cur.cols[c] = kept.cols[c];
+    rc = fold_lines(alpha_powers(alpha, k));
+    inner.push_back(kept);
     if (rc != SH_OK) {
-      next.free_all();
       cleanup();
       return rc;
     }
-    inner.push_back(cur);
-    cur = next;
-    line_log--;
-    line_init = sh::idx_reduce((uint64_t)line_init * 2);
   }
   // commit_last_layer (fri.rs:208-226)
   {
     std::vector<uint32_t> words(4ull << last_log);
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data(), cur.buf, 16ull << last_log, hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data(), cur.cols[0], 4ull << last_log, hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (1ull << last_log), cur.cols[1], 4ull << last_log,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (2ull << last_log), cur.cols[2], 4ull << last_log,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (3ull << last_log), cur.cols[3], 4ull << last_log,
+                                     hipMemcpyDeviceToHost, ctx->stream));
     SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<sh::qm31> v(1ull << last_log);
     for (uint64_t i = 0; i < v.size(); i++)
@@ -427,15 +511,20 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   sh_channel_draw_queries(ch, log_size, p->n_queries, q.data());
   std::sort(q.begin(), q.end());
   q.erase(std::unique(q.begin(), q.end()), q.end());
-  // decommit_on_queries (fri.rs:253-284)
+  // decommit_on_queries (fri.rs:253-284): queries fold by each layer's own step
+  auto fold_queries = [](std::vector<uint32_t>& v, uint32_t k) {  // Queries::fold (queries.rs:48-55)
+    for (auto& x : v) x >>= k;
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
   LayerProof first_proof;
-  TRY_CLEAN(decommit_layer(ctx, first.cols, first.tree, q, first_proof));
+  TRY_CLEAN(decommit_layer(ctx, first.cols, first.tree, q, fs, first.pack, first_proof));
   std::vector<LayerProof> inner_proofs(inner.size());
   std::vector<uint32_t> lq = q;
+  fold_queries(lq, fs);
   for (size_t k = 0; k < inner.size(); k++) {
-    for (auto& x : lq) x >>= 1;  // Queries::fold(1) (crates/stwo/src/core/queries.rs:48-55)
-    lq.erase(std::unique(lq.begin(), lq.end()), lq.end());
-    TRY_CLEAN(decommit_layer(ctx, inner[k].cols, inner[k].tree, lq, inner_proofs[k]));
+    TRY_CLEAN(decommit_layer(ctx, inner[k].cols, inner[k].tree, lq, inner[k].fold_step, inner[k].pack,
+                             inner_proofs[k]));
+    fold_queries(lq, inner[k].fold_step);
   }
   // FriProof (crates/cairo-serialize/src/serialize.rs:79-90)
   Felts f;

@@ -237,3 +237,46 @@ def preprocessed_positions(queries, lifting_log, pp_log):
     if lifting_log < pp_log:
         return [(p >> 1 << (pp_log - lifting_log + 1)) + (p & 1) for p in queries]
     return [(p >> (lifting_log - pp_log + 1) << 1) + (p & 1) for p in queries]
+
+
+def merkle_witness_count(positions, height):
+    """Number of hashes MerkleProverLifted::decommit sends for these leaf positions
+    (crates/stwo/src/prover/vcs_lifted/prover.rs:118-152): one per node whose sibling is not known."""
+    q = sorted(set(positions))
+    n = 0
+    for _ in range(height):
+        nxt, i = [], 0
+        while i < len(q):
+            first = q[i]
+            if i + 1 < len(q) and q[i + 1] == first ^ 1:
+                i += 2
+            else:
+                n += 1
+                i += 1
+            nxt.append(first >> 1)
+        q = nxt
+    return n
+
+
+def fri_opening_shape(queries, col_log, params):
+    """(values in fri_witness, hashes in the Merkle decommitment) of the first and every inner FRI
+    layer that FriProver::decommit_on_queries (crates/stwo/src/prover/fri.rs:253-284, 408-438) sends
+    for these queries: the layer schedule of commit_inner_layers (fri.rs:145-196), cosets of
+    2^fold_step positions, packed leaves when a layer folds more than once."""
+    fs = params["fold_step"]
+    last_log = params["log_last_layer_degree_bound"] + params["log_blowup_factor"]
+    layers = [(col_log, fs, col_log >= 2 and fs > 1)]
+    line_log = col_log - fs
+    while line_log > last_log:
+        k = fs if line_log > last_log + fs else line_log - last_log
+        layers.append((line_log, k, line_log >= 2 and k > 1))
+        line_log -= k
+    shape, qs = [], sorted(set(queries))
+    for log, k, pack in layers:
+        cosets = sorted(set(q >> k for q in qs))
+        positions = [p for c in cosets for p in range(c << k, (c + 1) << k)]
+        n_values = len(positions) - len(qs)
+        mpos = sorted(set(p >> 2 for p in positions)) if pack else positions
+        shape.append((n_values, merkle_witness_count(mpos, log - 2 if pack else log)))
+        qs = cosets
+    return shape

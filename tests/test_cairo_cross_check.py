@@ -13,8 +13,10 @@ Checked with this build's code:
     trace, interaction, composition: lifted Merkle trees over columns of mixed sizes, the
     preprocessed one at its own height) against the proof's roots; on the GPU the leaf hashes are
     recomputed by the library's leaf kernel.
-Not checked: the FRI layer openings, whose first-layer values are the DEEP quotients of the Cairo
-AIR's masks (the AIR is not part of this build).
+  * the FRI layer schedule and opening sizes (fold_step 3, packed leaves) equal those this build's
+    FRI driver produces for the same queries.
+Not checked: the values inside the FRI layer openings, whose first-layer values are the DEEP
+quotients of the Cairo AIR's masks (the AIR is not part of this build).
 """
 import ctypes
 import os
@@ -121,6 +123,19 @@ def test_decommitments_open_against_reference_roots(proof):
     for t in (1, 2):
         assert PR.merkle_verify(proof["commitments"][t], L, q, CT.tree_rows(proof, t, len(q)),
                                 proof["decommitments"][t])
+
+
+def test_fri_opening_shape_matches_reference(proof):
+    """The FRI layer schedule, coset openings and packed-leaf decommitments this build's FRI driver
+    makes for the drawn queries have exactly the sizes of the reference proof's (fold_step 3: six
+    inner layers, the last folding by 2)."""
+    L, q, _ = _open(proof)
+    fri = proof["fri"]
+    got = [(len(fri["first_layer"]["fri_witness"]), len(fri["first_layer"]["decommitment"]))]
+    got += [(len(lp["fri_witness"]), len(lp["decommitment"])) for lp in fri["inner_layers"]]
+    assert CT.fri_opening_shape(q, L, proof["config"]) == got
+    # and with the queries of a wrong lifting size the shape differs
+    assert CT.fri_opening_shape([x >> 1 for x in q], L - 1, proof["config"]) != got
 
 
 def test_tampered_openings_rejected(proof):

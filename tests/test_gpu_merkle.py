@@ -52,7 +52,11 @@ def test_trees_match_golden(ctx, backend, golden):
 
 
 @pytest.mark.parametrize("logs,L", [([4] * 16, 4), ([1] * 3, 1), ([5] * 40 + [7] * 17 + [9] * 5, 11),
-                                    ([2, 3, 3, 6, 6, 6, 10], 10), ([12] * 64, 12), ([8] * 1, 13)])
+                                    ([2, 3, 3, 6, 6, 6, 10], 10), ([12] * 64, 12), ([8] * 1, 13),
+                                    # group ends on a block boundary with more groups to come; a carried
+                                    # partial block completed exactly at a group's end; long groups
+                                    ([3] * 16 + [5] * 16 + [6] * 20, 6), ([3] * 10 + [4] * 6 + [5] * 3, 7),
+                                    ([6] * 100 + [8] * 33 + [9] * 15, 9)])
 def test_lifted_tree_matches_oracle(ctx, backend, logs, L):
     rng = np.random.default_rng(len(logs) * 31 + L)
     host = [rng.integers(0, P, 1 << lg, dtype=np.uint32) for lg in logs]

@@ -125,6 +125,13 @@ def params_of(d):
      [3, 3, 9]),
     ({"pow_bits": 12, "log_blowup_factor": 1, "log_last_layer_degree_bound": 3, "n_queries": 70, "fold_step": 1},
      [6, 12, 9, 11, 12]),
+    # fold_step > 1: packed leaves, multi-step folds, a shorter last fold
+    ({"pow_bits": 0, "log_blowup_factor": 2, "log_last_layer_degree_bound": 2, "n_queries": 3, "fold_step": 2},
+     [8, 6, 8]),
+    ({"pow_bits": 6, "log_blowup_factor": 1, "log_last_layer_degree_bound": 0, "n_queries": 40, "fold_step": 3},
+     [6, 12, 9, 11, 12]),
+    ({"pow_bits": 4, "log_blowup_factor": 2, "log_last_layer_degree_bound": 1, "n_queries": 20, "fold_step": 4},
+     [10, 7, 10]),
 ])
 def test_fri_prove_matches_oracle_and_verifies(ctx, backend, pdict, logs):
     b = pdict["log_blowup_factor"]
@@ -167,7 +174,10 @@ def test_fri_prove_rejects_high_degree(ctx, backend):
     with pytest.raises(HipError, match="invalid degree"):
         fri_prove(backend, tree, Channel(), p, col)
     with pytest.raises(HipError, match="fold_step"):
-        fri_prove(backend, tree, Channel(), FriParams(4, 1, 0, 3, 2), col)
+        fri_prove(backend, tree, Channel(), FriParams(4, 1, 0, 3, 0), col)
+    # the first layer's folds must not pass the last layer (fri.rs:165-168)
+    with pytest.raises(HipError, match="smaller line domain"):
+        fri_prove(backend, tree, Channel(), FriParams(4, 1, 1, 3, 7), col)
 
 
 def test_deep_fri_chain_on_device(ctx, backend):

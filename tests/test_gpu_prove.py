@@ -16,13 +16,15 @@ pytestmark = pytest.mark.gpu
 P = O.P
 
 
-def params(pow_bits=4, last=0, n_queries=3):
+def params(pow_bits=4, last=0, n_queries=3, fold_step=1):
     return {"pow_bits": pow_bits, "log_blowup_factor": 1, "log_last_layer_degree_bound": last,
-            "n_queries": n_queries, "fold_step": 1}
+            "n_queries": n_queries, "fold_step": fold_step}
 
 
 @pytest.mark.parametrize("n_cols,log_n,fri", [
-    (4, 4, params()), (12, 6, params(6, 1, 9)), (32, 10, params(8, 2, 20)), (64, 13, params(10, 0, 70))])
+    (4, 4, params()), (12, 6, params(6, 1, 9)), (32, 10, params(8, 2, 20)), (64, 13, params(10, 0, 70)),
+    # the fold step of the reference's Cairo proofs (packed FRI leaves)
+    (16, 9, params(6, 0, 30, 3)), (8, 7, params(4, 1, 12, 2))])
 def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n, fri):
     trace = wide_fib_trace(n_cols, log_n, 100 + n_cols)
     tw, itw = O.canonic_twiddles(log_n + 1)

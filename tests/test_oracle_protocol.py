@@ -150,6 +150,11 @@ def fri_round_trip(params, seed=0):
 @pytest.mark.parametrize("params", [
     fri_test_params(),
     {"pow_bits": 4, "log_blowup_factor": 2, "log_last_layer_degree_bound": 2, "n_queries": 20, "fold_step": 1},
+    # fold_step > 1 with packed leaves (crates/stwo/src/prover/fri.rs:512-532 test parameters)
+    {"pow_bits": 0, "log_blowup_factor": 2, "log_last_layer_degree_bound": 2, "n_queries": 3, "fold_step": 2},
+    {"pow_bits": 0, "log_blowup_factor": 2, "log_last_layer_degree_bound": 2, "n_queries": 3, "fold_step": 3},
+    {"pow_bits": 0, "log_blowup_factor": 2, "log_last_layer_degree_bound": 2, "n_queries": 3, "fold_step": 4},
+    {"pow_bits": 6, "log_blowup_factor": 1, "log_last_layer_degree_bound": 0, "n_queries": 30, "fold_step": 3},
 ])
 def test_fri_prove_verify_round_trip(params):
     proof, queries, ch, L, fle = fri_round_trip(params)
@@ -160,7 +165,8 @@ def test_fri_prove_verify_round_trip(params):
     vq = PR.fri_verify(PR.deserialize_fri_proof(felts), params, chv, L - params["log_blowup_factor"], fle)
     assert vq == queries
     assert chv.digest == ch.digest
-    assert len(proof["inner_layers"]) == L - 1 - (params["log_last_layer_degree_bound"] + params["log_blowup_factor"])
+    fs, last_log = params["fold_step"], params["log_last_layer_degree_bound"] + params["log_blowup_factor"]
+    assert len(proof["inner_layers"]) == -(-(L - fs - last_log) // fs)
     assert len(proof["last_layer_poly"]) == 1 << params["log_last_layer_degree_bound"]
     assert O.verify_pow_nonce(_digest_before_pow(proof, params, L), params["pow_bits"], proof["proof_of_work"])
 
@@ -177,14 +183,16 @@ def _digest_before_pow(proof, params, L):
     return ch.digest
 
 
+@pytest.mark.parametrize("fold_step", [1, 3])
 @pytest.mark.parametrize("tamper", ["fri_witness", "hash_witness", "last_layer", "pow", "inner_commitment",
                                     "first_eval"])
-def test_fri_verify_rejects_tampering(tamper):
-    params = {"pow_bits": 4, "log_blowup_factor": 1, "log_last_layer_degree_bound": 1, "n_queries": 8, "fold_step": 1}
+def test_fri_verify_rejects_tampering(tamper, fold_step):
+    params = {"pow_bits": 4, "log_blowup_factor": 1, "log_last_layer_degree_bound": 1, "n_queries": 8,
+              "fold_step": fold_step}
     proof, queries, ch, L, fle = fri_round_trip(params, seed=2)
     evals_fn = fle
     if tamper == "fri_witness":
-        lp = proof["inner_layers"][1]
+        lp = proof["inner_layers"][1 if fold_step == 1 else 0]
         lp["fri_witness"][0] = [(lp["fri_witness"][0][0] + 1) % P] + list(lp["fri_witness"][0][1:])
     elif tamper == "hash_witness":
         proof["first_layer"]["hash_witness"][0] = bytes(32)
