@@ -43,12 +43,18 @@ struct Cols4 {
   uint32_t* d[4];
 };
 
+// Alphas come by value, or from device memory (d_al != nullptr: 4 words per fold, written by an
+// earlier kernel on the stream, e.g. the device-side channel of the FRI driver).
 template <int F>
 __global__ __launch_bounds__(256) void k_fold_line(Cols4 io, const uint32_t* __restrict__ itw, uint32_t K,
-                                                   uint32_t n, Alphas al) {
+                                                   uint32_t n, Alphas al, const uint32_t* __restrict__ d_al) {
   const uint32_t n_out = 1u << (n - F);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (d_al) {
+#pragma unroll
+    for (int f = 0; f < F; f++) al.a[f] = sh::qload(d_al + 4 * f);
+  }
   sh::qm31 v[1 << F];
 #pragma unroll
   for (int j = 0; j < (1 << F); j++) v[j] = ld4(io.s, (i << F) + j);
@@ -72,10 +78,12 @@ __global__ __launch_bounds__(256) void k_fold_line(Cols4 io, const uint32_t* __r
 // y^-1 from the circle layer of the inverse tree; domains of log size <= 2 take
 // host-computed inverses (c0, c1) for outputs 0 and 1.
 __global__ __launch_bounds__(256) void k_fold_circle(Cols4 io, const uint32_t* __restrict__ itw, uint32_t K,
-                                                     uint32_t m, sh::qm31 alpha, uint32_t c0, uint32_t c1) {
+                                                     uint32_t m, sh::qm31 alpha, const uint32_t* __restrict__ d_al,
+                                                     uint32_t c0, uint32_t c1) {
   const uint32_t n_out = 1u << (m - 1);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
+  if (d_al) alpha = sh::qload(d_al);
   uint32_t yinv;
   if (m <= 2) {
     yinv = i == 0 ? c0 : c1;
@@ -127,10 +135,9 @@ __global__ __launch_bounds__(256) void k_decompose_apply(Cols4 io, uint32_t n, c
 
 }  // namespace
 
-extern "C" int sh_fold_line(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
-                            uint32_t line_initial_index, uint32_t log_size, const uint32_t* alphas,
+static int fold_line_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
+                            uint32_t line_initial_index, uint32_t log_size, const Alphas& al, const uint32_t* d_al,
                             uint32_t fold_step, uint32_t* const dst[4]) {
-  if (!ctx || !tw || !src || !dst || !alphas) return SH_ERR_INVALID;
   SH_REQUIRE(ctx, fold_step >= 1 && fold_step <= 4, "fold_step %u not in [1, 4]", fold_step);
   SH_REQUIRE(ctx, log_size >= fold_step && log_size <= 30, "Evaluation too small (log %u, fold_step %u)",
              log_size, fold_step);
@@ -141,23 +148,20 @@ extern "C" int sh_fold_line(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* 
     io.s[c] = src[c];
     io.d[c] = dst[c];
   }
-  Alphas al;
-  for (uint32_t f = 0; f < fold_step; f++) al.a[f] = sh::qload(alphas + 4 * f);
   uint32_t n_out = 1u << (log_size - fold_step);
   unsigned grid = (n_out + 255) / 256;
   SpanGuard span(ctx, "fri_fold_line");
   switch (fold_step) {
-    case 1: k_fold_line<1><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al); break;
-    case 2: k_fold_line<2><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al); break;
-    case 3: k_fold_line<3><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al); break;
-    default: k_fold_line<4><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al); break;
+    case 1: k_fold_line<1><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
+    case 2: k_fold_line<2><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
+    case 3: k_fold_line<3><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
+    default: k_fold_line<4><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
   }
   return sh_launch_check(ctx, "fold_line");
 }
 
-extern "C" int sh_fold_circle_into_line(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
-                                        uint32_t log_size, const uint32_t alpha[4], uint32_t* const dst[4]) {
-  if (!ctx || !tw || !src || !dst || !alpha) return SH_ERR_INVALID;
+static int fold_circle_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
+                              sh::qm31 alpha, const uint32_t* d_al, uint32_t* const dst[4]) {
   SH_REQUIRE(ctx, log_size >= 1 && log_size <= 30, "circle domain log size %u out of range", log_size);
   SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(log_size + 1), log_size - 1, tw->root_initial_index,
                                      tw->root_log_size),
@@ -177,9 +181,36 @@ extern "C" int sh_fold_circle_into_line(sh_ctx* ctx, const sh_twiddles* tw, cons
   }
   uint32_t n_out = 1u << (log_size - 1);
   SpanGuard span(ctx, "fri_fold_circle");
-  k_fold_circle<<<(n_out + 255) / 256, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size,
-                                                             sh::qload(alpha), c0, c1);
+  k_fold_circle<<<(n_out + 255) / 256, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, alpha, d_al,
+                                                             c0, c1);
   return sh_launch_check(ctx, "fold_circle_into_line");
+}
+
+extern "C" int sh_fold_line(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
+                            uint32_t line_initial_index, uint32_t log_size, const uint32_t* alphas,
+                            uint32_t fold_step, uint32_t* const dst[4]) {
+  if (!ctx || !tw || !src || !dst || !alphas) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, fold_step >= 1 && fold_step <= 4, "fold_step %u not in [1, 4]", fold_step);
+  Alphas al;
+  for (uint32_t f = 0; f < fold_step; f++) al.a[f] = sh::qload(alphas + 4 * f);
+  return fold_line_launch(ctx, tw, src, line_initial_index, log_size, al, nullptr, fold_step, dst);
+}
+
+extern "C" int sh_fold_circle_into_line(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
+                                        uint32_t log_size, const uint32_t alpha[4], uint32_t* const dst[4]) {
+  if (!ctx || !tw || !src || !dst || !alpha) return SH_ERR_INVALID;
+  return fold_circle_launch(ctx, tw, src, log_size, sh::qload(alpha), nullptr, dst);
+}
+
+// The same folds with the alphas read on the device (4 words per fold at d_alphas), for drivers
+// that draw them with a device-side channel and so never wait for the host between layers.
+int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t line_initial_index,
+                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4]) {
+  return fold_line_launch(ctx, tw, src, line_initial_index, log_size, Alphas{}, d_alphas, fold_step, dst);
+}
+int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
+                       const uint32_t* d_alpha, uint32_t* const dst[4]) {
+  return fold_circle_launch(ctx, tw, src, log_size, sh::qzero(), d_alpha, dst);
 }
 
 extern "C" int sh_fri_decompose(sh_ctx* ctx, const uint32_t* const src[4], uint32_t log_size,

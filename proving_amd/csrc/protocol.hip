@@ -16,6 +16,10 @@
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
                          uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready);
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers);
+int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t line_initial_index,
+                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4]);
+int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
+                       const uint32_t* d_alpha, uint32_t* const dst[4]);
 
 namespace {
 
@@ -67,6 +71,26 @@ int gather_words(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const st
   return SH_OK;
 }
 
+// The nodes (layer, index) of MerkleProverLifted::decommit's hash witness, in witness order
+// (crates/stwo/src/prover/vcs_lifted/prover.rs:118-152): one per node whose sibling is not known.
+std::vector<std::pair<uint32_t, uint32_t>> merkle_witness_nodes(uint32_t L, std::vector<uint32_t> queries) {
+  std::sort(queries.begin(), queries.end());
+  queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
+  std::vector<std::pair<uint32_t, uint32_t>> need;
+  for (int layer = (int)L - 1; layer >= 0; layer--) {
+    std::vector<uint32_t> next;
+    for (size_t i = 0; i < queries.size();) {
+      uint32_t first = queries[i];
+      size_t len = (i + 1 < queries.size() && (first ^ 1u) == queries[i + 1]) ? 2 : 1;
+      if (len == 1) need.push_back({(uint32_t)layer + 1, first ^ 1u});
+      next.push_back(first >> 1);
+      i += len;
+    }
+    queries = next;
+  }
+  return need;
+}
+
 // MerkleProverLifted::decommit hash witness (crates/stwo/src/prover/vcs_lifted/prover.rs:118-152).
 int merkle_decommit(sh_ctx* ctx, const std::vector<const uint8_t*>& layers, uint32_t L,
                     std::vector<uint32_t> queries, std::vector<Hash>& witness) {
@@ -112,7 +136,8 @@ struct DeviceTree {
   }
 };
 
-int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t) {
+int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t,
+                   bool fetch_root = true) {
   t.L = log_size;
   uint64_t total = 32ull * ((2ull << log_size) - 1);
   t.owner = ctx;
@@ -127,6 +152,7 @@ int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32
   SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), cols.size(), log_size, (uint32_t*)t.layers[log_size],
                               nullptr));
   SH_TRY(sh_build_layers_impl(ctx, log_size, (uint32_t* const*)t.layers.data()));
+  if (!fetch_root) return SH_OK;
   SH_CHECK_HIP(ctx, hipMemcpyAsync(t.root.data(), t.layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return SH_OK;
@@ -148,8 +174,9 @@ __global__ void k_pack_leaves(const uint32_t* c0, const uint32_t* c1, const uint
 // FriFirstLayerProver::new / FriInnerLayerProver::new (crates/stwo/src/prover/fri.rs:294-306,
 // 357-368): a tree over the 4 coordinate columns, one leaf per value, or with packed leaves one
 // leaf per 4 consecutive values (16 words: one Blake2s block) and a tree 2 levels shorter.
-int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_size, bool pack, DeviceTree& t) {
-  if (!pack) return commit_columns(ctx, {cols[0], cols[1], cols[2], cols[3]}, log_size, t);
+int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_size, bool pack, DeviceTree& t,
+                     bool fetch_root = true) {
+  if (!pack) return commit_columns(ctx, {cols[0], cols[1], cols[2], cols[3]}, log_size, t, fetch_root);
   const uint32_t n_out = 1u << (log_size - 2);
   uint32_t* packed;
   SH_TRY(sh_dev_alloc(ctx, 64ull * n_out, (void**)&packed));
@@ -158,10 +185,40 @@ int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_si
   if (rc == SH_OK) {
     std::vector<const uint32_t*> pc(16);
     for (int k = 0; k < 16; k++) pc[k] = packed + (uint64_t)k * n_out;
-    rc = commit_columns(ctx, pc, log_size - 2, t);  // synchronises the stream
+    rc = commit_columns(ctx, pc, log_size - 2, t, fetch_root);
   }
-  sh_dev_free(ctx, packed);
+  sh_dev_free(ctx, packed);  // stream-ordered: a later allocation reuses it after the leaves read it
   return rc;
+}
+
+// Blake2sChannel::mix_root followed by draw_secure_felt (crates/stwo/src/core/channel/blake2s.rs:
+// 33-117) on the device, in one thread.  ch holds the digest (8 LE words) and n_draws; the root is
+// layer 0 of a tree committed on the same stream.  Writes squared_alpha_powers(alpha, k)
+// (crates/stwo/src/prover/fri.rs:79-87) to alphas (4 words each) and a copy of the root to root_out.
+__global__ void k_fri_channel(const uint32_t* root, uint32_t* ch, uint32_t k, uint32_t* alphas, uint32_t* root_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t m[16], h[8], u[8];
+  for (int i = 0; i < 8; i++) {
+    m[i] = ch[i];
+    m[8 + i] = root[i];
+    root_out[i] = root[i];
+  }
+  sh::b2_init(h);
+  sh::b2_compress(h, m, 64, true);  // digest = Blake2s(digest || root), one block
+  for (int i = 0; i < 8; i++) m[i] = h[i];
+  for (int i = 8; i < 16; i++) m[i] = 0;  // n_draws = 0 (LE), then the 0x00 byte
+  sh::b2_init(u);
+  sh::b2_compress(u, m, 37, true);  // Blake2s(digest || LE32(n_draws) || 0x00)
+  for (int i = 0; i < 8; i++) ch[i] = h[i];
+  ch[8] = 1;
+  uint32_t f[8], w[4];
+  for (int i = 0; i < 8; i++) f[i] = sh::reduce_once((u[i] & sh::P) + (u[i] >> 31));  // M31::reduce(u32)
+  for (int i = 0; i < 4; i++) w[i] = sh::madd(sh::madd(f[2 * i], f[2 * i + 1]), f[2 * i + 1]);
+  sh::qm31 a = sh::qload(w);
+  for (uint32_t j = 0; j < k; j++) {
+    sh::qstore(a, alphas + 4 * j);
+    a = sh::qmul(a, a);
+  }
 }
 
 // Serialisation helpers (crates/cairo-serialize/src/serialize.rs).
@@ -211,50 +268,6 @@ struct FriLayer {
     tree.free_all();
   }
 };
-
-// compute_decommitment_positions_and_witness_evals (crates/stwo/src/prover/fri.rs:408-438): every
-// coset of 2^fold_step positions holding a query is opened and the values the verifier cannot
-// rebuild go to the witness; then the layer's Merkle decommitment (fri.rs:308-341, 370-395), at
-// the positions divided by 4 when the leaves are packed.
-int decommit_layer(sh_ctx* ctx, const uint32_t* const cols[4], const DeviceTree& tree, const std::vector<uint32_t>& queries,
-                   uint32_t fold_step, bool pack, LayerProof& out) {
-  std::vector<uint32_t> positions;
-  std::vector<bool> is_query;
-  for (size_t i = 0; i < queries.size();) {
-    const uint32_t start = (queries[i] >> fold_step) << fold_step;
-    size_t j = i;
-    while (j < queries.size() && (queries[j] >> fold_step) == (queries[i] >> fold_step)) j++;
-    size_t k = i;
-    for (uint32_t p = start; p < start + (1u << fold_step); p++) {
-      const bool q = k < j && queries[k] == p;
-      if (q) k++;
-      positions.push_back(p);
-      is_query.push_back(q);
-    }
-    i = j;
-  }
-  std::vector<const uint32_t*> gc;
-  std::vector<uint32_t> gr;
-  for (size_t k = 0; k < positions.size(); k++)
-    if (!is_query[k])
-      for (int c = 0; c < 4; c++) {
-        gc.push_back(cols[c]);
-        gr.push_back(positions[k]);
-      }
-  std::vector<uint32_t> words;
-  SH_TRY(gather_words(ctx, gc, gr, words));
-  out.fri_witness.clear();
-  for (size_t k = 0; k < words.size(); k += 4) out.fri_witness.push_back(sh::qload(&words[k]));
-  std::vector<uint32_t> mpos;
-  for (uint32_t p : positions) {
-    const uint32_t m = pack ? p >> 2 : p;
-    if (mpos.empty() || mpos.back() != m) mpos.push_back(m);
-  }
-  std::vector<const uint8_t*> layers(tree.layers.begin(), tree.layers.end());
-  SH_TRY(merkle_decommit(ctx, layers, tree.L, mpos, out.hash_witness));
-  out.commitment = tree.root;
-  return SH_OK;
-}
 
 // line_ifft (crates/stwo/src/core/poly/line.rs:202-219) on natural-order values.
 void line_ifft(std::vector<sh::qm31>& v, uint32_t init, uint32_t log) {
@@ -366,7 +379,7 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
                             const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
                             uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
   if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions) return SH_ERR_INVALID;
-  SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 16, "fold_step %u out of range", p->fold_step);
+  SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 4, "fold_step %u out of range [1, 4]", p->fold_step);
   SH_REQUIRE(ctx, p->log_blowup_factor >= 1 && p->log_blowup_factor <= 16 && p->log_last_layer_degree_bound <= 10,
              "invalid FRI parameters");
   const uint32_t fs = p->fold_step;
@@ -393,14 +406,43 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
       return rc;           \
     }                      \
   } while (0)
-  // commit_first_layer (fri.rs:130-140): packed leaves when fold_step > 1
-  for (int c = 0; c < 4; c++) first.cols[c] = column[c];
-  first.log_size = log_size;
-  first.fold_step = fs;
-  first.pack = log_size >= 2 && fs > 1;
-  TRY_CLEAN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree));
-  sh_channel_mix_root(ch, first.tree.root.data());
-  // commit_inner_layers (fri.rs:145-196)
+  // The commit phase runs without waiting for the host: each layer's root is mixed into a device
+  // copy of the channel and the folding alphas are drawn there (k_fri_channel), so the folds read
+  // them from device memory.  The host replays the same channel operations afterwards, from the
+  // downloaded roots, and checks that the alphas agree.
+  const uint32_t max_layers = log_size + 1;
+  uint32_t* d_aux = nullptr;  // [ch: 16 words][alphas: 16 words per layer][roots: 8 words per layer]
+  TRY_CLEAN(sh_dev_alloc(ctx, 4ull * (16 + 24 * max_layers), (void**)&d_aux));
+  uint32_t* d_ch = d_aux;
+  uint32_t* d_alphas = d_aux + 16;
+  uint32_t* d_roots = d_alphas + 16 * max_layers;
+  auto cleanup_all = [&]() {
+    cleanup();
+    if (d_aux) sh_dev_free(ctx, d_aux);
+    d_aux = nullptr;
+  };
+#undef TRY_CLEAN
+#define TRY_CLEAN(expr)    \
+  do {                     \
+    rc = (expr);           \
+    if (rc != SH_OK) {     \
+      cleanup_all();       \
+      return rc;           \
+    }                      \
+  } while (0)
+  std::vector<uint32_t> h_ch(16, 0);
+  memcpy(h_ch.data(), ch->digest, 32);
+  h_ch[8] = ch->n_draws;
+  {
+    void* staged;
+    TRY_CLEAN(sh_stage(ctx, h_ch.data(), 64, &staged));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ch, staged, 64, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  auto channel_step = [&](const DeviceTree& t, uint32_t layer, uint32_t k) -> int {
+    k_fri_channel<<<1, 64, 0, ctx->stream>>>((const uint32_t*)t.layers[0], d_ch, k, d_alphas + 16 * layer,
+                                              d_roots + 8 * layer);
+    return sh_launch_check(ctx, "fri_channel");
+  };
   auto alloc_eval = [&](FriLayer& l, uint32_t lg) -> int {
     l.log_size = lg;
     l.owner = ctx;
@@ -409,79 +451,75 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
     for (int c = 0; c < 4; c++) l.cols[c] = l.buf + ((uint64_t)c << lg);
     return SH_OK;
   };
-  // k line folds of cur (log line_log) with alphas[0..k) (FriOps::fold_line with
-  // squared_alpha_powers, fri.rs:79-87); the result replaces cur
+  auto out_cols = [](FriLayer& l) {  // the 4 coordinate arrays of an owned evaluation, writable
+    std::array<uint32_t*, 4> d;
+    for (int c = 0; c < 4; c++) d[c] = l.buf + ((uint64_t)c << l.log_size);
+    return d;
+  };
+  // commit_first_layer (fri.rs:130-140): packed leaves when fold_step > 1
+  for (int c = 0; c < 4; c++) first.cols[c] = column[c];
+  first.log_size = log_size;
+  first.fold_step = fs;
+  first.pack = log_size >= 2 && fs > 1;
+  TRY_CLEAN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree, false));
+  TRY_CLEAN(channel_step(first.tree, 0, fs));
+  // commit_inner_layers (fri.rs:145-196): the circle-to-line fold with alpha, then fold_step - 1
+  // line folds with alpha^2, alpha^4, ... (one multi-step launch)
   uint32_t line_log = log_size - 1;
   uint32_t line_init = sh::subgroup_gen(line_log + 2);  // half_odds(line_log)
-  auto fold_lines = [&](const std::vector<uint32_t>& alphas) -> int {
-    for (size_t k = 0; k < alphas.size() / 4; k++) {
-      int r = alloc_eval(tmp, line_log - 1);
-      if (r != SH_OK) return r;
-      uint32_t* dst[4] = {tmp.buf, tmp.buf + (1ull << (line_log - 1)), tmp.buf + (2ull << (line_log - 1)),
-                          tmp.buf + (3ull << (line_log - 1))};
-      r = sh_fold_line(ctx, tw, cur.cols, line_init, line_log, &alphas[4 * k], 1, dst);
-      if (r != SH_OK) return r;
-      cur.free_all();  // stream-ordered: the fold that reads it is queued first
-      std::swap(cur, tmp);
-      line_log--;
-      line_init = sh::idx_reduce((uint64_t)line_init * 2);
-    }
-    return SH_OK;
-  };
-  auto alpha_powers = [](uint32_t a[4], uint32_t n) {  // squared_alpha_powers (fri.rs:79-87)
-    std::vector<uint32_t> v(4 * n);
-    sh::qm31 x = sh::qload(a);
-    for (uint32_t k = 0; k < n; k++) {
-      sh::qstore(x, &v[4 * k]);
-      x = sh::qmul(x, x);
-    }
-    return v;
-  };
-  uint32_t alpha[4];
-  sh_channel_draw_secure_felt(ch, alpha);
   TRY_CLEAN(alloc_eval(cur, log_size - 1));
-  {
-    uint32_t* dst[4] = {cur.buf, cur.buf + (1ull << (log_size - 1)), cur.buf + (2ull << (log_size - 1)),
-                        cur.buf + (3ull << (log_size - 1))};
-    TRY_CLEAN(sh_fold_circle_into_line(ctx, tw, column, log_size, alpha, dst));
+  TRY_CLEAN(sh_fold_circle_dev(ctx, tw, column, log_size, d_alphas, out_cols(cur).data()));
+  if (fs > 1) {
+    TRY_CLEAN(alloc_eval(tmp, line_log - (fs - 1)));
+    TRY_CLEAN(sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 4, fs - 1, out_cols(tmp).data()));
+    cur.free_all();  // stream-ordered release: the fold that reads it is already queued
+    std::swap(cur, tmp);
+    for (uint32_t j = 0; j < fs - 1; j++) line_init = sh::idx_reduce((uint64_t)line_init * 2);
+    line_log -= fs - 1;
   }
-  {  // the circle-to-line fold counts as one of the first layer's fold_step folds (fri.rs:157-163)
-    uint32_t a2[4];
-    sh::qstore(sh::qmul(sh::qload(alpha), sh::qload(alpha)), a2);
-    TRY_CLEAN(fold_lines(alpha_powers(a2, fs - 1)));
-  }
+  uint32_t n_layers = 1;
   while (line_log > last_log) {
     // FriInnerLayerProver::new + mix root + draw alpha + fold_line; the last inner layer folds
     // exactly down to the last layer's domain (fri.rs:175-194)
     const uint32_t k = line_log > last_log + fs ? fs : line_log - last_log;
     cur.fold_step = k;
     cur.pack = line_log >= 2 && k > 1;
-    TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree));
-    sh_channel_mix_root(ch, cur.tree.root.data());
-    sh_channel_draw_secure_felt(ch, alpha);
-    // keep this layer's evaluation (the decommitment reads it): fold from a copy of the handle
-    FriLayer kept = cur;
-    cur.buf = nullptr;
-    cur.tree = DeviceTree();
-    for (int c = 0; c < 4; c++) cur.cols[c] = kept.cols[c];
-    rc = fold_lines(alpha_powers(alpha, k));
-    inner.push_back(kept);
+    TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
+    TRY_CLEAN(channel_step(cur.tree, n_layers, k));
+    TRY_CLEAN(alloc_eval(tmp, line_log - k));
+    rc = sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 16 * n_layers, k, out_cols(tmp).data());
+    inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
+    cur = FriLayer();
+    std::swap(cur, tmp);
     if (rc != SH_OK) {
-      cleanup();
+      cleanup_all();
       return rc;
+    }
+    for (uint32_t j = 0; j < k; j++) line_init = sh::idx_reduce((uint64_t)line_init * 2);
+    line_log -= k;
+    n_layers++;
+  }
+  // one download: the device channel, the alphas and roots of every layer, the last layer
+  std::vector<uint32_t> aux(16 + 24 * max_layers);
+  std::vector<uint32_t> words(4ull << last_log);
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(aux.data(), d_aux, 4 * aux.size(), hipMemcpyDeviceToHost, ctx->stream));
+  for (int c = 0; c < 4; c++)
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + ((uint64_t)c << last_log), cur.cols[c], 4ull << last_log,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (uint32_t l = 0; l < n_layers; l++) {
+    DeviceTree& t = l == 0 ? first.tree : inner[l - 1].tree;
+    memcpy(t.root.data(), &aux[16 + 16 * max_layers + 8 * l], 32);
+    sh_channel_mix_root(ch, t.root.data());
+    uint32_t a[4];
+    sh_channel_draw_secure_felt(ch, a);
+    if (memcmp(a, &aux[16 + 16 * l], 16) != 0) {
+      cleanup_all();
+      return sh_fail(ctx, SH_ERR_INVALID, "device-side FRI channel diverged from the host channel at layer %u", l);
     }
   }
   // commit_last_layer (fri.rs:208-226)
   {
-    std::vector<uint32_t> words(4ull << last_log);
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data(), cur.cols[0], 4ull << last_log, hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (1ull << last_log), cur.cols[1], 4ull << last_log,
-                                     hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (2ull << last_log), cur.cols[2], 4ull << last_log,
-                                     hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + (3ull << last_log), cur.cols[3], 4ull << last_log,
-                                     hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<sh::qm31> v(1ull << last_log);
     for (uint64_t i = 0; i < v.size(); i++)
       v[i] = {{words[i], words[v.size() + i]}, {words[2 * v.size() + i], words[3 * v.size() + i]}};
@@ -493,7 +531,7 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
     const uint64_t bound = 1ull << p->log_last_layer_degree_bound;
     for (uint64_t i = bound; i < v.size(); i++)
       if (!sh::qeq(v[i], sh::qzero())) {
-        cleanup();
+        cleanup_all();
         return sh_fail(ctx, SH_ERR_INVALID, "invalid degree");
       }
     v.resize(bound);
@@ -511,21 +549,68 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   sh_channel_draw_queries(ch, log_size, p->n_queries, q.data());
   std::sort(q.begin(), q.end());
   q.erase(std::unique(q.begin(), q.end()), q.end());
-  // decommit_on_queries (fri.rs:253-284): queries fold by each layer's own step
-  auto fold_queries = [](std::vector<uint32_t>& v, uint32_t k) {  // Queries::fold (queries.rs:48-55)
-    for (auto& x : v) x >>= k;
-    v.erase(std::unique(v.begin(), v.end()), v.end());
+  // decommit_on_queries (fri.rs:253-284): queries fold by each layer's own step.  Every layer's
+  // witness values and Merkle witness nodes are read with one gather.
+  std::vector<const FriLayer*> layers_all;
+  layers_all.push_back(&first);
+  for (auto& l : inner) layers_all.push_back(&l);
+  struct Plan {
+    std::vector<uint32_t> witness_pos;
+    std::vector<std::pair<uint32_t, uint32_t>> nodes;
   };
-  LayerProof first_proof;
-  TRY_CLEAN(decommit_layer(ctx, first.cols, first.tree, q, fs, first.pack, first_proof));
-  std::vector<LayerProof> inner_proofs(inner.size());
+  std::vector<Plan> plans(layers_all.size());
+  std::vector<const uint32_t*> gc;
+  std::vector<uint32_t> gr;
   std::vector<uint32_t> lq = q;
-  fold_queries(lq, fs);
-  for (size_t k = 0; k < inner.size(); k++) {
-    TRY_CLEAN(decommit_layer(ctx, inner[k].cols, inner[k].tree, lq, inner[k].fold_step, inner[k].pack,
-                             inner_proofs[k]));
-    fold_queries(lq, inner[k].fold_step);
+  for (size_t li = 0; li < layers_all.size(); li++) {
+    const FriLayer& L = *layers_all[li];
+    const uint32_t k = L.fold_step;
+    // compute_decommitment_positions_and_witness_evals (fri.rs:408-438)
+    std::vector<uint32_t> mpos;
+    for (size_t i = 0; i < lq.size();) {
+      const uint32_t start = (lq[i] >> k) << k;
+      size_t j = i;
+      while (j < lq.size() && (lq[j] >> k) == (lq[i] >> k)) j++;
+      size_t qi = i;
+      for (uint32_t pos = start; pos < start + (1u << k); pos++) {
+        if (qi < j && lq[qi] == pos)
+          qi++;
+        else
+          plans[li].witness_pos.push_back(pos);
+        const uint32_t m = L.pack ? pos >> 2 : pos;
+        if (mpos.empty() || mpos.back() != m) mpos.push_back(m);
+      }
+      i = j;
+    }
+    plans[li].nodes = merkle_witness_nodes(L.tree.L, mpos);
+    for (uint32_t pos : plans[li].witness_pos)
+      for (int c = 0; c < 4; c++) {
+        gc.push_back(L.cols[c]);
+        gr.push_back(pos);
+      }
+    for (auto& nd : plans[li].nodes)
+      for (uint32_t w = 0; w < 8; w++) {
+        gc.push_back((const uint32_t*)L.tree.layers[nd.first]);
+        gr.push_back(8 * nd.second + w);
+      }
+    for (auto& x : lq) x >>= k;  // Queries::fold (crates/stwo/src/core/queries.rs:48-55)
+    lq.erase(std::unique(lq.begin(), lq.end()), lq.end());
   }
+  std::vector<uint32_t> gw;
+  TRY_CLEAN(gather_words(ctx, gc, gr, gw));
+  std::vector<LayerProof> proofs(layers_all.size());
+  size_t off = 0;
+  for (size_t li = 0; li < layers_all.size(); li++) {
+    for (size_t k = 0; k < plans[li].witness_pos.size(); k++, off += 4) proofs[li].fri_witness.push_back(sh::qload(&gw[off]));
+    for (size_t k = 0; k < plans[li].nodes.size(); k++, off += 8) {
+      Hash h;
+      memcpy(h.data(), &gw[off], 32);
+      proofs[li].hash_witness.push_back(h);
+    }
+    proofs[li].commitment = layers_all[li]->tree.root;
+  }
+  LayerProof first_proof = proofs[0];
+  std::vector<LayerProof> inner_proofs(proofs.begin() + 1, proofs.end());
   // FriProof (crates/cairo-serialize/src/serialize.rs:79-90)
   Felts f;
   first_proof.serialize(f);
@@ -535,7 +620,7 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   for (auto& x : last_coeffs) f.q(x);
   f.u(p->log_last_layer_degree_bound);
   f.u(nonce);
-  cleanup();
+  cleanup_all();
 #undef TRY_CLEAN
   *n_felts = f.v.size();
   *n_query_positions = q.size();

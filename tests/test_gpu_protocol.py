@@ -177,7 +177,7 @@ def test_fri_prove_rejects_high_degree(ctx, backend):
         fri_prove(backend, tree, Channel(), FriParams(4, 1, 0, 3, 0), col)
     # the first layer's folds must not pass the last layer (fri.rs:165-168)
     with pytest.raises(HipError, match="smaller line domain"):
-        fri_prove(backend, tree, Channel(), FriParams(4, 1, 1, 3, 7), col)
+        fri_prove(backend, tree, Channel(), FriParams(4, 1, 4, 3, 4), col)
 
 
 def test_deep_fri_chain_on_device(ctx, backend):
