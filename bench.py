@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -221,12 +223,12 @@ def bench_prove(args):
         res["felts"] = felts
 
     step()
-    first = list(res["felts"])
+    first = res["felts"].copy()
     for _ in range(args.warmup):
         step()
     steps = max(1, args.steps // 4)
     sec = _timed(ctx, step, steps)  # timed without per-launch event spans
-    assert res["felts"] == first, "proofs differ between runs"
+    assert np.array_equal(res["felts"], first), "proofs differ between runs"
     ms = sec * 1e3 / steps
     ctx.profile(True)  # a separate pass with spans, for the kernel table
     _timed(ctx, step, steps)
@@ -243,7 +245,7 @@ def bench_prove(args):
         from oracle import protocol as PR
         log("[bench] verifying the proof with the oracle verifier ...")
         t0 = time.perf_counter()
-        out["oracle_verifies"] = bool(PR.wide_fib_verify(first, N, n, w["fri"]))
+        out["oracle_verifies"] = bool(PR.wide_fib_verify(first.tolist(), N, n, w["fri"]))
         out["oracle_verify_s"] = round(time.perf_counter() - t0, 2)
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -298,7 +300,7 @@ def bench_serve(args):
             server.serve(srv_reqs, realtime=True)
         finally:
             server.close()
-        same = same and all(x.felts == y.felts for x, y in zip(seq, srv_reqs))
+        same = same and all(np.array_equal(x.felts, y.felts) for x, y in zip(seq, srv_reqs))
         loads[str(load)] = {"arrival_rate": round(rate, 2), "server": stats(srv_reqs), "plain_sequential": stats(seq)}
         log(f"[bench] load {load}: {loads[str(load)]}")
     top = loads["1.05"]

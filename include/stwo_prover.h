@@ -34,6 +34,9 @@ void sh_channel_draw_secure_felt(sh_channel *ch, uint32_t out[4]);
 void sh_channel_draw_random_point(sh_channel *ch, uint32_t out[8]);
 /* draw_queries (crates/stwo/src/core/queries.rs:11-28): n unsorted positions. */
 void sh_channel_draw_queries(sh_channel *ch, uint32_t log_domain_size, uint32_t n, uint32_t *out);
+/* AccumulationOps::generate_secure_powers (crates/stwo/src/prover/air/accumulation.rs:188-196):
+ * out[4k..4k+4] = felt^k for k < n (host arithmetic). */
+void sh_generate_secure_powers(const uint32_t felt[4], uint64_t n, uint32_t *out);
 
 /* ---------------------------------------------------------------- Merkle decommitment */
 /* MerkleProverLifted::decommit hash witness (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162)

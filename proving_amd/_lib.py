@@ -79,6 +79,7 @@ SIGNATURES = {
     "sh_channel_draw_secure_felt": (None, [c_vp, c_vp]),
     "sh_channel_draw_random_point": (None, [c_vp, c_vp]),
     "sh_channel_draw_queries": (None, [c_vp, c_u32, c_u32, c_vp]),
+    "sh_generate_secure_powers": (None, [c_vp, c_u64, c_vp]),
     "sh_merkle_decommit": (c_int, [c_vp, c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u64p]),
     "sh_gather": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
     "sh_fri_prove": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),

@@ -473,12 +473,11 @@ class _Ops:
                   "accumulate")
         return column
 
-    @staticmethod
-    def generate_secure_powers(felt, n_powers, qmul):
-        out, acc = [], [1, 0, 0, 0]
-        for _ in range(n_powers):
-            out.append(acc)
-            acc = qmul(acc, felt)
+    def generate_secure_powers(self, felt, n_powers):
+        """AccumulationOps::generate_secure_powers (crates/stwo/src/prover/air/accumulation.rs:
+        188-196): [1, felt, felt^2, ...] as an (n_powers, 4) uint32 array."""
+        out = np.empty((max(n_powers, 0), 4), np.uint32)
+        self.lib.sh_generate_secure_powers(_u32s([int(x) for x in felt]), n_powers, out.ctypes.data)
         return out
 
     # ---- ComponentProver of the wide-Fibonacci AIR ------------------------------------------
@@ -487,9 +486,9 @@ class _Ops:
         component_prover.rs:118-193) for WideFibonacciEval: adds the constraint quotients of the
         trace columns (committed evaluations) into accum (SecureColumnByCoords of 2^eval_log);
         powers[j] is constraint j's random-coefficient power."""
-        flat = [int(x) for p in powers for x in p]
+        flat = np.ascontiguousarray(np.asarray(powers, dtype=np.uint32).reshape(-1))
         self._chk(self.lib.sh_wide_fibonacci_composition(self.h, _ptrs(cols), len(cols), trace_log, eval_log,
-                                                         log_expansion, _u32s(flat), _ptrs(accum.columns)),
+                                                         log_expansion, flat.ctypes.data, _ptrs(accum.columns)),
                   "wide_fibonacci_composition")
         return accum
 
@@ -597,9 +596,10 @@ def gather(backend, cols, rows):
     return out[:len(rows)].tolist()
 
 
-def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22):
+def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22, as_array=False):
     """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284) on a device QM31
-    column; returns (proof felts, sorted query positions)."""
+    column; returns (proof felts, sorted query positions); the felts as a uint64 array when
+    as_array, else as a list."""
     felts = np.empty(cap, np.uint64)  # not zero-filled: only the written prefix is read
     nf = ctypes.c_uint64()
     qp = np.empty(max(1, params.n_queries), np.uint32)
@@ -607,7 +607,8 @@ def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22):
     backend._chk(backend.lib.sh_fri_prove(backend.h, twiddles.h, ctypes.byref(channel.s), ctypes.byref(params),
                                           _ptrs(column.columns), column.log_size, felts.ctypes.data, cap,
                                           ctypes.byref(nf), qp.ctypes.data, ctypes.byref(nq)), "fri_prove")
-    return felts[:nf.value].tolist(), qp[:nq.value].tolist()
+    out = felts[:nf.value].copy() if as_array else felts[:nf.value].tolist()
+    return out, qp[:nq.value].tolist()
 
 
 def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log_size, log_blowup, out=None):
@@ -615,14 +616,13 @@ def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log
     columns: committed evaluations (BaseColumn) in tree/column order; samples[i]: list of
     (point[8], value[4]) for column i."""
     out = out or SecureColumnByCoords(backend.ctx, lifting_log_size)
-    offs, pts, vals = [0], [], []
-    for smp in samples:
-        for pt, v in smp:
-            pts.extend(int(x) for x in pt)
-            vals.extend(int(x) for x in v)
-        offs.append(offs[-1] + len(smp))
+    offs = np.zeros(len(samples) + 1, np.uint32)
+    np.cumsum([len(smp) for smp in samples], out=offs[1:])
+    rows = np.array([[*pt, *v] for smp in samples for pt, v in smp], dtype=np.uint64).reshape(-1, 12)
+    pts = np.ascontiguousarray(rows[:, :8], dtype=np.uint32)
+    vals = np.ascontiguousarray(rows[:, 8:], dtype=np.uint32)
     backend._chk(backend.lib.sh_fri_quotients(
-        backend.h, twiddles.h, _ptrs(columns), _u32s([c.log_size for c in columns]), len(columns), _u32s(offs),
-        _u32s(pts), _u32s(vals), _u32s(list(random_coeff)), lifting_log_size, log_blowup, _ptrs(out.columns)),
-        "fri_quotients")
+        backend.h, twiddles.h, _ptrs(columns), _u32s([c.log_size for c in columns]), len(columns),
+        offs.ctypes.data, pts.ctypes.data if len(pts) else None, vals.ctypes.data if len(vals) else None,
+        _u32s(list(random_coeff)), lifting_log_size, log_blowup, _ptrs(out.columns)), "fri_quotients")
     return out

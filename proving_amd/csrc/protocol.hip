@@ -350,6 +350,15 @@ extern "C" void sh_channel_draw_queries(sh_channel* ch, uint32_t log_domain_size
   }
 }
 
+extern "C" void sh_generate_secure_powers(const uint32_t felt[4], uint64_t n, uint32_t* out) {
+  const sh::qm31 f = sh::qload(felt);
+  sh::qm31 acc = sh::qone();
+  for (uint64_t i = 0; i < n; i++) {
+    sh::qstore(acc, out + 4 * i);
+    acc = sh::qmul(acc, f);
+  }
+}
+
 extern "C" int sh_gather(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* rows, uint64_t n, uint32_t* out) {
   if (!ctx || (n && (!d_cols || !rows || !out))) return SH_ERR_INVALID;
   std::vector<const uint32_t*> c(d_cols, d_cols + n);

@@ -102,19 +102,22 @@ class CommitmentSchemeProver:
 
     def prove_values(self, sampled_points, channel, oods_proof_of_work):
         """prove_values (crates/stwo/src/prover/pcs/mod.rs:224-315). sampled_points[tree][col] is
-        a list of QM31 circle points (8 words). Returns (felts of the CommitmentSchemeProof, the
+        a list of QM31 circle points (8 words). Returns (felts of the CommitmentSchemeProof as a uint64
+        array, the
         proof as a dict)."""
         L = self.trees[-1].lifting_log_size
         b = self.fri["log_blowup_factor"]
         # compute_samples (crates/stwo/src/prover/pcs/mod.rs:176-222): coefficient evaluation at the
         # point doubled down to the column's own domain
         # one batched evaluation per distinct (doubled) point over all columns that sample it
-        jobs = {}
+        jobs, doubled = {}, {}
         for ti, (tree, pts_tree) in enumerate(zip(self.trees, sampled_points)):
             for ci, (c, e, pts) in enumerate(zip(tree.coeffs, tree.evals, pts_tree)):
                 for pi, p in enumerate(pts):
-                    q = tuple(F.qpoint_repeated_double(p, L - e.log_size))
-                    jobs.setdefault(q, []).append((ti, ci, pi, c))
+                    key = (tuple(p), L - e.log_size)
+                    if key not in doubled:
+                        doubled[key] = tuple(F.qpoint_repeated_double(p, L - e.log_size))
+                    jobs.setdefault(doubled[key], []).append((ti, ci, pi, c))
         values = {}
         for q, items in jobs.items():
             res = self.be.eval_at_points([it[3] for it in items], [list(q)])
@@ -135,7 +138,7 @@ class CommitmentSchemeProver:
         quotients = fri_quotients(self.be, self.tw, flat_cols, flat_samples, random_coeff, L, b)
         params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
                            self.fri["fold_step"])
-        fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients)
+        fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients, as_array=True)
         quotients.free()
         pp_height = self.trees[0].lifting_log_size
         pp_positions = prepare_preprocessed_query_positions(queries, L, pp_height)
@@ -148,7 +151,7 @@ class CommitmentSchemeProver:
                  "oods_proof_of_work": oods_proof_of_work, "sampled_values": sampled_values,
                  "decommitments": decommitments, "queried_values": queried_values, "fri_felts": fri_felts,
                  "query_positions": queries}
-        return serialize_commitment_scheme_proof(proof), proof
+        return serialize_commitment_scheme_proof(proof, as_array=True), proof
 
 
 def prepare_preprocessed_query_positions(query_positions, max_log_size, pp_max_log_size):
@@ -160,45 +163,53 @@ def prepare_preprocessed_query_positions(query_positions, max_log_size, pp_max_l
     return [((p >> (max_log_size - pp_max_log_size + 1)) << 1) + (p & 1) for p in query_positions]
 
 
-def serialize_commitment_scheme_proof(proof):
+def serialize_commitment_scheme_proof(proof, as_array=False):
     """CairoSerialize for CommitmentSchemeProof (crates/cairo-serialize/src/serialize.rs:121-143):
     PcsConfig (its FriParams only), commitments, oods_proof_of_work, sampled_values,
     decommitments, queried_values, fri_proof; vectors are length-prefixed, a hash is 8 LE u32,
-    a QM31 4 felts."""
+    a QM31 4 felts.  Returns a list of ints, or a uint64 array when as_array."""
     parts = []
+    head = []  # small scalars gathered into one array per run
 
-    def scalars(*v):
-        parts.append(np.asarray(v, dtype=np.uint64))
+    def flush():
+        if head:
+            parts.append(np.asarray(head, dtype=np.uint64))
+            head.clear()
 
     def hashes(hs):
-        scalars(len(hs))
+        head.append(len(hs))
         if hs:
+            flush()
             parts.append(np.frombuffer(b"".join(hs), dtype="<u4").astype(np.uint64))
 
     f = proof["fri_params"]
-    scalars(f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"], f["fold_step"])
+    head.extend([f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"],
+                 f["fold_step"]])
     hashes(proof["commitments"])
-    scalars(proof["oods_proof_of_work"], len(proof["sampled_values"]))
+    head.extend([proof["oods_proof_of_work"], len(proof["sampled_values"])])
     for tree in proof["sampled_values"]:
-        scalars(len(tree))
+        head.append(len(tree))
         for col in tree:
-            scalars(len(col))
-            if col:
-                parts.append(np.asarray(col, dtype=np.uint64).ravel())
-    scalars(len(proof["decommitments"]))
+            head.append(len(col))
+            for v in col:
+                head.extend(int(x) for x in v)
+    head.append(len(proof["decommitments"]))
     for wit in proof["decommitments"]:
         hashes(wit)
-    scalars(len(proof["queried_values"]))
+    head.append(len(proof["queried_values"]))
     for tree in proof["queried_values"]:
         tree = np.asarray(tree, dtype=np.uint64).reshape(len(tree), -1) if len(tree) else np.zeros((0, 0), np.uint64)
-        scalars(tree.shape[0])
+        head.append(tree.shape[0])
         if tree.shape[0]:
+            flush()
             block = np.empty((tree.shape[0], tree.shape[1] + 1), np.uint64)
             block[:, 0] = tree.shape[1]
             block[:, 1:] = tree
             parts.append(block.ravel())
+    flush()
     parts.append(np.asarray(proof["fri_felts"], dtype=np.uint64))
-    return np.concatenate(parts).tolist()
+    out = np.concatenate(parts)
+    return out if as_array else out.tolist()
 
 
 def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None):
@@ -231,7 +242,7 @@ def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
     # compute_composition_polynomial (crates/stwo/src/prover/air/component_prover.rs:103-131)
     alpha = [int(x) for x in channel.draw_secure_felt()]
     m = n_cols - 2
-    powers = F.secure_powers(alpha, m)[::-1]               # ColumnAccumulator powers, reversed
+    powers = backend.generate_secure_powers(alpha, m)[::-1]  # ColumnAccumulator powers, reversed
     eval_log = log_n + 1                                   # max_constraint_log_degree_bound
     log_expansion = b - 1                                  # EvaluationMode::infer: SubDomain
     assert log_expansion == 0, "only log_blowup_factor 1 (SubDomain {log_expansion: 0}) is driven here"

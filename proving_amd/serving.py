@@ -37,7 +37,7 @@ class ProveRequest:
     arrival: float = 0.0           # seconds after the stream starts
     admitted: float = None
     done: float = None
-    felts: list = field(default=None, repr=False)
+    felts: np.ndarray = field(default=None, repr=False)  # CairoSerde felts of the proof
 
 
 def device_bytes(n_cols, log_n, log_blowup=1):

@@ -32,8 +32,8 @@ def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n,
     tree = backend.precompute_canonic_twiddles(log_n + 1)
     cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
     felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri)
-    assert felts == exp_felts
-    assert PR.wide_fib_verify(felts, n_cols, log_n, fri)
+    assert felts.tolist() == exp_felts
+    assert PR.wide_fib_verify(felts.tolist(), n_cols, log_n, fri)
 
 
 def test_wide_fib_bad_trace_rejected(ctx, backend):
@@ -44,7 +44,7 @@ def test_wide_fib_bad_trace_rejected(ctx, backend):
     tree = backend.precompute_canonic_twiddles(log_n + 1)
     felts, _, _ = prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri)
     with pytest.raises(AssertionError):
-        PR.wide_fib_verify(felts, n_cols, log_n, fri)
+        PR.wide_fib_verify(felts.tolist(), n_cols, log_n, fri)
 
 
 def test_wide_fib_full_size_verifies(ctx, backend):
@@ -56,4 +56,4 @@ def test_wide_fib_full_size_verifies(ctx, backend):
     cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
     del trace
     felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, w["log_size"], w["fri"])
-    assert PR.wide_fib_verify(felts, w["n_cols"], w["log_size"], w["fri"])
+    assert PR.wide_fib_verify(felts.tolist(), w["n_cols"], w["log_size"], w["fri"])

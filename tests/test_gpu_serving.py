@@ -1,6 +1,7 @@
 """The proof-serving loop: a mix of requests proved with cross-request trace commits and
 overlapping worker contexts gives, for every request, exactly the proof made alone; the oracle
 verifier accepts them."""
+import numpy as np
 import pytest
 
 from oracle import protocol as PR
@@ -27,6 +28,6 @@ def test_each_proof_is_the_same_alone_and_in_the_mix():
     tw = be.precompute_canonic_twiddles(12)
     prove_sequential(be, tw, alone, FRI, realtime=False)
     for m, a in zip(mixed, alone):
-        assert m.felts == a.felts, f"request {m.index} differs"
-        assert PR.wide_fib_verify(m.felts, m.n_cols, m.log_n, FRI)
+        assert np.array_equal(m.felts, a.felts), f"request {m.index} differs"
+        assert PR.wide_fib_verify(m.felts.tolist(), m.n_cols, m.log_n, FRI)
     ctx.close()
