@@ -35,6 +35,14 @@ struct sh_ctx {
   uint8_t* d_ring = nullptr;
   uint64_t ring_size = 0, ring_off = 0;
   std::unordered_map<std::string, uint64_t> table_cache;
+  // staged tables whose copy is not known to be complete: ring offset -> (event after the copy,
+  // stream that copied); a reuse from another stream waits on the event instead of the host
+  struct StageEv {
+    hipEvent_t ev;
+    hipStream_t stream;
+  };
+  std::unordered_map<uint64_t, StageEv> stage_pending;
+  std::vector<hipEvent_t> stage_ev_free;
   // profiling (hipEvent pairs around each kernel launch, tagged by span class)
   bool profile = false;
   std::vector<sh_prof_rec> prof;
@@ -56,6 +64,9 @@ struct sh_ctx {
     uint8_t block_log, stride_layers, tile_log, max_stage;
   } fft_tune[32];
 };
+
+// Every staged copy is complete (the caller has synchronised the context's streams).
+void sh_stage_all_complete(sh_ctx* ctx);
 
 struct sh_twiddles {
   uint32_t root_initial_index;

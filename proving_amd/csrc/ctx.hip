@@ -71,6 +71,8 @@ extern "C" int sh_ctx_destroy(sh_ctx* ctx) {
   }
   for (auto e : ctx->event_pool) hipEventDestroy(e);
   for (auto e : ctx->sync_events) hipEventDestroy(e);
+  for (auto& kv : ctx->stage_pending) hipEventDestroy(kv.second.ev);
+  for (auto e : ctx->stage_ev_free) hipEventDestroy(e);
   for (auto a : ctx->aux)
     if (a) hipStreamSynchronize(a);
   for (auto& s : ctx->scratch)
@@ -103,6 +105,11 @@ extern "C" int sh_sync(sh_ctx* ctx) {
 // reset, which first waits for every stream of the context: a table that an
 // enqueued kernel reads stays valid, and a table seen before is reused without
 // another copy (its bytes are compared in full, not just hashed).
+void sh_stage_all_complete(sh_ctx* ctx) {
+  for (auto& kv : ctx->stage_pending) ctx->stage_ev_free.push_back(kv.second.ev);
+  ctx->stage_pending.clear();
+}
+
 int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
   uint64_t need = (bytes + 255) & ~255ull;
   SH_REQUIRE(ctx, need <= ctx->ring_size, "per-call table of %llu bytes exceeds the staging arena",
@@ -110,6 +117,11 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
   std::string key((const char*)host, bytes);
   auto it = ctx->table_cache.find(key);
   if (it != ctx->table_cache.end()) {
+    // the first copy was enqueued on some stream of the context: a reuse on another stream is
+    // ordered after it by an event wait (same-stream reuse is ordered already)
+    auto p = ctx->stage_pending.find(it->second);
+    if (p != ctx->stage_pending.end() && p->second.stream != ctx->stream)
+      SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream, p->second.ev, 0));
     *dev = ctx->d_ring + it->second;
     return SH_OK;
   }
@@ -122,15 +134,34 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
     for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
     ctx->ring_off = 0;
     ctx->table_cache.clear();
+    sh_stage_all_complete(ctx);
   }
   uint8_t* h = ctx->h_ring + ctx->ring_off;
   uint8_t* d = ctx->d_ring + ctx->ring_off;
   memcpy(h, host, bytes);
   SH_CHECK_HIP(ctx, hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
   if (key.size() <= 64 * 1024) {
-    // A cached table may later be read from another stream of the context: make
-    // the first copy complete before any reuse can be enqueued.
-    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // A cached table may later be read from another stream of the context: record the copy's
+    // completion for such reuses.  Completed records are reclaimed now and then.
+    if (ctx->stage_pending.size() >= 256) {
+      for (auto p = ctx->stage_pending.begin(); p != ctx->stage_pending.end();) {
+        if (hipEventQuery(p->second.ev) == hipSuccess) {
+          ctx->stage_ev_free.push_back(p->second.ev);
+          p = ctx->stage_pending.erase(p);
+        } else {
+          ++p;
+        }
+      }
+    }
+    hipEvent_t ev = nullptr;
+    if (!ctx->stage_ev_free.empty()) {
+      ev = ctx->stage_ev_free.back();
+      ctx->stage_ev_free.pop_back();
+    } else {
+      SH_CHECK_HIP(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    SH_CHECK_HIP(ctx, hipEventRecord(ev, ctx->stream));
+    ctx->stage_pending[ctx->ring_off] = {ev, ctx->stream};
     ctx->table_cache.emplace(std::move(key), ctx->ring_off);
   }
   ctx->ring_off += need;

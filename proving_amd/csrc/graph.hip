@@ -23,6 +23,7 @@ struct sh_graph {
   uint8_t* d_ring = nullptr;
   uint64_t ring_size = 0, ring_off = 0;
   std::unordered_map<std::string, uint64_t> table_cache;
+  std::unordered_map<uint64_t, sh_ctx::StageEv> stage_pending;
 };
 
 namespace {
@@ -33,6 +34,7 @@ void swap_state(sh_ctx* ctx, sh_graph* g) {
   std::swap(ctx->ring_size, g->ring_size);
   std::swap(ctx->ring_off, g->ring_off);
   std::swap(ctx->table_cache, g->table_cache);
+  std::swap(ctx->stage_pending, g->stage_pending);
 }
 void release(sh_graph* g) {
   if (g->exec) hipGraphExecDestroy(g->exec);
@@ -41,6 +43,7 @@ void release(sh_graph* g) {
     if (s.ptr) hipFree(s.ptr);
   if (g->d_ring) hipFree(g->d_ring);
   if (g->h_ring) hipHostFree(g->h_ring);
+  for (auto& kv : g->stage_pending) hipEventDestroy(kv.second.ev);
   delete g;
 }
 }  // namespace
@@ -71,6 +74,7 @@ extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, 
     for (auto a : ctx->aux)
       if (e == hipSuccess) e = hipStreamSynchronize(a);
     if (e != hipSuccess) rc = sh_fail(ctx, SH_ERR_HIP, "warm-up failed: %s", hipGetErrorString(e));
+    sh_stage_all_complete(ctx);  // the capture below only reuses tables, with no event waits
   }
   if (rc == SH_OK) {
     hipError_t e = hipStreamBeginCapture(ctx->main_stream, hipStreamCaptureModeThreadLocal);
