@@ -105,6 +105,21 @@ def cpu_baseline(work, expected_root):
             "root_matches_gpu": root == expected_root}
 
 
+def _tune(args, be, logs, log_blowup, tree):
+    """FFT pass shapes: the committed table (proving_amd/fft_tuning_gfx950.json, written by
+    python -m proving_amd.autotune) unless --autotune asks for a live autotune or there is no
+    table; --no-autotune keeps the built-in shapes.  Returns whether shapes were set."""
+    if args.no_autotune:
+        return False
+    if not args.autotune and be.load_fft_tuning():
+        log("[bench] FFT pass shapes from proving_amd/fft_tuning_gfx950.json")
+        return True
+    t0 = time.perf_counter()
+    be.autotune_fft(logs, log_blowup, tree)
+    log(f"[bench] autotuned FFT pass shapes in {time.perf_counter() - t0:.1f} s")
+    return True
+
+
 def _timed(ctx, step_fn, steps):
     ctx.sync()
     t0 = time.perf_counter()
@@ -133,6 +148,7 @@ def bench_fri_deep(args):
     ctx = Context(0)
     be = HipBackend(ctx)
     tree = be.precompute_canonic_twiddles(L)
+    _tune(args, be, [n] * N, b, tree)
     log(f"[bench] fri_deep: {N} columns x 2^{n}, LDE 2^{L}; generating inputs")
     host = splitmix64_columns(w["seed"], [n] * N)
     evals = [BaseColumn.from_cpu(ctx, c) for c in host]
@@ -214,8 +230,7 @@ def bench_prove(args):
     tree = be.precompute_canonic_twiddles(n + w["fri"]["log_blowup_factor"])
     log(f"[bench] prove: wide Fibonacci {N} x 2^{n}; generating the trace")
     trace = [BaseColumn.from_cpu(ctx, c) for c in wide_fib_trace(N, n, w["seed"])]
-    if not args.no_autotune:
-        be.autotune_fft([n] * N, w["fri"]["log_blowup_factor"], tree)
+    _tune(args, be, [n] * N, w["fri"]["log_blowup_factor"], tree)
     res = {}
 
     def step():
@@ -264,6 +279,7 @@ def bench_serve(args):
     n = args.requests
     ctx = Context(0)
     be = HipBackend(ctx)
+    be.load_fft_tuning()  # the same pass shapes as the server's contexts
     tw = be.precompute_canonic_twiddles(21)
     log(f"[bench] serve: {n} requests; warm-up and capacity (back-to-back plain proving)")
     prove_sequential(be, tw, make_requests(4, 99), fri, realtime=False)  # warm-up
@@ -330,7 +346,9 @@ def main():
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--serve-workers", type=int, default=2, help="serve: worker contexts of the server")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in pass shapes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the FFT pass shapes now instead of loading proving_amd/fft_tuning_gfx950.json")
     ap.add_argument("--fft-tuning", default=None,
                     help="JSON {log_size: [block_log, stride_layers, tile_log, max_stage]} to use instead "
                          "of autotuning (the 'fft_tuning' field of an earlier run), so that a profiled run "
@@ -386,11 +404,9 @@ def main():
         for m, t in json.loads(args.fft_tuning).items():
             be.set_fft_tuning(int(m), *[int(x) for x in t])
         log(f"[bench] FFT pass shapes set from --fft-tuning")
-    elif not args.no_autotune:
-        t0 = time.perf_counter()
-        be.autotune_fft(logs, b, tree)
+    elif _tune(args, be, logs, b, tree):
         tuned = {m: be.fft_tuning(m) for m in sorted(set(logs) | {lg + b for lg in logs})}
-        log(f"[bench] autotuned FFT pass shapes in {time.perf_counter() - t0:.1f} s: {tuned}")
+        log(f"[bench] FFT pass shapes: {tuned}")
 
     _, _, _, root0 = be.commit_trace(evals, b, tree, coeffs, ldes, layers)
     for _ in range(args.warmup):

@@ -9,6 +9,8 @@ panic (``assert!``) the C-ABI returns SH_ERR_INVALID and this module raises
 ``HipError``.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 
@@ -332,6 +334,18 @@ class HipBackend:
     def autotune_fft(self, log_sizes, log_blowup, twiddles):
         self._chk(self.lib.sh_autotune_fft(self.h, twiddles.h, _u32s(log_sizes), len(log_sizes), log_blowup),
                   "autotune_fft")
+
+    def load_fft_tuning(self, path=None):
+        """Pass shapes per transform log size from the table proving_amd.autotune writes
+        (fft_tuning_gfx950.json beside this file by default); returns False when there is none."""
+        path = path or os.path.join(os.path.dirname(os.path.abspath(__file__)), "fft_tuning_gfx950.json")
+        if not os.path.exists(path):
+            return False
+        with open(path) as f:
+            table = json.load(f)["shapes"]
+        for m, t in table.items():
+            self.set_fft_tuning(int(m), *[int(x) for x in t])
+        return True
 
     def fft_tuning(self, log_size):
         out = (ctypes.c_uint32 * 4)()

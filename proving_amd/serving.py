@@ -57,6 +57,7 @@ class ProofServer:
         self.fri = dict(fri)
         self.ctx = Context(0)
         self.be = HipBackend(self.ctx)
+        self.be.load_fft_tuning()  # the committed pass-shape table, when there is one
         self.tw = self.be.precompute_canonic_twiddles(twiddle_log)   # resident for every request
         self.budget = hbm_budget
         self.max_in_flight = max_in_flight or 2 * n_workers
@@ -76,6 +77,7 @@ class ProofServer:
     def _worker(self):
         ctx = Context(0)
         be = HipBackend(ctx)
+        be.load_fft_tuning()
         while True:
             item = self._work.get()
             if item is None:
