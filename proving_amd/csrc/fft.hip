@@ -56,6 +56,18 @@ std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
 // addresses differ by multiples of 64 words hit the same bank.
 __device__ __forceinline__ uint32_t sw(uint32_t a) { return a ^ (((a >> 6) & 15u) << 2); }
 
+// LDS words by byte address in the LDS address space.  The pass kernels declare no static LDS, so
+// their dynamic array starts at byte 0 and a word's byte address is 4 * its index: the XOR of a
+// lane's address with a stage's constant element offset is then the whole address computation.
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));  // plain vector: address-space qualifiable
+typedef __attribute__((address_space(1))) uint32_t gword;    // global memory (not FLAT)
+typedef __attribute__((address_space(1))) v4u gquad;
+constexpr int kMaxQuads = 8;  // 2^13 words (the largest tile sh_set_fft_tuning allows) / (4 * kThreads)
+typedef __attribute__((address_space(3))) uint32_t lds_word;
+typedef __attribute__((address_space(3))) v4u lds_quad;
+__device__ __forceinline__ lds_word* lds_at(uint32_t byte_addr) { return (lds_word*)(size_t)byte_addr; }
+__device__ __forceinline__ lds_quad* lds_quad_at(uint32_t byte_addr) { return (lds_quad*)(size_t)byte_addr; }
+
 __device__ __forceinline__ uint32_t circle_twiddle(const uint32_t* l1, uint32_t h) {
   // crates/stwo/src/prover/backend/cpu/circle.rs:306-323: [x, y] -> [y, -y, -x, x];
   // branch-free (lanes of one wave take different cases), one 64-bit load
@@ -82,7 +94,12 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
     const uint32_t jr = q >> tl;
     const uint32_t jf = ((jr >> s) << (s + R)) | (jr & ((1u << s) - 1));
     uint32_t v[1 << R];
-    const uint32_t lane_addr = sw((jf << tl) | t);
+    // byte addresses: the element offsets XOR straight into the address the LDS access takes
+    // (one VALU op per access instead of an XOR and a scale)
+    uint32_t lane_b = sw((jf << tl) | t) << 2;
+    // opaque to the optimiser: otherwise (x << 2) ^ (c << 2) is refactored into ((x ^ c) << 2),
+    // one more op per access
+    asm("" : "+v"(lane_b));
     // s == 0 in a contiguous block: the lane's 2^R words are adjacent, so they
     // move as 128-bit LDS accesses instead of 2^R lanes-strided words (which
     // would serialise on a few banks)
@@ -91,7 +108,7 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
       if (vec) {
 #pragma unroll
         for (int e = 0; e < (1 << R); e += 4) {
-          uint4 q4 = *(const uint4*)&lds[lane_addr ^ sw((uint32_t)e)];
+          v4u q4 = *lds_quad_at(lane_b ^ (sw((uint32_t)e) << 2));
           v[e] = q4.x;
           v[e + 1] = q4.y;
           v[e + 2] = q4.z;
@@ -101,7 +118,8 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
     }
     if (!vec || R < 2) {
 #pragma unroll
-      for (int e = 0; e < (1 << R); e++) v[e] = lds[lane_addr ^ sw((uint32_t)e << (s + tl))];
+      for (int e = 0; e < (1 << R); e++)
+        v[e] = *lds_at(lane_b ^ (sw((uint32_t)e << (s + tl)) << 2));
     }
     // position of the lane's first word; the pair (eh, el) of layer i = lo+s+b
     // sits at pb + eh * 2^(i+1) + el * 2^(lo+s), since jf is zero in bits
@@ -140,13 +158,48 @@ __device__ __forceinline__ void fft_stage(uint32_t* lds, uint32_t G, uint32_t tl
       if (vec) {
 #pragma unroll
         for (int e = 0; e < (1 << R); e += 4)
-          *(uint4*)&lds[lane_addr ^ sw((uint32_t)e)] = make_uint4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+          *lds_quad_at(lane_b ^ (sw((uint32_t)e) << 2)) = v4u{v[e], v[e + 1], v[e + 2], v[e + 3]};
       }
     }
     if (!vec || R < 2) {
 #pragma unroll
-      for (int e = 0; e < (1 << R); e++) lds[lane_addr ^ sw((uint32_t)e << (s + tl))] = v[e];
+      for (int e = 0; e < (1 << R); e++) *lds_at(lane_b ^ (sw((uint32_t)e << (s + tl)) << 2)) = v[e];
     }
+  }
+}
+
+// A tile of NQ quads per lane between HBM and LDS: every global load of the lane is issued before
+// the first LDS write (and every LDS read before the first global store), so a lane waits for
+// memory once per tile.  Position of tile word w: base + ((w >> tl) << lo) + (w & tmask).
+template <int NQ>
+__device__ __forceinline__ void tile_to_lds(const gword* __restrict__ src, uint32_t base, uint32_t lo, uint32_t tl,
+                                            uint32_t tmask, uint32_t src_mask) {
+  v4u r[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) {
+    const uint32_t w = 4 * (threadIdx.x + k * kThreads);
+    r[k] = *(const gquad*)(src + ((base + ((w >> tl) << lo) + (w & tmask)) & src_mask));
+  }
+#pragma unroll
+  for (int k = 0; k < NQ; k++) *lds_quad_at(sw(4 * (threadIdx.x + k * kThreads)) << 2) = r[k];
+}
+template <int NQ>
+__device__ __forceinline__ void tile_from_lds(gword* __restrict__ dst, uint32_t base, uint32_t lo, uint32_t tl,
+                                              uint32_t tmask, uint32_t scale) {
+  v4u r[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; k++) r[k] = *lds_quad_at(sw(4 * (threadIdx.x + k * kThreads)) << 2);
+#pragma unroll
+  for (int k = 0; k < NQ; k++) {
+    v4u v = r[k];
+    if (scale) {
+      v.x = sh::mmul(v.x, scale);
+      v.y = sh::mmul(v.y, scale);
+      v.z = sh::mmul(v.z, scale);
+      v.w = sh::mmul(v.w, scale);
+    }
+    const uint32_t w = 4 * (threadIdx.x + k * kThreads);
+    *(gquad*)(dst + base + ((w >> tl) << lo) + (w & tmask)) = v;
   }
 }
 
@@ -194,16 +247,26 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
   const bool vec = (tl >= 2 || lo == 0) && G + tl >= 2 && src_mask >= 3 &&
                    ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
 
-  if (vec) {
-    for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
-      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      *(uint4*)&lds[sw(w)] = *(const uint4*)&src[pos & src_mask];
-    }
+  // Tile load: a lane's quads are loaded in batches of four, each batch issued before its first
+  // LDS write (a tile is at most 2^13 words, 8 quads per lane), from the global address space
+  // (no FLAT counter waits).
+  const gword* gsrc = (const gword*)src;
+  // whole quads per lane (tiles of 2^10..2^13 words) take the straight-line path
+  const uint32_t nq = n_words >> 10;
+  if (vec && nq == 4) {
+    tile_to_lds<4>(gsrc, base, lo, tl, tmask, src_mask);
+  } else if (vec && nq == 8) {
+    tile_to_lds<8>(gsrc, base, lo, tl, tmask, src_mask);
+  } else if (vec && nq == 2) {
+    tile_to_lds<2>(gsrc, base, lo, tl, tmask, src_mask);
+  } else if (vec && nq == 1) {
+    tile_to_lds<1>(gsrc, base, lo, tl, tmask, src_mask);
+  } else if (vec) {
+    for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads)
+      *lds_quad_at(sw(w) << 2) = *(const gquad*)(gsrc + ((base + ((w >> tl) << lo) + (w & tmask)) & src_mask));
   } else {
-    for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
-      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      lds[sw(w)] = src[pos & src_mask];
-    }
+    for (uint32_t w = threadIdx.x; w < n_words; w += kThreads)
+      *lds_at(sw(w) << 2) = gsrc[(base + ((w >> tl) << lo) + (w & tmask)) & src_mask];
   }
   __syncthreads();
 
@@ -223,23 +286,32 @@ __global__ __launch_bounds__(kThreads) void k_fft_pass(const uint32_t* const* sr
     __syncthreads();
   }
 
-  if (vec) {
+  gword* gdst = (gword*)dst;
+  if (vec && nq == 4) {
+    tile_from_lds<4>(gdst, base, lo, tl, tmask, scale);
+  } else if (vec && nq == 8) {
+    tile_from_lds<8>(gdst, base, lo, tl, tmask, scale);
+  } else if (vec && nq == 2) {
+    tile_from_lds<2>(gdst, base, lo, tl, tmask, scale);
+  } else if (vec && nq == 1) {
+    tile_from_lds<1>(gdst, base, lo, tl, tmask, scale);
+  } else if (vec) {
     for (uint32_t w = 4 * threadIdx.x; w < n_words; w += 4 * kThreads) {
-      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      uint4 v = *(const uint4*)&lds[sw(w)];
-      if (scale) {
-        v.x = sh::mmul(v.x, scale);
-        v.y = sh::mmul(v.y, scale);
-        v.z = sh::mmul(v.z, scale);
-        v.w = sh::mmul(v.w, scale);
+      {
+        v4u v = *lds_quad_at(sw(w) << 2);
+        if (scale) {
+          v.x = sh::mmul(v.x, scale);
+          v.y = sh::mmul(v.y, scale);
+          v.z = sh::mmul(v.z, scale);
+          v.w = sh::mmul(v.w, scale);
+        }
+        *(gquad*)(gdst + base + ((w >> tl) << lo) + (w & tmask)) = v;
       }
-      *(uint4*)&dst[pos] = v;
     }
   } else {
     for (uint32_t w = threadIdx.x; w < n_words; w += kThreads) {
-      uint32_t pos = base + ((w >> tl) << lo) + (w & tmask);
-      uint32_t v = lds[sw(w)];
-      dst[pos] = scale ? sh::mmul(v, scale) : v;
+      const uint32_t v = *lds_at(sw(w) << 2);
+      gdst[base + ((w >> tl) << lo) + (w & tmask)] = scale ? sh::mmul(v, scale) : v;
     }
   }
 }
