@@ -24,7 +24,7 @@
 namespace {
 
 constexpr uint32_t kEvalLoLog = 8;   // 256 lanes: the low 8 index bits
-constexpr uint32_t kEvalHiPerBlock = 64;
+constexpr uint32_t kEvalHiPerBlock = 256;
 constexpr uint32_t kEvalMaxPoints = 4;
 
 __device__ __forceinline__ void lfold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
@@ -42,55 +42,76 @@ __global__ __launch_bounds__(256) void k_eval_points(const uint32_t* const* __re
                                                      const uint32_t* __restrict__ b_lo,
                                                      const uint32_t* __restrict__ b_hi,
                                                      unsigned long long* __restrict__ acc) {
-  __shared__ uint32_t red[256][4];
+  __shared__ uint32_t red[4 * kEvalMaxPoints][4];  // [point * 4 + wave]
   const uint32_t s = log_n < kEvalLoLog ? log_n : kEvalLoLog;
   const uint32_t lo = threadIdx.x;
   const uint32_t n_hi = 1u << (log_n - s);
   const uint32_t h0 = blockIdx.x * kEvalHiPerBlock;
   const uint32_t h1 = h0 + kEvalHiPerBlock < n_hi ? h0 + kEvalHiPerBlock : n_hi;
-  const uint32_t* __restrict__ c = cols[blockIdx.y];
+  // global address space (no FLAT counter waits); a column is below 2^30 words
+  const __attribute__((address_space(1))) uint32_t* __restrict__ c =
+      (const __attribute__((address_space(1))) uint32_t*)cols[blockIdx.y];
   const bool live = lo < (1u << s);
   uint64_t a[NP][4];
 #pragma unroll
   for (int pt = 0; pt < NP; pt++) a[pt][0] = a[pt][1] = a[pt][2] = a[pt][3] = 0;
+  auto mac = [&](uint32_t v, uint32_t hi) {
+#pragma unroll
+    for (int pt = 0; pt < NP; pt++) {
+      const uint4 bh = *(const uint4*)&b_hi[4 * (hi * NP + pt)];
+      a[pt][0] += (uint64_t)v * bh.x;
+      a[pt][1] += (uint64_t)v * bh.y;
+      a[pt][2] += (uint64_t)v * bh.z;
+      a[pt][3] += (uint64_t)v * bh.w;
+    }
+  };
+  auto fold_all = [&]() {
+#pragma unroll
+    for (int pt = 0; pt < NP; pt++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) lfold(a[pt][q]);
+  };
   if (live) {
-    uint32_t k = 0;
-    for (uint32_t hi = h0; hi < h1; hi++) {
-      const uint32_t v = c[(hi << s) + lo];
+    uint32_t hi = h0;
+    // six coefficients per step, their loads issued together; a fold after every three
+    // products (three products < 2^62 on a folded value < 2^34 stay below 2^64)
+    for (; hi + 6 <= h1; hi += 6) {
+      uint32_t v[6];
 #pragma unroll
-      for (int pt = 0; pt < NP; pt++) {
-        const uint4 bh = *(const uint4*)&b_hi[4 * (hi * NP + pt)];
-        a[pt][0] += (uint64_t)v * bh.x;
-        a[pt][1] += (uint64_t)v * bh.y;
-        a[pt][2] += (uint64_t)v * bh.z;
-        a[pt][3] += (uint64_t)v * bh.w;
-      }
-      if (++k == 3) {  // three products < 2^62 on a folded value < 2^34 stay below 2^64
-        k = 0;
+      for (int u = 0; u < 6; u++) v[u] = c[((hi + u) << s) + lo];
 #pragma unroll
-        for (int pt = 0; pt < NP; pt++)
-#pragma unroll
-          for (int q = 0; q < 4; q++) lfold(a[pt][q]);
+      for (int u = 0; u < 6; u++) {
+        mac(v[u], hi + u);
+        if (u == 2 || u == 5) fold_all();
       }
     }
+    for (; hi < h1; hi++) {
+      mac(c[(hi << s) + lo], hi);
+      fold_all();
+    }
   }
+  // block sum per point: within each wave by lane shuffles, then the four wave sums through LDS
+  // (one barrier), then one 64-bit atomic add per coordinate
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int pt = 0; pt < NP; pt++) {
     sh::qm31 part = {{lfinal(a[pt][0]), lfinal(a[pt][1])}, {lfinal(a[pt][2]), lfinal(a[pt][3])}};
     part = live ? sh::qmul(part, sh::qload(b_lo + 4 * (lo * NP + pt))) : sh::qzero();
-    red[lo][0] = part.a.a;
-    red[lo][1] = part.a.b;
-    red[lo][2] = part.b.a;
-    red[lo][3] = part.b.b;
-    __syncthreads();
-    for (uint32_t h = 128; h > 0; h >>= 1) {
-      if (lo < h)
+    uint32_t w[4] = {part.a.a, part.a.b, part.b.a, part.b.b};
 #pragma unroll
-        for (int q = 0; q < 4; q++) red[lo][q] = sh::madd(red[lo][q], red[lo + h][q]);
-      __syncthreads();
-    }
-    if (lo < 4) atomicAdd(&acc[4 * ((uint64_t)blockIdx.y * NP + pt) + lo], (unsigned long long)red[0][lo]);
-    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[q] = sh::madd(w[q], __shfl_xor(w[q], off, 64));
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 4; q++) red[pt * 4 + wave][q] = w[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 * NP) {
+    const uint32_t pt = threadIdx.x >> 2, q = threadIdx.x & 3;
+    uint32_t sum = 0;
+    for (uint32_t wv = 0; wv < 4; wv++) sum = sh::madd(sum, red[pt * 4 + wv][q]);
+    atomicAdd(&acc[4 * ((uint64_t)blockIdx.y * NP + pt) + q], (unsigned long long)sum);
   }
 }
 
