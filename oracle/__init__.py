@@ -375,3 +375,51 @@ def splitmix64_columns(seed, log_sizes):
     """The shared input generator (proving_amd.workloads), re-exported for the tests."""
     from proving_amd.workloads import splitmix64_columns as gen
     return gen(seed, log_sizes)
+
+
+# ---------------------------------------------------------------- SimdBackend restatement
+_SIMD_PATH = os.path.join(_HERE, "libsimdport.so")
+_simd = None
+
+
+def simd_port_available():
+    """True when libsimdport.so is built and this CPU has AVX-512F."""
+    global _simd
+    if _simd is None:
+        if not os.path.exists(_SIMD_PATH):
+            build()
+        _simd = ctypes.CDLL(_SIMD_PATH)
+        _simd.sp_commit_lifted.restype = ctypes.c_int
+    return bool(_simd.sp_cpu_has_avx512())
+
+
+def _aligned_copy(a):
+    """uint32 copy of a whose data starts on a 64-byte boundary."""
+    a = np.asarray(a, dtype=np.uint32)
+    raw = np.empty(a.size + 16, np.uint32)
+    off = (-raw.ctypes.data % 64) // 4
+    out = raw[off:off + a.size]
+    out[:] = a
+    return out
+
+
+def simd_port_commit(cols, logs, log_blowup, lifting_log_size, nthreads, reps=1):
+    """The trace commitment (interpolate -> LDE -> lifted Merkle root) by the multi-threaded
+    AVX-512 restatement of the SimdBackend path.  cols: bit-reversed evaluations, logs ascending
+    and >= 4.  Returns (root bytes, (fft_s, leaves_s, layers_s)) of the fastest of reps commits."""
+    assert simd_port_available(), "this CPU has no AVX-512F"
+    ok = (len(logs) > 0 and 5 <= lifting_log_size <= 28 and all(lg >= 4 for lg in logs)
+          and list(logs) == sorted(logs) and max(logs) + log_blowup <= lifting_log_size)
+    if not ok:
+        raise ValueError("simd_port_commit: logs must be ascending, >= 4, and fit the tree height")
+    arrs = [_aligned_copy(c) for c in cols]
+    ptrs = (u32p * len(arrs))(*[a.ctypes.data_as(u32p) for a in arrs])
+    lg = np.array(logs, np.uint32)
+    tw, itw = canonic_twiddles(lifting_log_size)
+    root = np.zeros(32, np.uint8)
+    times = (ctypes.c_double * 3)()
+    rc = _simd.sp_commit_lifted(ptrs, _p32(lg), len(arrs), log_blowup, lifting_log_size, _p32(tw),
+                                _p32(itw), ctypes.c_uint64(len(tw)), nthreads, reps, _p8(root), times)
+    if rc != 0:
+        raise ValueError(f"sp_commit_lifted rejected the input ({rc})")
+    return root.tobytes(), tuple(times)
