@@ -171,8 +171,8 @@ def bench_fri_deep(args):
     200 columns x 2^21 (LDE 2^22): one step = OOD sampling of every column at 3 points, mixing the
     samples, drawing the random coefficient, compute_fri_quotients, and FRI commit + decommit with
     32 queries; the channel starts from the same state each step (the phase is deterministic)."""
-    from proving_amd.backend import (BaseColumn, Channel, Context, FriParams, HashLayer, HipBackend,
-                                     fri_prove, fri_quotients)
+    from proving_amd.backend import (BaseColumn, Channel, ColumnSet, Context, FriParams, HashLayerSet, HipBackend,
+                                     fri_prove, fri_quotients_arrays)
     from proving_amd.workloads import FRI_DEEP, splitmix64_columns
     w = FRI_DEEP
     n, N, b = w["log_size"], w["n_cols"], w["fri"]["log_blowup_factor"]
@@ -185,9 +185,9 @@ def bench_fri_deep(args):
     host = splitmix64_columns(w["seed"], [n] * N)
     evals = [BaseColumn.from_cpu(ctx, c) for c in host]
     del host
-    coeffs = [BaseColumn(ctx, n) for _ in range(N)]
-    ldes = [BaseColumn(ctx, L) for _ in range(N)]
-    layers = [HashLayer(ctx, k) for k in range(L + 1)]
+    coeffs = ColumnSet(ctx, [n] * N)
+    ldes = ColumnSet(ctx, [L] * N)
+    layers = HashLayerSet(ctx, L)
     be.commit_trace(evals, b, tree, coeffs, ldes, layers)  # setup: interpolate + LDE (not timed)
     for e in evals:
         e.free()
@@ -198,16 +198,18 @@ def bench_fri_deep(args):
                                                "n_queries", "fold_step")])
     state = bytes(ch0.s.digest), ch0.s.n_draws
     result = {}
+    k = len(points)
+    offsets = np.arange(0, k * N + 1, k, dtype=np.uint32)          # k samples per column
+    sample_points = np.tile(np.asarray(points, np.uint32), (N, 1))  # column-major, point order
 
     def step():
         ch = Channel(ctx.lib)
         ch.s.digest[:] = list(state[0])
         ch.s.n_draws = state[1]
-        vals = be.eval_at_points(coeffs, points)
-        samples = [[(p, v) for p, v in zip(points, vc)] for vc in vals]
-        ch.mix_felts([v for col in samples for _, v in col])
+        vals = be.eval_at_points(coeffs, points, as_array=True).reshape(-1, 4)
+        ch.mix_felts(vals)
         rc = [int(x) for x in ch.draw_secure_felt()]
-        quot = fri_quotients(be, tree, ldes, samples, rc, L, b)
+        quot = fri_quotients_arrays(be, tree, ldes, offsets, sample_points, vals, rc, L, b)
         felts, queries = fri_prove(be, tree, ch, params, quot)
         quot.free()
         result["felts"], result["queries"], result["digest"] = felts, queries, ch.digest
@@ -396,7 +398,7 @@ def main():
               "single MI355X and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
-    from proving_amd.backend import BaseColumn, Context, HashLayer, HipBackend
+    from proving_amd.backend import BaseColumn, ColumnSet, Context, HashLayerSet, HipBackend
     from proving_amd.workloads import PLAIN_COMMIT, UNEVEN_COMMIT, cells as count_cells, splitmix64_columns
 
     if args.copy_calibration:
@@ -428,9 +430,9 @@ def main():
     tree = be.precompute_canonic_twiddles(L)
     host = splitmix64_columns(seed, logs)
     evals = [BaseColumn.from_cpu(ctx, c) for c in host]
-    coeffs = [BaseColumn(ctx, lg) for lg in logs]
-    ldes = [BaseColumn(ctx, lg + b) for lg in logs]
-    layers = [HashLayer(ctx, k) for k in range(L + 1)]
+    coeffs = ColumnSet(ctx, logs)                      # one pooled allocation per output kind
+    ldes = ColumnSet(ctx, [lg + b for lg in logs])
+    layers = HashLayerSet(ctx, L)
     log("[bench] inputs resident in HBM")
     if args.fft_tuning:
         for m, t in json.loads(args.fft_tuning).items():

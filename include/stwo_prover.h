@@ -45,6 +45,15 @@ void sh_generate_secure_powers(const uint32_t felt[4], uint64_t n, uint32_t *out
  * witness and their count to n_witness. */
 int sh_merkle_decommit(sh_ctx *ctx, const uint8_t *const *d_layers, uint32_t L, const uint32_t *positions,
                        uint64_t n_positions, uint8_t *witness, uint64_t cap, uint64_t *n_witness);
+/* CommitmentTreeProver::decommit (crates/stwo/src/prover/pcs/mod.rs:392-398) ->
+ * MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162) in one round
+ * trip: values[c * n_pos + k] = column c (log size log_sizes[c], in [1, L]) at the lifted row of
+ * positions[k] (positions in the given order, duplicates kept), and the hash witness of the
+ * tree of height L (layers d_layers[0..L]) for the sorted, deduplicated positions: up to cap
+ * hashes (32 bytes each) in witness, their count in n_witness. */
+int sh_tree_decommit(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *log_sizes, uint64_t n_cols,
+                     const uint8_t *const *d_layers, uint32_t L, const uint32_t *positions, uint64_t n_pos,
+                     uint32_t *values, uint8_t *witness, uint64_t cap, uint64_t *n_witness);
 /* Column::at for many (column, row) pairs: out[i] = d_cols[i][rows[i]] (one gather launch). */
 int sh_gather(sh_ctx *ctx, const uint32_t *const *d_cols, const uint32_t *rows, uint64_t n, uint32_t *out);
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "sh_generate_secure_powers": (None, [c_vp, c_u64, c_vp]),
     "sh_merkle_decommit": (c_int, [c_vp, c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u64p]),
     "sh_gather": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
+    "sh_tree_decommit": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_u32, c_vp, c_u64, c_vp, c_vp, c_u64, c_u64p]),
     "sh_fri_prove": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),
     "sh_fri_quotients": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
 }

@@ -20,8 +20,24 @@ P = (1 << 31) - 1
 
 
 def _ptrs(bufs):
+    if isinstance(bufs, (ColumnRefs, HashLayerSet)):
+        return bufs.c_ptrs
     arr = (ctypes.c_void_p * max(1, len(bufs)))(*[b.ptr for b in bufs])
     return arr
+
+
+def _logs(cols):
+    """Log sizes of a list of columns or of a ColumnSet (its cached array)."""
+    if isinstance(cols, ColumnRefs):
+        return cols.log_sizes
+    return [c.log_size for c in cols]
+
+
+def _ptr_array(cols):
+    """Device addresses of a list of columns or of a ColumnSet, as a uint64 array."""
+    if isinstance(cols, ColumnRefs):
+        return cols.ptrs
+    return np.array([c.ptr for c in cols], np.uint64)
 
 
 def _u32s(vals):
@@ -152,6 +168,87 @@ class BaseColumn(DeviceBuffer):
                        "bit_reverse_column")
 
 
+class ColumnView(BaseColumn):
+    """A column inside a ColumnSet's allocation.  It borrows the memory: free() only drops the
+    pointer, and the set releases the allocation once no view refers to it."""
+
+    def __init__(self, owner, ptr, log_size):
+        self.ctx = owner.ctx
+        self.nbytes = 4 << log_size
+        self.ptr = ptr
+        self.log_size = log_size
+        self._owner = owner
+
+    def free(self):
+        self.ptr = None
+
+    def __del__(self):
+        pass
+
+
+class ColumnRefs:
+    """Device columns given by address and log size (uint64 / uint32 arrays), not owned: the
+    batched entry points take them without building per-column objects.  The caller keeps the
+    columns alive."""
+
+    def __init__(self, ctx, ptrs, log_sizes):
+        self.ctx = ctx
+        self.ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64).reshape(-1)
+        self.log_sizes = np.ascontiguousarray(log_sizes, dtype=np.uint32).reshape(-1)
+        assert len(self.ptrs) == len(self.log_sizes)
+        n = len(self.ptrs)
+        self.c_ptrs = (ctypes.c_void_p * max(1, n)).from_buffer(self.ptrs if n else np.zeros(1, np.uint64))
+
+    def __len__(self):
+        return len(self.log_sizes)
+
+
+class ColumnSet(ColumnRefs):
+    """Many device columns in one pooled allocation (the role of BaseColumnPool,
+    crates/stwo/src/prover/mempool.rs:14-58, for a whole tree's columns): one sh_alloc and one
+    sh_free for all of them, and the pointer and log-size arrays that the batched entry points take,
+    built once.  Indexing and iteration give ColumnView objects (created on first use).  A view's
+    log_size must not change (split_at_mid works on owned columns only)."""
+
+    def __init__(self, ctx, log_sizes):
+        self.ctx = ctx
+        self.log_sizes = np.ascontiguousarray(log_sizes, dtype=np.uint32).reshape(-1)
+        n = len(self.log_sizes)
+        sizes = np.left_shift(np.uint64(4), self.log_sizes.astype(np.uint64))
+        sizes = (sizes + np.uint64(255)) // np.uint64(256) * np.uint64(256)  # 256-byte aligned columns
+        offs = np.zeros(n, np.uint64)
+        if n > 1:
+            np.cumsum(sizes[:-1], out=offs[1:])
+        self.buf = DeviceBuffer(ctx, int(sizes.sum()) if n else 256)
+        super().__init__(ctx, offs + np.uint64(self.buf.ptr), self.log_sizes)
+        self._views = None
+
+    def _all(self):
+        if self._views is None:
+            self._views = [ColumnView(self, int(p), int(lg)) for p, lg in zip(self.ptrs, self.log_sizes)]
+        return self._views
+
+    def __len__(self):
+        return len(self.log_sizes)
+
+    def __getitem__(self, i):
+        return self._all()[i]
+
+    def __iter__(self):
+        return iter(self._all())
+
+    def __add__(self, other):
+        return list(self) + list(other)
+
+    def __radd__(self, other):
+        return list(other) + list(self)
+
+    def free(self):
+        """Returns the whole allocation to the context pool (every view becomes invalid)."""
+        self.buf.free()
+        self._views = None
+
+
 class SecureColumnByCoords:
     """QM31 column stored as 4 coordinate columns (crates/stwo/src/prover/secure_column.rs:13-63)."""
 
@@ -186,6 +283,47 @@ class HashLayer(DeviceBuffer):
 
     def to_cpu(self):
         return self.download(np.uint8, 32 << self.log_size).reshape(-1, 32)
+
+
+class HashLayerView(HashLayer):
+    """A layer inside a HashLayerSet's allocation (borrowed, like ColumnView)."""
+
+    def __init__(self, owner, ptr, log_size):
+        self.ctx = owner.ctx
+        self.nbytes = 32 << log_size
+        self.ptr = ptr
+        self.log_size = log_size
+        self._owner = owner
+
+    def free(self):
+        self.ptr = None
+
+    def __del__(self):
+        pass
+
+
+class HashLayerSet:
+    """The layers of a Merkle tree of height L (layer k: 2^k hashes, root-first indexing) in one
+    pooled allocation; indexing gives HashLayerView objects."""
+
+    def __init__(self, ctx, L):
+        self.ctx = ctx
+        self.L = L
+        sizes = [max(256, 32 << k) for k in range(L + 1)]
+        offs = np.concatenate([[0], np.cumsum(sizes[:-1])]).astype(np.uint64)
+        self.buf = DeviceBuffer(ctx, int(sum(sizes)))
+        self.ptrs = np.ascontiguousarray(offs + np.uint64(self.buf.ptr))
+        self.c_ptrs = (ctypes.c_void_p * (L + 1)).from_buffer(self.ptrs)
+        self._views = [HashLayerView(self, int(p), k) for k, p in enumerate(self.ptrs)]
+
+    def __len__(self):
+        return self.L + 1
+
+    def __getitem__(self, i):
+        return self._views[i]
+
+    def __iter__(self):
+        return iter(self._views)
 
 
 class TwiddleTree:
@@ -264,7 +402,7 @@ class HipBackend:
 
     def interpolate_columns(self, cols, twiddles):
         self._chk(self.lib.sh_interpolate_columns(self.h, twiddles.h, _ptrs(cols),
-                                                  _u32s([c.log_size for c in cols]), len(cols)),
+                                                  _u32s(_logs(cols)), len(cols)),
                   "interpolate_columns")
         return cols
 
@@ -275,9 +413,9 @@ class HipBackend:
         return out
 
     def evaluate_polynomials(self, polys, log_blowup, twiddles, outs=None):
-        outs = outs or [BaseColumn(self.ctx, p.log_size + log_blowup) for p in polys]
+        outs = outs if outs is not None else ColumnSet(self.ctx, np.asarray(_logs(polys), np.uint32) + log_blowup)
         self._chk(self.lib.sh_evaluate_polynomials(self.h, twiddles.h, _ptrs(polys),
-                                                   _u32s([p.log_size for p in polys]), len(polys),
+                                                   _u32s(_logs(polys)), len(polys),
                                                    log_blowup, _ptrs(outs)), "evaluate_polynomials")
         return outs
 
@@ -285,7 +423,7 @@ class HipBackend:
     def build_leaves(self, cols_sorted_asc, lifting_log_size):
         out = HashLayer(self.ctx, lifting_log_size if cols_sorted_asc else 0)
         self._chk(self.lib.sh_merkle_build_leaves(self.h, _ptrs(cols_sorted_asc),
-                                                  _u32s([c.log_size for c in cols_sorted_asc]),
+                                                  _u32s(_logs(cols_sorted_asc)),
                                                   len(cols_sorted_asc), lifting_log_size, out.ptr),
                   "build_leaves")
         return out
@@ -298,9 +436,9 @@ class HipBackend:
 
     def commit_lifted(self, cols, lifting_log_size, layers=None, want_root=True):
         """MerkleProverLifted::commit: returns (layers root-first, root bytes or None)."""
-        layers = layers or [HashLayer(self.ctx, k) for k in range(lifting_log_size + 1)]
+        layers = layers if layers is not None else HashLayerSet(self.ctx, lifting_log_size)
         root = (ctypes.c_uint8 * 32)() if want_root else None
-        self._chk(self.lib.sh_merkle_commit_lifted(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
+        self._chk(self.lib.sh_merkle_commit_lifted(self.h, _ptrs(cols), _u32s(_logs(cols)),
                                                    len(cols), lifting_log_size, _ptrs(layers), root),
                   "commit_lifted")
         return layers, (bytes(root) if want_root else None)
@@ -316,7 +454,7 @@ class HipBackend:
         max_log = max(c.log_size for c in cols) if cols else 0
         layers = [HashLayer(self.ctx, k) for k in range(max_log + 1)]
         root = (ctypes.c_uint8 * 32)()
-        self._chk(self.lib.sh_merkle_commit_mixed(self.h, _ptrs(cols), _u32s([c.log_size for c in cols]),
+        self._chk(self.lib.sh_merkle_commit_mixed(self.h, _ptrs(cols), _u32s(_logs(cols)),
                                                   len(cols), _ptrs(layers), root), "commit_mixed")
         return layers, bytes(root)
 
@@ -324,7 +462,7 @@ class HipBackend:
         """Captures commit_trace on these buffers as a HIP graph; returns a CommitGraph."""
         h = ctypes.c_void_p()
         self._chk(self.lib.sh_commit_trace_graph_create(
-            self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s([e.log_size for e in evals]), len(evals),
+            self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s(_logs(evals)), len(evals),
             log_blowup, _ptrs(ldes), _ptrs(layers), ctypes.byref(h)), "commit_trace_graph_create")
         g = CommitGraph(self.ctx, h, layers)
         g.buffers = (list(evals), list(coeffs), list(ldes), twiddles)  # the graph reads them on every launch
@@ -359,13 +497,14 @@ class HipBackend:
     # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------
     def commit_trace(self, evals, log_blowup, twiddles, coeffs=None, ldes=None, layers=None,
                      want_root=True):
-        coeffs = coeffs or [BaseColumn(self.ctx, e.log_size) for e in evals]
-        ldes = ldes or [BaseColumn(self.ctx, e.log_size + log_blowup) for e in evals]
-        L = max(e.log_size for e in evals) + log_blowup
-        layers = layers or [HashLayer(self.ctx, k) for k in range(L + 1)]
+        logs = np.asarray(_logs(evals), np.uint32)
+        coeffs = coeffs if coeffs is not None else ColumnSet(self.ctx, logs)
+        ldes = ldes if ldes is not None else ColumnSet(self.ctx, logs + log_blowup)
+        L = int(logs.max()) + log_blowup
+        layers = layers if layers is not None else HashLayerSet(self.ctx, L)
         root = (ctypes.c_uint8 * 32)() if want_root else None
         self._chk(self.lib.sh_commit_trace(self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs),
-                                           _u32s([e.log_size for e in evals]), len(evals), log_blowup,
+                                           _u32s(_logs(evals)), len(evals), log_blowup,
                                            _ptrs(ldes), _ptrs(layers), root), "commit_trace")
         return coeffs, ldes, layers, (bytes(root) if want_root else None)
 
@@ -393,14 +532,16 @@ class _Ops:
         self._chk(self.lib.sh_eval_at_point(self.h, coeffs.ptr, coeffs.log_size, pt, out), "eval_at_point")
         return np.array(out, np.uint32)
 
-    def eval_at_points(self, coeffs, points):
+    def eval_at_points(self, coeffs, points, as_array=False):
         """eval_at_point of every polynomial at every point, one batched call; returns
-        out[i][j] = coeffs[i] at points[j] (QM31 as 4 ints)."""
+        out[i][j] = coeffs[i] at points[j] (QM31 as 4 ints; a (n, k, 4) uint32 array when
+        as_array)."""
         n, k = len(coeffs), len(points)
         out = np.empty(max(1, 4 * n * k), np.uint32)
-        self._chk(self.lib.sh_eval_at_points(self.h, _ptrs(coeffs), _u32s([c.log_size for c in coeffs]), n,
+        self._chk(self.lib.sh_eval_at_points(self.h, _ptrs(coeffs), _u32s(_logs(coeffs)), n,
                                              _u32s(points), k, out.ctypes.data), "eval_at_points")
-        return out[:4 * n * k].reshape(n, k, 4).tolist()
+        out = out[:4 * n * k].reshape(n, k, 4)
+        return out if as_array else out.tolist()
 
     def barycentric_weights(self, log_size, point):
         w = SecureColumnByCoords(self.ctx, log_size)
@@ -478,7 +619,7 @@ class _Ops:
             return None
         out = SecureColumnByCoords(self.ctx, cols[-1].log_size)
         self._chk(self.lib.sh_lift_and_accumulate(self.h, _ptrs([c for col in cols for c in col.columns]),
-                                                  _u32s([c.log_size for c in cols]), len(cols),
+                                                  _u32s(_logs(cols)), len(cols),
                                                   _ptrs(out.columns)), "lift_and_accumulate")
         return out
 
@@ -551,7 +692,11 @@ class Channel:
         self.lib.sh_channel_mix_u64(ctypes.byref(self.s), v)
 
     def mix_felts(self, felts):
-        w = [int(x) for f in felts for x in f]
+        """felts: QM31 values as 4-int sequences, or a (m, 4) uint32 array."""
+        if isinstance(felts, np.ndarray):
+            w = np.ascontiguousarray(felts, dtype=np.uint32).reshape(-1)
+        else:
+            w = np.asarray([int(x) for f in felts for x in f], dtype=np.uint32)
         self.lib.sh_channel_mix_felts(ctypes.byref(self.s), _u32s(w), len(w) // 4)
 
     def mix_root(self, root: bytes):
@@ -593,6 +738,26 @@ def merkle_decommit(backend, layers, positions):
     return [raw[32 * i:32 * i + 32] for i in range(nw.value)]
 
 
+def tree_decommit(backend, cols, layers, positions):
+    """CommitmentTreeProver::decommit (crates/stwo/src/prover/pcs/mod.rs:392-398) in one round
+    trip: the queried values (n_cols, len(positions)) at the lifted rows of the positions (in the
+    given order), and the hash witness of the sorted, deduplicated positions as a (n, 32) uint8
+    array.  layers: root-first, as commit_lifted returns them."""
+    L = len(layers) - 1
+    pos = np.ascontiguousarray(positions, dtype=np.uint32).reshape(-1)
+    n, k = len(cols), len(pos)
+    values = np.empty((n, k), np.uint32)
+    cap = max(1, k * max(1, L))
+    wit = np.empty((cap, 32), np.uint8)
+    nw = ctypes.c_uint64()
+    backend._chk(backend.lib.sh_tree_decommit(backend.h, _ptrs(cols), _u32s(_logs(cols)), n,
+                                              _ptrs(layers) if L >= 0 else None, max(L, 0),
+                                              pos.ctypes.data if k else None, k,
+                                              values.ctypes.data if n and k else None, wit.ctypes.data, cap,
+                                              ctypes.byref(nw)), "tree_decommit")
+    return values, wit[:nw.value]
+
+
 def gather_ptrs(backend, ptrs, rows):
     """out[i] = word rows[i] of the device column at address ptrs[i] (numpy arrays), one launch."""
     ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64)
@@ -614,7 +779,11 @@ def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22, as_array=
     """FriProver::commit + decommit (crates/stwo/src/prover/fri.rs:113-284) on a device QM31
     column; returns (proof felts, sorted query positions); the felts as a uint64 array when
     as_array, else as a list."""
-    felts = np.empty(cap, np.uint64)  # not zero-filled: only the written prefix is read
+    buf = getattr(backend, "_fri_felt_buf", None)
+    if buf is None or len(buf) < cap:
+        # kept on the backend: a fresh buffer would be paged in again on every proof
+        buf = backend._fri_felt_buf = np.empty(cap, np.uint64)
+    felts = buf
     nf = ctypes.c_uint64()
     qp = np.empty(max(1, params.n_queries), np.uint32)
     nq = ctypes.c_uint64()
@@ -629,14 +798,24 @@ def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log
     """compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160).
     columns: committed evaluations (BaseColumn) in tree/column order; samples[i]: list of
     (point[8], value[4]) for column i."""
-    out = out or SecureColumnByCoords(backend.ctx, lifting_log_size)
     offs = np.zeros(len(samples) + 1, np.uint32)
     np.cumsum([len(smp) for smp in samples], out=offs[1:])
     rows = np.array([[*pt, *v] for smp in samples for pt, v in smp], dtype=np.uint64).reshape(-1, 12)
-    pts = np.ascontiguousarray(rows[:, :8], dtype=np.uint32)
-    vals = np.ascontiguousarray(rows[:, 8:], dtype=np.uint32)
+    return fri_quotients_arrays(backend, twiddles, columns, offs, rows[:, :8], rows[:, 8:], random_coeff,
+                                lifting_log_size, log_blowup, out)
+
+
+def fri_quotients_arrays(backend, twiddles, columns, offsets, points, values, random_coeff, lifting_log_size,
+                         log_blowup, out=None):
+    """fri_quotients with the samples as arrays: column i's samples are rows offsets[i] ..
+    offsets[i + 1] - 1 of points ((m, 8) words, x then y) and values ((m, 4) words)."""
+    out = out or SecureColumnByCoords(backend.ctx, lifting_log_size)
+    offs = np.ascontiguousarray(offsets, dtype=np.uint32)
+    assert len(offs) == len(columns) + 1
+    pts = np.ascontiguousarray(points, dtype=np.uint32).reshape(-1, 8)
+    vals = np.ascontiguousarray(values, dtype=np.uint32).reshape(-1, 4)
     backend._chk(backend.lib.sh_fri_quotients(
-        backend.h, twiddles.h, _ptrs(columns), _u32s([c.log_size for c in columns]), len(columns),
+        backend.h, twiddles.h, _ptrs(columns), _u32s(_logs(columns)), len(columns),
         offs.ctypes.data, pts.ctypes.data if len(pts) else None, vals.ctypes.data if len(vals) else None,
         _u32s(list(random_coeff)), lifting_log_size, log_blowup, _ptrs(out.columns)), "fri_quotients")
     return out

@@ -83,6 +83,7 @@ enum {
   SCR_MISC = 4,
   SCR_QUOT = 5,
   SCR_GRIND = 6,
+  SCR_EVAL = 7,
 };
 
 int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...);

@@ -49,26 +49,60 @@ __global__ void k_gather(const uint32_t* const* cols, const uint32_t* rows, uint
   if (i < n) out[i] = cols[i][rows[i]];
 }
 
+// 32-byte nodes: out[2i], out[2i + 1] = the two halves of *nodes[i].
+__global__ void k_gather_nodes(const uint4* const* nodes, uint32_t n, uint4* out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * n) out[i] = nodes[i >> 1][i & 1];
+}
+
+// Reads words (cols[i][rows[i]]) and 32-byte nodes (*nodes[j]) into host memory: one upload of
+// the address tables, one launch each, one download, one wait.
+int gather_words_nodes(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& rows,
+                       const std::vector<const uint8_t*>& nodes, std::vector<uint32_t>& words,
+                       std::vector<uint32_t>& node_words) {
+  const uint64_t nw = cols.size(), nn = nodes.size();
+  words.assign(nw, 0);
+  node_words.assign(8 * nn, 0);
+  if (nw + nn == 0) return SH_OK;
+  // device layout: [node addresses][column addresses][rows (padded to 16 B)][node words][words]
+  const uint64_t rows_bytes = (4 * nw + 15) & ~15ull;
+  const uint64_t tab_bytes = 8 * nn + 8 * nw + rows_bytes;
+  const uint64_t out_bytes = 32 * nn + 4 * nw;
+  std::vector<uint8_t> tab(tab_bytes, 0);
+  if (nn) memcpy(tab.data(), nodes.data(), 8 * nn);
+  if (nw) {
+    memcpy(tab.data() + 8 * nn, cols.data(), 8 * nw);
+    memcpy(tab.data() + 8 * nn + 8 * nw, rows.data(), 4 * nw);
+  }
+  void* scr;
+  SH_TRY(sh_scratch_get(ctx, SCR_MISC, tab_bytes + out_bytes, &scr));
+  uint8_t* base = (uint8_t*)scr;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(base, tab.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+  uint8_t* d_out = base + tab_bytes;
+  if (nn) {
+    k_gather_nodes<<<(unsigned)((2 * nn + 255) / 256), 256, 0, ctx->stream>>>((const uint4* const*)base,
+                                                                               (uint32_t)nn, (uint4*)d_out);
+    SH_TRY(sh_launch_check(ctx, "gather_nodes"));
+  }
+  if (nw) {
+    k_gather<<<(unsigned)((nw + 255) / 256), 256, 0, ctx->stream>>>(
+        (const uint32_t* const*)(base + 8 * nn), (const uint32_t*)(base + 8 * nn + 8 * nw), (uint32_t)nw,
+        (uint32_t*)(d_out + 32 * nn));
+    SH_TRY(sh_launch_check(ctx, "gather"));
+  }
+  std::vector<uint32_t> host(out_bytes / 4);
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(host.data(), d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nn) memcpy(node_words.data(), host.data(), 32 * nn);
+  if (nw) memcpy(words.data(), host.data() + 8 * nn, 4 * nw);
+  return SH_OK;
+}
+
 // Reads n words (d_cols[i][rows[i]]) into host memory with one launch.
 int gather_words(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& rows,
                  std::vector<uint32_t>& out) {
-  const uint64_t n = cols.size();
-  out.assign(n, 0);
-  if (n == 0) return SH_OK;
-  // pointer and row tables are per call (not cached): stage them through scratch
-  void* scr;
-  SH_TRY(sh_scratch_get(ctx, SCR_MISC, n * (sizeof(void*) + 8), &scr));
-  uint8_t* base = (uint8_t*)scr;
-  const uint32_t** d_cols = (const uint32_t**)base;
-  uint32_t* d_rows = (uint32_t*)(base + n * sizeof(void*));
-  uint32_t* d_out = d_rows + n;
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_cols, cols.data(), n * sizeof(void*), hipMemcpyHostToDevice, ctx->stream));
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_rows, rows.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
-  k_gather<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(d_cols, d_rows, (uint32_t)n, d_out);
-  SH_TRY(sh_launch_check(ctx, "gather"));
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(out.data(), d_out, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return SH_OK;
+  std::vector<uint32_t> none;
+  return gather_words_nodes(ctx, cols, rows, {}, out, none);
 }
 
 // The nodes (layer, index) of MerkleProverLifted::decommit's hash witness, in witness order
@@ -109,15 +143,10 @@ int merkle_decommit(sh_ctx* ctx, const std::vector<const uint8_t*>& layers, uint
     }
     queries = next;
   }
-  std::vector<const uint32_t*> cols;
-  std::vector<uint32_t> rows;
-  for (auto& nd : need)
-    for (uint32_t w = 0; w < 8; w++) {
-      cols.push_back((const uint32_t*)layers[nd.first]);
-      rows.push_back(8 * nd.second + w);
-    }
-  std::vector<uint32_t> words;
-  SH_TRY(gather_words(ctx, cols, rows, words));
+  std::vector<const uint8_t*> nodes(need.size());
+  for (size_t k = 0; k < need.size(); k++) nodes[k] = layers[need[k].first] + 32ull * need[k].second;
+  std::vector<uint32_t> none, words;
+  SH_TRY(gather_words_nodes(ctx, {}, {}, nodes, none, words));
   witness.resize(need.size());
   for (size_t k = 0; k < need.size(); k++) memcpy(witness[k].data(), &words[8 * k], 32);
   return SH_OK;
@@ -237,19 +266,6 @@ struct Felts {
       memcpy(&w, x.data() + 4 * i, 4);
       u(w);
     }
-  }
-};
-
-struct LayerProof {
-  std::vector<sh::qm31> fri_witness;
-  std::vector<Hash> hash_witness;
-  Hash commitment{};
-  void serialize(Felts& f) const {  // FriLayerProof (serialize.rs:66-77)
-    f.u(fri_witness.size());
-    for (auto& x : fri_witness) f.q(x);
-    f.u(hash_witness.size());
-    for (auto& x : hash_witness) f.h(x);
-    f.h(commitment);
   }
 };
 
@@ -380,6 +396,77 @@ extern "C" int sh_merkle_decommit(sh_ctx* ctx, const uint8_t* const* d_layers, u
   SH_REQUIRE(ctx, w.size() <= cap, "witness of %zu hashes exceeds the buffer (%llu)", w.size(),
              (unsigned long long)cap);
   for (size_t k = 0; k < w.size(); k++) memcpy(witness + 32 * k, w[k].data(), 32);
+  return SH_OK;
+}
+
+// Queried values at lifted rows: out[c * n_pos + k] = cols[c][lift(pos[k])] with
+// lift(p) = ((p >> (L - log_c + 1)) << 1) + (p & 1) (crates/stwo/src/prover/vcs_lifted/prover.rs:97-116).
+__global__ void k_gather_lifted(const uint32_t* const* cols, const uint32_t* logs, uint32_t n_cols, uint32_t L,
+                                const uint32_t* pos, uint32_t n_pos, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cols * n_pos) return;
+  const uint32_t c = i / n_pos, k = i - c * n_pos;
+  const uint32_t p = pos[k];
+  out[i] = cols[c][((p >> (L - logs[c] + 1)) << 1) + (p & 1)];
+}
+
+extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
+                                uint64_t n_cols, const uint8_t* const* d_layers, uint32_t L,
+                                const uint32_t* positions, uint64_t n_pos, uint32_t* values, uint8_t* witness,
+                                uint64_t cap, uint64_t* n_witness) {
+  if (!ctx || !n_witness || (n_pos && !positions) || (n_cols && n_pos && (!d_cols || !log_sizes || !values)) ||
+      (L > 0 && n_pos && !d_layers))
+    return SH_ERR_INVALID;
+  *n_witness = 0;
+  SH_REQUIRE(ctx, L < 31, "tree height %u out of range", L);
+  SH_REQUIRE(ctx, n_cols * n_pos < (1ull << 31), "too many queried values");
+  for (uint64_t k = 0; k < n_pos; k++)
+    SH_REQUIRE(ctx, positions[k] < (1u << L), "query position %u out of range", positions[k]);
+  for (uint64_t c = 0; c < n_cols && n_pos; c++)
+    SH_REQUIRE(ctx, log_sizes[c] >= 1 && log_sizes[c] <= L, "column log size %u not in [1, %u]", log_sizes[c], L);
+  std::vector<std::pair<uint32_t, uint32_t>> need;
+  if (L > 0 && n_pos) need = merkle_witness_nodes(L, std::vector<uint32_t>(positions, positions + n_pos));
+  SH_REQUIRE(ctx, need.size() <= cap, "witness of %zu hashes exceeds the buffer (%llu)", need.size(),
+             (unsigned long long)cap);
+  const uint64_t nn = need.size(), nv = n_pos ? n_cols * n_pos : 0;
+  if (nn + nv == 0) return SH_OK;
+  // device layout: [node addresses][column addresses][log sizes][positions][node words][values]
+  const uint64_t nc = nv ? n_cols : 0, np_ = nv ? n_pos : 0;
+  const uint64_t o_cols = 8 * nn, o_logs = o_cols + 8 * nc, o_pos = o_logs + 4 * nc;
+  const uint64_t tab_bytes = (o_pos + 4 * np_ + 15) & ~15ull;
+  const uint64_t out_bytes = 32 * nn + 4 * nv;
+  std::vector<uint8_t> tab(tab_bytes, 0);
+  for (uint64_t k = 0; k < nn; k++) {
+    const uint8_t* a = d_layers[need[k].first] + 32ull * need[k].second;
+    memcpy(tab.data() + 8 * k, &a, 8);
+  }
+  if (nv) {
+    memcpy(tab.data() + o_cols, d_cols, 8 * nc);
+    memcpy(tab.data() + o_logs, log_sizes, 4 * nc);
+    memcpy(tab.data() + o_pos, positions, 4 * np_);
+  }
+  void* scr;
+  SH_TRY(sh_scratch_get(ctx, SCR_MISC, tab_bytes + out_bytes, &scr));
+  uint8_t* base = (uint8_t*)scr;
+  uint8_t* d_out = base + tab_bytes;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(base, tab.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
+  if (nn) {
+    k_gather_nodes<<<(unsigned)((2 * nn + 255) / 256), 256, 0, ctx->stream>>>((const uint4* const*)base,
+                                                                               (uint32_t)nn, (uint4*)d_out);
+    SH_TRY(sh_launch_check(ctx, "gather_nodes"));
+  }
+  if (nv) {
+    k_gather_lifted<<<(unsigned)((nv + 255) / 256), 256, 0, ctx->stream>>>(
+        (const uint32_t* const*)(base + o_cols), (const uint32_t*)(base + o_logs), (uint32_t)nc, L,
+        (const uint32_t*)(base + o_pos), (uint32_t)np_, (uint32_t*)(d_out + 32 * nn));
+    SH_TRY(sh_launch_check(ctx, "gather_lifted"));
+  }
+  std::vector<uint8_t> host(out_bytes);
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(host.data(), d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nn) memcpy(witness, host.data(), 32 * nn);
+  if (nv) memcpy(values, host.data() + 32 * nn, 4 * nv);
+  *n_witness = nn;
   return SH_OK;
 }
 
@@ -570,6 +657,7 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   std::vector<Plan> plans(layers_all.size());
   std::vector<const uint32_t*> gc;
   std::vector<uint32_t> gr;
+  std::vector<const uint8_t*> gn;
   std::vector<uint32_t> lq = q;
   for (size_t li = 0; li < layers_all.size(); li++) {
     const FriLayer& L = *layers_all[li];
@@ -597,34 +685,30 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
         gc.push_back(L.cols[c]);
         gr.push_back(pos);
       }
-    for (auto& nd : plans[li].nodes)
-      for (uint32_t w = 0; w < 8; w++) {
-        gc.push_back((const uint32_t*)L.tree.layers[nd.first]);
-        gr.push_back(8 * nd.second + w);
-      }
+    for (auto& nd : plans[li].nodes) gn.push_back(L.tree.layers[nd.first] + 32ull * nd.second);
     for (auto& x : lq) x >>= k;  // Queries::fold (crates/stwo/src/core/queries.rs:48-55)
     lq.erase(std::unique(lq.begin(), lq.end()), lq.end());
   }
-  std::vector<uint32_t> gw;
-  TRY_CLEAN(gather_words(ctx, gc, gr, gw));
-  std::vector<LayerProof> proofs(layers_all.size());
-  size_t off = 0;
-  for (size_t li = 0; li < layers_all.size(); li++) {
-    for (size_t k = 0; k < plans[li].witness_pos.size(); k++, off += 4) proofs[li].fri_witness.push_back(sh::qload(&gw[off]));
-    for (size_t k = 0; k < plans[li].nodes.size(); k++, off += 8) {
-      Hash h;
-      memcpy(h.data(), &gw[off], 32);
-      proofs[li].hash_witness.push_back(h);
-    }
-    proofs[li].commitment = layers_all[li]->tree.root;
-  }
-  LayerProof first_proof = proofs[0];
-  std::vector<LayerProof> inner_proofs(proofs.begin() + 1, proofs.end());
-  // FriProof (crates/cairo-serialize/src/serialize.rs:79-90)
+  std::vector<uint32_t> gw, gnw;
+  TRY_CLEAN(gather_words_nodes(ctx, gc, gr, gn, gw, gnw));
+  // FriProof (crates/cairo-serialize/src/serialize.rs:79-90): the first layer, the inner layers
+  // (length-prefixed), each FriLayerProof (serialize.rs:66-77) = fri_witness, decommitment,
+  // commitment, written straight from the gathered words
   Felts f;
-  first_proof.serialize(f);
-  f.u(inner_proofs.size());
-  for (auto& lp : inner_proofs) lp.serialize(f);
+  f.v.reserve(gw.size() + gnw.size() + 16 * layers_all.size() + 8 * last_coeffs.size() + 16);
+  size_t off = 0, noff = 0;
+  for (size_t li = 0; li < layers_all.size(); li++) {
+    if (li == 1) f.u(layers_all.size() - 1);
+    const size_t nwit = plans[li].witness_pos.size(), nnodes = plans[li].nodes.size();
+    f.u(nwit);
+    f.v.insert(f.v.end(), gw.begin() + off, gw.begin() + off + 4 * nwit);
+    off += 4 * nwit;
+    f.u(nnodes);
+    f.v.insert(f.v.end(), gnw.begin() + noff, gnw.begin() + noff + 8 * nnodes);
+    noff += 8 * nnodes;
+    f.h(layers_all[li]->tree.root);
+  }
+  if (layers_all.size() == 1) f.u(0);
   f.u(last_coeffs.size());
   for (auto& x : last_coeffs) f.q(x);
   f.u(p->log_last_layer_degree_bound);

@@ -6,7 +6,8 @@
 // f_k = pi^(k-1)(p.x) (the fold of crates/stwo/src/core/poly/utils.rs:27-38).
 // Batched on the device: the basis value of index i = hi * 2^s + lo factors as
 // B_hi[hi] * B_lo[lo] (products of the factors of the high and the low bits), so
-// with B_lo (2^s words) and B_hi (2^(n-s)) tabulated per point on the host, lane
+// with B_lo (2^s words) and B_hi (2^(n-s)) tabulated per point on the device (each entry
+// the product of its bits' factors, k_eval_tables; the host computes only the factors), lane
 // lo of a block sums c[hi * 2^s + lo] * B_hi[hi] over a range of hi (4 lazy
 // 32x32->64 multiply-adds per coefficient and point; B_hi is uniform across the
 // block), multiplies by B_lo[lo] once, and the block's sum is added into 64-bit
@@ -26,6 +27,28 @@ namespace {
 constexpr uint32_t kEvalLoLog = 8;   // 256 lanes: the low 8 index bits
 constexpr uint32_t kEvalHiPerBlock = 256;
 constexpr uint32_t kEvalMaxPoints = 4;
+
+// The factors f_k of up to kEvalMaxPoints points (QM31, k < 31), passed by value.
+struct EvalFactors {
+  uint32_t w[kEvalMaxPoints][31][4];
+};
+
+// b_lo[(i * np + pt)] = prod_{bit j of i} f[pt][j] for i < 2^sl, and
+// b_hi[(i * np + pt)] = prod_{bit j of i} f[pt][sl + j] for i < 2^nh (QM31 words).
+__global__ __launch_bounds__(256) void k_eval_tables(EvalFactors f, uint32_t np, uint32_t sl, uint32_t nh,
+                                                     uint32_t* __restrict__ b_lo, uint32_t* __restrict__ b_hi) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n_lo = np << sl, n_hi = np << nh;
+  if (t >= n_lo + n_hi) return;
+  const bool lo = t < n_lo;
+  const uint32_t e = lo ? t : t - n_lo;
+  const uint32_t i = e / np, pt = e - i * np;
+  const uint32_t bits = lo ? sl : nh, first = lo ? 0 : sl;
+  sh::qm31 acc = sh::qone();
+  for (uint32_t j = 0; j < bits; j++)
+    if ((i >> j) & 1) acc = sh::qmul(acc, sh::qload(f.w[pt][first + j]));
+  sh::qstore(acc, (lo ? b_lo : b_hi) + 4ull * e);
+}
 
 __device__ __forceinline__ void lfold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
 __device__ __forceinline__ uint32_t lfinal(uint64_t v) {
@@ -212,28 +235,18 @@ extern "C" int sh_eval_at_points(sh_ctx* ctx, const uint32_t* const* d_coeffs, c
     for (uint32_t n : classes) {
       // factor of index bit k: bit 0 -> y, bit 1 -> x, bit k -> pi^(k-1)(x)
       const uint32_t sl = std::min(n, kEvalLoLog), nh = n - sl;
-      std::vector<uint32_t> blo(4ull * np << sl), bhi(4ull * np << nh);
+      EvalFactors fac;
       for (uint32_t pt = 0; pt < np; pt++) {
         const uint32_t* P8 = points + 8ull * (p0 + pt);
-        std::vector<sh::qm31> f(n);
         sh::qm31 x = sh::qload(P8);
         for (uint32_t k = 0; k < n; k++) {
           if (k == 0) {
-            f[0] = sh::qload(P8 + 4);
+            sh::qstore(sh::qload(P8 + 4), fac.w[pt][0]);
           } else {
-            f[k] = x;
+            sh::qstore(x, fac.w[pt][k]);
             x = sh::qdouble_x(x);
           }
         }
-        auto fill = [&](std::vector<uint32_t>& t, uint32_t bits, uint32_t first) {
-          sh::qstore(sh::qone(), &t[4ull * pt]);
-          for (uint32_t j = 0; j < bits; j++)
-            for (uint64_t i = 1ull << j; i < (2ull << j); i++)
-              sh::qstore(sh::qmul(sh::qload(&t[4ull * ((i - (1ull << j)) * np + pt)]), f[first + j]),
-                         &t[4ull * (i * np + pt)]);
-        };
-        fill(blo, sl, 0);
-        fill(bhi, nh, sl);
       }
       std::vector<const void*> ptrs;
       std::vector<uint64_t> idx;
@@ -243,10 +256,15 @@ extern "C" int sh_eval_at_points(sh_ctx* ctx, const uint32_t* const* d_coeffs, c
           idx.push_back(i);
         }
       // columns of this class may not be contiguous in the output: accumulate into a class slice
-      void *d_ptrs, *d_lo, *d_hi, *d_cls;
+      void *d_ptrs, *d_tab, *d_cls;
       SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_ptrs));
-      SH_TRY(sh_stage(ctx, blo.data(), 4 * blo.size(), &d_lo));
-      SH_TRY(sh_stage(ctx, bhi.data(), 4 * bhi.size(), &d_hi));
+      const uint64_t lo_words = (4ull * np) << sl, hi_words = (4ull * np) << nh;
+      SH_TRY(sh_scratch_get(ctx, SCR_EVAL, 4 * (lo_words + hi_words), &d_tab));
+      uint32_t* d_lo = (uint32_t*)d_tab;
+      uint32_t* d_hi = d_lo + lo_words;
+      const uint32_t n_tab = (np << sl) + (np << nh);
+      k_eval_tables<<<(n_tab + 255) / 256, 256, 0, ctx->stream>>>(fac, np, sl, nh, d_lo, d_hi);
+      SH_TRY(sh_launch_check(ctx, "eval_tables"));
       SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 32ull * ptrs.size() * np, &d_cls));
       SH_TRY(sh_memset(ctx, d_cls, 0, 32ull * ptrs.size() * np));
       dim3 grid((unsigned)(((1ull << nh) + kEvalHiPerBlock - 1) / kEvalHiPerBlock), (unsigned)ptrs.size());

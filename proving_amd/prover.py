@@ -16,8 +16,8 @@ import ctypes
 import numpy as np
 
 from . import field as F
-from .backend import (BaseColumn, Channel, FriParams, HashLayer, SecureColumnByCoords, fri_prove, fri_quotients,
-                      gather_ptrs, merkle_decommit)
+from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, FriParams, HashLayerSet, SecureColumnByCoords,
+                      _logs, _ptr_array, fri_prove, fri_quotients_arrays, tree_decommit)
 
 OODS_POW_BITS = 16  # crates/stwo/src/core/proof_of_work.rs:8
 COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
@@ -35,17 +35,10 @@ class CommitmentTree:
         """CommitmentTreeProver::decommit (crates/stwo/src/prover/pcs/mod.rs:392-398) ->
         MerkleProverLifted::decommit (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162):
         queried values per column at every position (duplicates kept), and the hash witness."""
-        L = self.lifting_log_size
-        k = len(positions)
-        pos = np.asarray(positions, dtype=np.int64)
-        if k and self.evals:
-            rows = np.concatenate([((pos >> (L - e.log_size + 1)) << 1) + (pos & 1) for e in self.evals])
-            ptrs = np.repeat(np.array([e.ptr for e in self.evals], np.uint64), k)
-            queried = gather_ptrs(backend, ptrs, rows).reshape(len(self.evals), k)  # one launch
-        else:
-            queried = np.zeros((len(self.evals), k), np.uint32)
-        witness = merkle_decommit(backend, self.layers, sorted(set(positions))) if L > 0 else []
-        return queried, witness
+        if self.lifting_log_size == 0:
+            assert not len(self.evals), "a tree of height 0 holds no columns"
+            return np.zeros((0, len(positions)), np.uint32), np.zeros((0, 32), np.uint8)
+        return tree_decommit(backend, self.evals, self.layers, positions)  # one round trip
 
 
 class CommitmentSchemeProver:
@@ -63,11 +56,17 @@ class CommitmentSchemeProver:
         return self.preprocessed_lifting_log_size if tree_index == 0 else self.trace_lifting_log_size
 
     def _commit(self, coeffs, evals, channel):
-        ctx = self.be.ctx
         L = self.lifting_log_size(len(self.trees))
-        if not evals:
+        if not len(evals):
             assert L == 0, "lifting_log_size must be 0 when no columns are committed"
-        layers, root = self.be.commit_lifted(evals, L)
+            # the empty tree (root = the hash of nothing) is the same for every proof: committed
+            # once per backend, then reused
+            empty = getattr(self.be, "_empty_tree", None)
+            if empty is None:
+                empty = self.be._empty_tree = self.be.commit_lifted([], 0)
+            layers, root = empty
+        else:
+            layers, root = self.be.commit_lifted(evals, L)
         channel.mix_root(root)
         self.trees.append(CommitmentTree(coeffs, evals, layers, root, L))
         return self.trees[-1]
@@ -78,10 +77,12 @@ class CommitmentSchemeProver:
         the tree height is the largest column's blown-up size (the usual case)."""
         b = self.fri["log_blowup_factor"]
         L = self.lifting_log_size(len(self.trees))
-        if evals and L == max(e.log_size for e in evals) + b:
-            coeffs = [BaseColumn(self.be.ctx, e.log_size) for e in evals]
-            ldes = [BaseColumn(self.be.ctx, e.log_size + b) for e in evals]
-            layers = [HashLayer(self.be.ctx, k) for k in range(L + 1)]
+        logs = np.asarray(_logs(evals), np.uint32)
+        if len(evals) and L == int(logs.max()) + b:
+            # every coefficient and every LDE column of the tree in one pooled allocation each
+            coeffs = ColumnSet(self.be.ctx, logs)
+            ldes = ColumnSet(self.be.ctx, logs + b)
+            layers = HashLayerSet(self.be.ctx, L)
             _, _, _, root = self.be.commit_trace(evals, b, self.tw, coeffs=coeffs, ldes=ldes, layers=layers)
             channel.mix_root(root)
             self.trees.append(CommitmentTree(coeffs, ldes, layers, root, L))
@@ -91,7 +92,8 @@ class CommitmentSchemeProver:
             c = BaseColumn(self.be.ctx, e.log_size)
             c.copy_from(e)
             coeffs.append(c)
-        self.be.interpolate_columns(coeffs, self.tw)
+        if coeffs:
+            self.be.interpolate_columns(coeffs, self.tw)
         return self.commit_polys(coeffs, channel)
 
     def commit_polys(self, coeffs, channel):
@@ -107,35 +109,60 @@ class CommitmentSchemeProver:
         proof as a dict)."""
         L = self.trees[-1].lifting_log_size
         b = self.fri["log_blowup_factor"]
+        ctx = self.be.ctx
         # compute_samples (crates/stwo/src/prover/pcs/mod.rs:176-222): coefficient evaluation at the
-        # point doubled down to the column's own domain
-        # one batched evaluation per distinct (doubled) point over all columns that sample it
-        jobs, doubled = {}, {}
-        for ti, (tree, pts_tree) in enumerate(zip(self.trees, sampled_points)):
-            for ci, (c, e, pts) in enumerate(zip(tree.coeffs, tree.evals, pts_tree)):
-                for pi, p in enumerate(pts):
-                    key = (tuple(p), L - e.log_size)
-                    if key not in doubled:
-                        doubled[key] = tuple(F.qpoint_repeated_double(p, L - e.log_size))
-                    jobs.setdefault(doubled[key], []).append((ti, ci, pi, c))
-        values = {}
-        for q, items in jobs.items():
-            res = self.be.eval_at_points([it[3] for it in items], [list(q)])
-            for it, r in zip(items, res):
-                values[it[:3]] = r[0]
-        sampled_values = []
-        flat_cols, flat_samples = [], []
-        for ti, (tree, pts_tree) in enumerate(zip(self.trees, sampled_points)):
-            vals_tree = []
-            for ci, (e, pts) in enumerate(zip(tree.evals, pts_tree)):
-                vals = [values[(ti, ci, pi)] for pi in range(len(pts))]
-                vals_tree.append(vals)
-                flat_cols.append(e)
-                flat_samples.append([(list(p), v) for p, v in zip(pts, vals)])
-            sampled_values.append(vals_tree)
-        channel.mix_felts([v for t in sampled_values for col in t for v in col])
+        # point doubled down to the column's own domain.  Samples are numbered tree-major, column,
+        # point (the order of the channel mix and of the quotient batches); one batched evaluation
+        # per distinct (point, doubling count) over every column that samples it.
+        counts, flat_pts = [], []
+        for tree, pts_tree in zip(self.trees, sampled_points):
+            assert len(pts_tree) == len(tree.evals), "one point list per committed column"
+            counts.extend(len(p) for p in pts_tree)
+            flat_pts.extend(p for pts in pts_tree for p in pts)
+        counts = np.asarray(counts, np.int64)
+        n_samples = len(flat_pts)
+        coeff_ptrs = np.concatenate([_ptr_array(t.coeffs) for t in self.trees])
+        coeff_logs = np.concatenate([np.asarray(_logs(t.coeffs), np.uint32) for t in self.trees])
+        eval_ptrs = np.concatenate([_ptr_array(t.evals) for t in self.trees])
+        eval_logs = np.concatenate([np.asarray(_logs(t.evals), np.uint32) for t in self.trees])
+        values = np.zeros((n_samples, 4), np.uint32)
+        if n_samples:
+            col_of = np.repeat(np.arange(len(counts)), counts)          # column of each sample
+            ids = np.fromiter((id(p) for p in flat_pts), np.int64, n_samples)
+            _, first, inv = np.unique(ids, return_index=True, return_inverse=True)
+            # points equal in value but given as different objects share one evaluation
+            by_value, remap = {}, np.empty(len(first), np.int64)
+            for k, i in enumerate(first):
+                remap[k] = by_value.setdefault(tuple(int(x) for x in flat_pts[i]), len(by_value))
+            uniq = list(by_value)
+            pt_of = remap[inv]                                            # point of each sample
+            dbl = L - eval_logs[col_of].astype(np.int64)                  # doublings of each sample
+            keys, job_of = np.unique(pt_of * 64 + dbl, return_inverse=True)
+            for j, key in enumerate(keys):
+                u, d = divmod(int(key), 64)
+                sids = np.nonzero(job_of == j)[0]
+                cols = col_of[sids]
+                q = list(F.qpoint_repeated_double(list(uniq[u]), d))
+                res = self.be.eval_at_points(ColumnRefs(ctx, coeff_ptrs[cols], coeff_logs[cols]), [q], as_array=True)
+                values[sids] = res[:, 0, :]
+            points = np.asarray(uniq, dtype=np.uint32).reshape(-1, 8)[pt_of]
+        else:
+            points = np.zeros((0, 8), np.uint32)
+        channel.mix_felts(values)
         random_coeff = [int(x) for x in channel.draw_secure_felt()]
-        quotients = fri_quotients(self.be, self.tw, flat_cols, flat_samples, random_coeff, L, b)
+        offsets = np.zeros(len(counts) + 1, np.uint32)
+        np.cumsum(counts, out=offsets[1:])
+        quotients = fri_quotients_arrays(self.be, self.tw, ColumnRefs(ctx, eval_ptrs, eval_logs), offsets, points,
+                                         values, random_coeff, L, b)
+        # sampled values per tree and column (the proof's nested vectors), and their array blocks
+        vl = values.tolist()
+        sampled_values, blocks, c = [], [], 0
+        for pts_tree in sampled_points:
+            n = len(pts_tree)
+            o = offsets[c:c + n + 1].tolist()
+            sampled_values.append([vl[o[i]:o[i + 1]] for i in range(n)])
+            blocks.append((counts[c:c + n].tolist(), values[o[0]:o[-1]]))
+            c += n
         params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
                            self.fri["fold_step"])
         fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients, as_array=True)
@@ -149,6 +176,7 @@ class CommitmentSchemeProver:
             decommitments.append(wit)
         proof = {"fri_params": dict(self.fri), "commitments": [t.root for t in self.trees],
                  "oods_proof_of_work": oods_proof_of_work, "sampled_values": sampled_values,
+                 "sampled_value_blocks": blocks,
                  "decommitments": decommitments, "queried_values": queried_values, "fri_felts": fri_felts,
                  "query_positions": queries}
         return serialize_commitment_scheme_proof(proof, as_array=True), proof
@@ -178,17 +206,30 @@ def serialize_commitment_scheme_proof(proof, as_array=False):
 
     def hashes(hs):
         head.append(len(hs))
-        if hs:
+        if len(hs):
             flush()
-            parts.append(np.frombuffer(b"".join(hs), dtype="<u4").astype(np.uint64))
+            raw = np.ascontiguousarray(hs, np.uint8).reshape(-1) if isinstance(hs, np.ndarray) else \
+                np.frombuffer(b"".join(hs), np.uint8)
+            parts.append(raw.view("<u4").astype(np.uint64))
 
     f = proof["fri_params"]
     head.extend([f["pow_bits"], f["log_blowup_factor"], f["log_last_layer_degree_bound"], f["n_queries"],
                  f["fold_step"]])
     hashes(proof["commitments"])
     head.extend([proof["oods_proof_of_work"], len(proof["sampled_values"])])
-    for tree in proof["sampled_values"]:
+    blocks = proof.get("sampled_value_blocks")
+    for ti, tree in enumerate(proof["sampled_values"]):
         head.append(len(tree))
+        cnt = blocks[ti][0] if blocks else None
+        if cnt and len(set(cnt)) == 1 and cnt[0]:
+            # every column has the same number of samples: one block [count, 4 * count felts] per column
+            flush()
+            k = cnt[0]
+            block = np.empty((len(cnt), 1 + 4 * k), np.uint64)
+            block[:, 0] = k
+            block[:, 1:] = np.asarray(blocks[ti][1], np.uint64).reshape(len(cnt), 4 * k)
+            parts.append(block.ravel())
+            continue
         for col in tree:
             head.append(len(col))
             for v in col:

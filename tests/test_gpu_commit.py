@@ -88,8 +88,8 @@ def test_pool_reuses_freed_buffers(ctx, backend):
     assert b.ptr == pa
     logs = [9] * 4 + [12] * 2
     host, tree, evals, coeffs, ldes, layers, root = run_commit(ctx, backend, logs, 4)
-    for c in coeffs + ldes:
-        c.free()
+    coeffs.free()  # the commit's outputs: one pooled allocation each
+    ldes.free()
     for _ in range(2):
         _, _, _, r2 = backend.commit_trace(evals, 1, tree)
         assert r2 == root
