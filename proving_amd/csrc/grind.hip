@@ -7,13 +7,26 @@
 // (crates/stwo/src/prover/backend/cpu/grind.rs:5-15) and the SIMD backend's
 // deterministic chunked search (crates/stwo/src/prover/backend/simd/grind.rs:78-122).
 //
-// Device search: rounds of consecutive nonces; each lane hashes 16 nonces (one
-// 40-byte single-block compression each) and the round's smallest hit is kept
-// with atomicMin.  Rounds run in increasing order and stop at the first round
-// with a hit, so the answer is the global minimum.  The first round covers
-// 2^(pow_bits + 5) nonces (a hit with probability 1 - e^-32), later rounds
-// double up to 2^26, so a small grind costs a small launch.
+// Device search (pow_bits <= 32): one launch over 2^(pow_bits + 6) consecutive nonces.
+// Workgroups claim chunks of 4096 nonces in increasing order (an atomic counter); each lane
+// hashes 16 consecutive nonces of the chunk and the smallest hit is kept with atomicMin.  A
+// workgroup stops claiming once the best hit lies below the next chunk, so the search costs
+// about the position of the smallest valid nonce (plus the chunks in flight) instead of whole
+// rounds.  Every chunk below the smallest hit is claimed before any chunk above it and is
+// hashed completely, so the result is the smallest nonce.  Without a hit in the range the
+// next range is searched.
+//
+// Per nonce, one 40-byte single-block compression.  Round 0's column step reads the 8
+// prefixed-digest words and three of its four diagonal G functions read zero words: all of
+// that is the same for every nonce and is computed once on the host; only G(0, 5, 10, 15) of
+// round 0 sees the nonce.  Only digest word 0 is tested (pow_bits <= 32), so the compiler
+// drops what of the last round does not feed v[0] or v[8].
+//
+// pow_bits above 32 (never used by the protocol) take rounds of a plain per-nonce
+// compression with all four tested words.
 #include <string.h>
+
+#include <algorithm>
 
 #include "blake2s.h"
 #include "common.h"
@@ -23,6 +36,89 @@ namespace {
 constexpr uint32_t kPerLane = 16;
 constexpr uint32_t kRoundLog = 26;     // largest round
 constexpr uint32_t kMinRoundLog = 12;  // kPerLane * 256 nonces: one workgroup
+constexpr uint32_t kChunk = kPerLane * 256;
+constexpr uint32_t kMaxGroups = 2048;  // 8 workgroups of 256 lanes per CU
+
+// The compression state after the nonce-independent part of round 0, and the message words
+// 0..7 (the prefixed digest).
+struct GrindPre {
+  uint32_t v[16];
+  uint32_t m[8];
+};
+
+SH_HD void grind_prefix_state(const uint32_t pre[8], GrindPre* g) {
+  uint32_t h[8], v[16];
+  sh::b2_init(h);
+  for (int i = 0; i < 8; i++) v[i] = h[i];
+  v[8] = SH_B2_IV0;
+  v[9] = SH_B2_IV1;
+  v[10] = SH_B2_IV2;
+  v[11] = SH_B2_IV3;
+  v[12] = SH_B2_IV4 ^ 40u;  // t = 40 bytes, one block
+  v[13] = SH_B2_IV5;
+  v[14] = ~SH_B2_IV6;       // final block
+  v[15] = SH_B2_IV7;
+  using sh::rotr32;
+  SH_B2_G(0, 4, 8, 12, pre[0], pre[1]);
+  SH_B2_G(1, 5, 9, 13, pre[2], pre[3]);
+  SH_B2_G(2, 6, 10, 14, pre[4], pre[5]);
+  SH_B2_G(3, 7, 11, 15, pre[6], pre[7]);
+  SH_B2_G(1, 6, 11, 12, 0u, 0u);
+  SH_B2_G(2, 7, 8, 13, 0u, 0u);
+  SH_B2_G(3, 4, 9, 14, 0u, 0u);
+  for (int i = 0; i < 16; i++) g->v[i] = v[i];
+  for (int i = 0; i < 8; i++) g->m[i] = pre[i];
+}
+
+// Digest word 0 of Blake2s(prefixed || LE64(nonce)) from the prefix state.
+__device__ __forceinline__ uint32_t grind_word0(const GrindPre& g, uint32_t lo, uint32_t hi) {
+  using sh::rotr32;
+  uint32_t v[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = g.v[i];
+  const uint32_t m[16] = {g.m[0], g.m[1], g.m[2], g.m[3], g.m[4], g.m[5], g.m[6], g.m[7], lo, hi, 0, 0, 0, 0, 0, 0};
+  SH_B2_G(0, 5, 10, 15, m[8], m[9]);  // the rest of round 0
+  SH_B2_ROUND(14, 10, 4, 8, 9, 15, 13, 6, 1, 12, 0, 2, 11, 7, 5, 3);
+  SH_B2_ROUND(11, 8, 12, 0, 5, 2, 15, 13, 10, 14, 3, 6, 7, 1, 9, 4);
+  SH_B2_ROUND(7, 9, 3, 1, 13, 12, 11, 14, 2, 6, 5, 10, 4, 0, 15, 8);
+  SH_B2_ROUND(9, 0, 5, 7, 2, 4, 10, 15, 14, 1, 11, 12, 6, 8, 3, 13);
+  SH_B2_ROUND(2, 12, 6, 10, 0, 11, 8, 3, 4, 13, 7, 5, 15, 14, 1, 9);
+  SH_B2_ROUND(12, 5, 1, 15, 14, 13, 4, 10, 0, 7, 6, 3, 9, 2, 8, 11);
+  SH_B2_ROUND(13, 11, 7, 14, 12, 1, 3, 9, 5, 0, 15, 4, 8, 6, 2, 10);
+  SH_B2_ROUND(6, 15, 14, 9, 11, 3, 0, 8, 12, 2, 13, 7, 1, 4, 10, 5);
+  SH_B2_ROUND(10, 2, 8, 4, 7, 6, 1, 5, 15, 11, 9, 14, 3, 12, 13, 0);
+  return (SH_B2_IV0 ^ 0x01010020u) ^ v[0] ^ v[8];
+}
+
+// ctl[0..1]: best nonce (u64, all ones when none), ctl[2]: next chunk to claim.
+__global__ __launch_bounds__(256) void k_grind32(GrindPre g, uint32_t mask, uint64_t base, uint32_t n_chunks,
+                                                 unsigned int* ctl) {
+  unsigned long long* best = (unsigned long long*)ctl;
+  unsigned int* next = ctl + 2;
+  __shared__ uint32_t s_chunk;
+  __shared__ int s_stop;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const uint32_t c = atomicAdd(next, 1u);
+      const unsigned long long b = __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_chunk = c;
+      s_stop = c >= n_chunks || b < base + (uint64_t)c * kChunk;
+    }
+    __syncthreads();
+    const uint32_t c = s_chunk;
+    const int stop = s_stop;
+    __syncthreads();  // s_chunk and s_stop are rewritten by the next claim
+    if (stop) return;
+    const uint64_t first = base + (uint64_t)c * kChunk + (uint64_t)threadIdx.x * kPerLane;
+    for (uint32_t k = 0; k < kPerLane; k++) {
+      const uint64_t nonce = first + k;
+      if ((grind_word0(g, (uint32_t)nonce, (uint32_t)(nonce >> 32)) & mask) == 0) {
+        atomicMin(best, (unsigned long long)nonce);
+        break;
+      }
+    }
+  }
+}
 
 __global__ __launch_bounds__(256) void k_grind(const uint32_t* __restrict__ pre_w, uint32_t pow_bits,
                                                uint64_t base, unsigned long long* best) {
@@ -72,8 +168,34 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
   SH_TRY(sh_scratch_get(ctx, SCR_GRIND, 64, &scr));
   uint32_t* d_pre = (uint32_t*)scr;
   unsigned long long* d_best = (unsigned long long*)((uint8_t*)scr + 32);
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_pre, pre, 32, hipMemcpyHostToDevice, ctx->stream));
   const unsigned long long none = ~0ull;
+  if (pow_bits <= 32) {
+    uint32_t pw[8];
+    memcpy(pw, pre, 32);
+    GrindPre g;
+    grind_prefix_state(pw, &g);
+    const uint32_t mask = pow_bits == 32 ? 0xffffffffu : ((1u << pow_bits) - 1);
+    const uint32_t range_log = std::max<uint32_t>(pow_bits + 6, 16);  // 2^range_log nonces per launch
+    const uint64_t n_chunks = (1ull << range_log) / kChunk;
+    const unsigned groups = (unsigned)std::min<uint64_t>(n_chunks, kMaxGroups);
+    unsigned int* d_ctl = (unsigned int*)d_best;
+    SpanGuard span(ctx, "grind");
+    for (uint64_t base = 0;; base += n_chunks * kChunk) {
+      const uint32_t init[4] = {0xffffffffu, 0xffffffffu, 0u, 0u};
+      SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ctl, init, 16, hipMemcpyHostToDevice, ctx->stream));
+      k_grind32<<<groups, 256, 0, ctx->stream>>>(g, mask, base, (uint32_t)n_chunks, d_ctl);
+      SH_TRY(sh_launch_check(ctx, "grind"));
+      unsigned long long best = 0;
+      SH_CHECK_HIP(ctx, hipMemcpyAsync(&best, d_best, 8, hipMemcpyDeviceToHost, ctx->stream));
+      SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (best != none) {
+        *nonce = best;
+        return SH_OK;
+      }
+      if (base > (~0ull) - 2 * n_chunks * kChunk) return sh_fail(ctx, SH_ERR_NOT_FOUND, "no nonce found");
+    }
+  }
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(d_pre, pre, 32, hipMemcpyHostToDevice, ctx->stream));
   uint32_t round_log = pow_bits + 5 < kMinRoundLog ? kMinRoundLog : (pow_bits + 5 > kRoundLog ? kRoundLog : pow_bits + 5);
   SpanGuard span(ctx, "grind");
   for (uint64_t base = 0;;) {

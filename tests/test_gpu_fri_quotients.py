@@ -214,6 +214,19 @@ def test_grind(ctx, backend, golden):
     assert backend.grind(d, 0) == 0
 
 
+@pytest.mark.parametrize("bits", [1, 4, 8, 12, 16])
+def test_grind_smallest_nonce_vs_brute_force(ctx, backend, bits):
+    """The chunked search returns the smallest valid nonce (the oracle's linear search), also
+    when many chunks hold hits at once (small bits) and when the first hits lie past the first
+    chunks (16 bits)."""
+    rng = np.random.default_rng(500 + bits)
+    for _ in range(4):
+        d = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        want = O.grind(d, bits)
+        assert backend.grind(d, bits) == want
+        assert O.verify_pow_nonce(d, bits, want)
+
+
 @pytest.mark.parametrize("n_pts", [1, 3, 4, 6])
 def test_eval_at_points_batched(ctx, backend, n_pts):
     """Many polynomials of mixed sizes at many points in one call == eval_at_point of each."""
