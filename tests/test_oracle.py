@@ -326,6 +326,36 @@ def test_lift_and_accumulate_definition():
         assert list(out[:, i]) == list(exp % P)
 
 
+def test_lift_is_evaluation_at_doubled_points():
+    """What lifting means geometrically (crates/stwo/src/core/vcs_lifted/ and
+    crates/stwo/src/prover/air/accumulation.rs:26-47): a column of the evaluations of p on
+    CanonicCoset(n), lifted to log size L, holds p at the (L - n)-fold doubled points of
+    CanonicCoset(L) (bit-reversed order).  Pins the index map ((i >> (L-n+1)) << 1) + (i & 1) of
+    lift_and_accumulate and of the lifted Merkle leaves against the circle group, not against
+    its own formula (the maps i >> d, (i >> (d+1)) << 1 and a shifted parity bit all fail it)."""
+    from oracle import protocol as PR
+    rng = np.random.default_rng(11)
+    L = 7
+
+    def dbl(x, y):  # the doubling map pi on circle points
+        return (2 * x * x - 1) % P, (2 * x * y) % P
+
+    half = O.canonic_half_initial(L)
+    for n in (2, 3, 5, 7):
+        coeffs = rng.integers(0, P, 1 << n, dtype=np.uint32)
+        tw, _ = O.canonic_twiddles(n)
+        col = O.evaluate(tw, coeffs, n)
+        cols = [np.vstack([col[None, :], np.zeros((3, 1 << n), np.uint32)])]
+        if n < L:
+            cols.append(np.zeros((4, 1 << L), np.uint32))
+        lifted = O.lift_and_accumulate(cols)[0]
+        for i in range(1 << L):
+            x, y = (int(v) for v in O.index_to_point(PR._circle_domain_index(half, L - 1, PR.bitrev(i, L))))
+            for _ in range(L - n):
+                x, y = dbl(x, y)
+            assert int(lifted[i]) == int(O.eval_at_point(coeffs, [x, 0, 0, 0, y, 0, 0, 0])[0]), (n, i)
+
+
 def test_quotients_low_degree_and_match_row_formula():
     """crates/stwo/src/prover/pcs/quotient_ops.rs:188-225: with exact sample values the combined
     quotient is a polynomial of degree < 2^(L - b) (high coefficients vanish).  The pointwise
