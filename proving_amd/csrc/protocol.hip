@@ -11,6 +11,7 @@
 
 #include "../../include/stwo_prover.h"
 #include "blake2s.h"
+#include "blake2s_quad.h"
 #include "common.h"
 
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
@@ -247,6 +248,139 @@ __global__ void k_fri_channel(const uint32_t* root, uint32_t* ch, uint32_t k, ui
   for (uint32_t j = 0; j < k; j++) {
     sh::qstore(a, alphas + 4 * j);
     a = sh::qmul(a, a);
+  }
+}
+
+// ---- the small FRI layers in one launch (fold_step 1)
+// At 2^7 values and below a layer's leaves, tree, channel step and fold are each too small to fill the
+// device, so each launch costs its latency.  One workgroup runs every remaining layer: the leaves
+// (Blake2s of the 4 coordinates, 16 bytes) and the tree with four lanes per hash through LDS, the
+// channel step (mix the root, draw alpha: FriInnerLayerProver::new + draw_secure_felt,
+// crates/stwo/src/prover/fri.rs:175-194) and the fold (fold_line, crates/stwo/src/prover/fri.rs:
+// 41-45) into the next layer's evaluation.  Every layer's evaluation and tree stay in device memory
+// for the decommitment, exactly as the per-layer launches leave them.
+constexpr uint32_t kFriTailMaxLog = 7;
+constexpr uint32_t kFriTailMaxLayers = 12;
+constexpr uint32_t kFriTailThreads = 512;
+
+struct FriTailLayer {
+  const uint32_t* eval;  // 4 coordinate columns of 2^log words, contiguous
+  uint8_t* tree;         // Merkle layer k (2^k hashes) at 32 * (2^k - 1)
+  const uint32_t* itw;   // inverse twiddles of the line domain (one per fold output)
+  uint32_t log, slot;    // log size; index into alphas (16 words each) and roots (8 words each)
+};
+struct FriTailArgs {
+  FriTailLayer layer[kFriTailMaxLayers];
+  uint32_t* next_eval[kFriTailMaxLayers];  // fold output of layer j (the evaluation of layer j + 1)
+  uint32_t n_layers;
+  uint32_t* ch;  // device channel: digest (8 words), n_draws
+  uint32_t* alphas;
+  uint32_t* roots;
+};
+
+__global__ __launch_bounds__(kFriTailThreads) void k_fri_tail(FriTailArgs a) {
+  __shared__ uint32_t hashes[8u << kFriTailMaxLog];  // leaf hashes, then tree levels
+  __shared__ uint32_t work[(kFriTailThreads / 4) * 16];  // leaf messages, then tree levels
+  __shared__ uint32_t msg[16];
+  __shared__ uint32_t digest[8];
+  __shared__ uint32_t draw[8];
+  __shared__ uint32_t alpha_w[4];
+  const uint32_t tid = threadIdx.x, q = tid & 3, quad = tid >> 2;
+  constexpr uint32_t kQuads = kFriTailThreads / 4;
+  uint32_t sidx[40];
+  quad_sigma(q, sidx);
+  uint32_t h_lo, h_hi, d_leaf, d_node, d_draw;
+  quad_init(q, 16, h_lo, h_hi, d_leaf);  // a leaf: the 4 coordinates, 16 bytes
+  quad_init(q, 64, h_lo, h_hi, d_node);  // a node, or digest || root
+  quad_init(q, 37, h_lo, h_hi, d_draw);  // digest || LE32(n_draws) || 0x00
+  if (tid < 8) digest[tid] = a.ch[tid];
+  for (uint32_t j = 0; j < a.n_layers; j++) {
+    const FriTailLayer L = a.layer[j];
+    const uint32_t N = 1u << L.log;
+    // leaves, one per quad: lane q stores coordinate q into the quad's zero-padded message
+    for (uint32_t base = 0; base < N; base += kQuads) {
+      const uint32_t g = base + quad;
+      for (uint32_t w = 4 + q; w < 16; w += 4) work[16 * quad + w] = 0;
+      if (g < N) work[16 * quad + q] = L.eval[(uint64_t)q * N + g];
+      __syncthreads();
+      if (g < N) {
+        uint32_t lo, hi;
+        quad_node_hash(work + 16 * quad, sidx, q, h_lo, h_hi, d_leaf, lo, hi);
+        hashes[8 * g + q] = lo;
+        hashes[8 * g + 4 + q] = hi;
+        uint32_t* o = (uint32_t*)(L.tree + 32ull * (N - 1)) + 8 * g;
+        o[q] = lo;
+        o[4 + q] = hi;
+      }
+      __syncthreads();
+    }
+    // the tree: level l (2^l nodes) from level l + 1
+    uint32_t* cur = hashes;
+    uint32_t* nxt = work;
+    for (int l = (int)L.log - 1; l >= 0; l--) {
+      const uint32_t cnt = 1u << l;
+      for (uint32_t g = quad; g < cnt; g += kQuads) {
+        uint32_t lo, hi;
+        quad_node_hash(cur + 16 * g, sidx, q, h_lo, h_hi, d_node, lo, hi);
+        nxt[8 * g + q] = lo;
+        nxt[8 * g + 4 + q] = hi;
+        uint32_t* o = (uint32_t*)(L.tree + 32ull * (cnt - 1)) + 8 * g;
+        o[q] = lo;
+        o[4 + q] = hi;
+      }
+      __syncthreads();
+      uint32_t* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    // channel: digest = Blake2s(digest || root); draw = Blake2s(digest || LE32(0) || 0x00)
+    if (tid < 16) msg[tid] = tid < 8 ? digest[tid] : cur[tid - 8];
+    if (tid < 8) a.roots[8 * L.slot + tid] = cur[tid];
+    __syncthreads();
+    if (tid < 4) {
+      uint32_t lo, hi;
+      quad_node_hash(msg, sidx, q, h_lo, h_hi, d_node, lo, hi);
+      digest[q] = lo;
+      digest[4 + q] = hi;
+    }
+    __syncthreads();
+    if (tid < 16) msg[tid] = tid < 8 ? digest[tid] : 0;
+    __syncthreads();
+    if (tid < 4) {
+      uint32_t lo, hi;
+      quad_node_hash(msg, sidx, q, h_lo, h_hi, d_draw, lo, hi);
+      draw[q] = lo;
+      draw[4 + q] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t f[8], w[4];
+      for (int i = 0; i < 8; i++) f[i] = sh::reduce_once((draw[i] & sh::P) + (draw[i] >> 31));  // M31::reduce(u32)
+      for (int i = 0; i < 4; i++) w[i] = sh::madd(sh::madd(f[2 * i], f[2 * i + 1]), f[2 * i + 1]);
+      for (int i = 0; i < 4; i++) {
+        alpha_w[i] = w[i];
+        a.alphas[16 * L.slot + i] = w[i];
+      }
+      for (int i = 0; i < 8; i++) a.ch[i] = digest[i];
+      a.ch[8] = 1;
+    }
+    __syncthreads();
+    // fold: out[i] = (v[2i] + v[2i+1]) + alpha * ((v[2i] - v[2i+1]) * x_i^-1)
+    const sh::qm31 alpha = sh::qload(alpha_w);
+    const uint32_t M = N / 2;
+    uint32_t* dst = a.next_eval[j];
+    for (uint32_t i = tid; i < M; i += kFriTailThreads) {
+      sh::qm31 x = {{L.eval[2 * i], L.eval[N + 2 * i]}, {L.eval[2 * N + 2 * i], L.eval[3 * N + 2 * i]}};
+      sh::qm31 y = {{L.eval[2 * i + 1], L.eval[N + 2 * i + 1]}, {L.eval[2 * N + 2 * i + 1], L.eval[3 * N + 2 * i + 1]}};
+      sh::qm31 f0 = sh::qadd(x, y);
+      sh::qm31 f1 = sh::qmulm(sh::qsub(x, y), L.itw[i]);
+      sh::qm31 r = sh::qadd(f0, sh::qmul(alpha, f1));
+      dst[i] = r.a.a;
+      dst[M + i] = r.a.b;
+      dst[2 * M + i] = r.b.a;
+      dst[3 * M + i] = r.b.b;
+    }
+    __syncthreads();  // the next layer reads dst
   }
 }
 
@@ -575,6 +709,42 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   }
   uint32_t n_layers = 1;
   while (line_log > last_log) {
+    if (fs == 1 && line_log <= kFriTailMaxLog && line_log - last_log <= kFriTailMaxLayers) {
+      // every remaining layer in one launch (k_fri_tail)
+      SH_REQUIRE(ctx, sh::is_doubling_of(line_init, line_log, tw->root_initial_index, tw->root_log_size),
+                 "line domain (initial %u, log %u) is not in the twiddle tower", line_init, line_log);
+      FriTailArgs ta = {};
+      ta.n_layers = line_log - last_log;
+      ta.ch = d_ch;
+      ta.alphas = d_alphas;
+      ta.roots = d_roots;
+      for (uint32_t j = 0; j < ta.n_layers; j++) {
+        const uint32_t lg = line_log - j;
+        cur.fold_step = 1;
+        cur.pack = false;
+        DeviceTree& t = cur.tree;
+        t.L = lg;
+        t.owner = ctx;
+        TRY_CLEAN(sh_dev_alloc(ctx, 32ull * ((2ull << lg) - 1), (void**)&t.buf));
+        t.layers.resize(lg + 1);
+        for (uint32_t k = 0; k <= lg; k++) t.layers[k] = t.buf + 32ull * ((1ull << k) - 1);
+        TRY_CLEAN(alloc_eval(tmp, lg - 1));
+        ta.layer[j] = {cur.buf, t.buf, tw->itw + (1ull << tw->root_log_size) - (1ull << lg), lg, n_layers + j};
+        ta.next_eval[j] = tmp.buf;
+        inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
+        cur = FriLayer();
+        std::swap(cur, tmp);
+        line_init = sh::idx_reduce((uint64_t)line_init * 2);
+      }
+      {
+        SpanGuard span(ctx, "fri_tail");
+        k_fri_tail<<<1, kFriTailThreads, 0, ctx->stream>>>(ta);
+        TRY_CLEAN(sh_launch_check(ctx, "fri_tail"));
+      }
+      n_layers += ta.n_layers;
+      line_log = last_log;
+      break;
+    }
     // FriInnerLayerProver::new + mix root + draw alpha + fold_line; the last inner layer folds
     // exactly down to the last layer's domain (fri.rs:175-194)
     const uint32_t k = line_log > last_log + fs ? fs : line_log - last_log;
