@@ -170,14 +170,16 @@ class BaseColumn(DeviceBuffer):
 
 class ColumnView(BaseColumn):
     """A column inside a ColumnSet's allocation.  It borrows the memory: free() only drops the
-    pointer, and the set releases the allocation once no view refers to it."""
+    pointer, and the allocation is released once neither the set nor any view refers to it.
+    A view holds the allocation, not the set, so a set and its views form no reference cycle
+    (which would hold the memory until the cyclic garbage collector runs)."""
 
     def __init__(self, owner, ptr, log_size):
         self.ctx = owner.ctx
         self.nbytes = 4 << log_size
         self.ptr = ptr
         self.log_size = log_size
-        self._owner = owner
+        self._buf = owner.buf
 
     def free(self):
         self.ptr = None
@@ -293,7 +295,7 @@ class HashLayerView(HashLayer):
         self.nbytes = 32 << log_size
         self.ptr = ptr
         self.log_size = log_size
-        self._owner = owner
+        self._buf = owner.buf  # the allocation, not the set: no reference cycle
 
     def free(self):
         self.ptr = None

@@ -177,7 +177,10 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
     const uint32_t mask = pow_bits == 32 ? 0xffffffffu : ((1u << pow_bits) - 1);
     const uint32_t range_log = std::max<uint32_t>(pow_bits + 6, 16);  // 2^range_log nonces per launch
     const uint64_t n_chunks = (1ull << range_log) / kChunk;
-    const unsigned groups = (unsigned)std::min<uint64_t>(n_chunks, kMaxGroups);
+    // about four times the expected position of the first hit in flight at once: a small grind
+    // does not hash a whole device's worth of chunks before the first claims see the hit
+    const uint64_t want = std::max<uint64_t>(64, ((1ull << pow_bits) / kChunk) * 4);
+    const unsigned groups = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_chunks, want), kMaxGroups);
     unsigned int* d_ctl = (unsigned int*)d_best;
     SpanGuard span(ctx, "grind");
     for (uint64_t base = 0;; base += n_chunks * kChunk) {
