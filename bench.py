@@ -323,7 +323,7 @@ def bench_serve(args):
     capacity = n / (time.perf_counter() - t0)
     # the server's capacity: every request queued at time 0
     cap_reqs = make_requests(n, 4)
-    server = ProofServer(21, fri, n_workers=args.serve_workers, hbm_budget=128 << 30)
+    server = ProofServer(21, fri, n_workers=args.serve_workers, hbm_budget=128 << 30, whole_proofs=args.serve_whole)
     try:
         t0 = time.perf_counter()
         server.serve(cap_reqs, realtime=False)
@@ -345,7 +345,8 @@ def bench_serve(args):
         seq = make_requests(n, 4, rate=rate)
         prove_sequential(be, tw, seq, fri, realtime=True)
         srv_reqs = make_requests(n, 4, rate=rate)
-        server = ProofServer(21, fri, n_workers=args.serve_workers, hbm_budget=128 << 30)
+        server = ProofServer(21, fri, n_workers=args.serve_workers, hbm_budget=128 << 30,
+                             whole_proofs=args.serve_whole)
         try:
             server.serve(srv_reqs, realtime=True)
         finally:
@@ -379,6 +380,8 @@ def main():
                          "requests through the continuous-batching server")
     ap.add_argument("--requests", type=int, default=60, help="serve: number of requests")
     ap.add_argument("--serve-workers", type=int, default=2, help="serve: worker contexts of the server")
+    ap.add_argument("--serve-whole", action="store_true",
+                    help="serve: each worker proves whole requests (no batched trace commits)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in pass shapes")
     ap.add_argument("--autotune", action="store_true",

@@ -53,8 +53,15 @@ def device_bytes(n_cols, log_n, log_blowup=1):
 
 
 class ProofServer:
-    def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05):
+    """whole_proofs: each worker generates, commits and proves its requests on its own context
+    (the proofs of different requests overlap on the device, with no handoff between contexts);
+    otherwise the scheduler commits the traces of each admitted batch together on its context
+    and the workers finish the proofs."""
+
+    def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05,
+                 whole_proofs=False):
         self.fri = dict(fri)
+        self.whole_proofs = whole_proofs
         self.ctx = Context(0)
         self.be = HipBackend(self.ctx)
         self.be.load_fft_tuning()  # the committed pass-shape table, when there is one
@@ -78,24 +85,38 @@ class ProofServer:
         ctx = Context(0)
         be = HipBackend(ctx)
         be.load_fft_tuning()
+        tw = be.precompute_canonic_twiddles(self.tw.root_log_size + 1) if self.whole_proofs else None
         while True:
             item = self._work.get()
             if item is None:
+                if tw is not None:
+                    tw.free()
                 ctx.close()
                 return
             req, scheme, channel, cost = item
             try:
-                scheme.be = be                  # continue the proof on this worker's context
-                req.felts, _ = finish_wide_fibonacci(scheme, channel, req.n_cols, req.log_n)
+                if scheme is None:              # the whole proof on this worker's context
+                    c0, c1 = BaseColumn.from_cpu(ctx, req.c0), BaseColumn.from_cpu(ctx, req.c1)
+                    trace = be.wide_fibonacci_trace(c0, c1, req.n_cols)
+                    req.felts, _, _ = prove_wide_fibonacci(be, tw, trace, req.log_n, self.fri)
+                else:
+                    scheme.be = be              # continue the proof on this worker's context
+                    req.felts, _ = finish_wide_fibonacci(scheme, channel, req.n_cols, req.log_n)
                 ctx.sync()
             except Exception as e:  # surfaced by serve()
                 self._errors.append((req.index, repr(e)))
             finally:
-                scheme.trees = None             # release this request's buffers (back to the pools)
+                if scheme is not None:
+                    scheme.trees = None         # release this request's buffers (back to the pools)
                 req.done = time.perf_counter() - self._t0
                 with self._lock:
                     self._reserved -= cost
                     self._in_flight -= 1
+
+    def _check_workers(self):
+        """A worker thread that died would leave its requests in flight forever."""
+        if not all(w.is_alive() for w in self._workers):
+            raise RuntimeError(f"a worker thread stopped; errors so far: {self._errors[:3]}")
 
     def close(self):
         for _ in self._workers:
@@ -148,14 +169,19 @@ class ProofServer:
                     self._in_flight += 1
                     r.admitted = now
                     batch.append(waiting.pop(0))
-            if batch:
+            if batch and self.whole_proofs:
+                for r in batch:
+                    self._work.put((r, None, None, device_bytes(r.n_cols, r.log_n, b)))
+            elif batch:
                 self._admit(batch)
             else:
+                self._check_workers()
                 time.sleep(0.0005)
         while True:
             with self._lock:
                 if self._in_flight == 0:
                     break
+            self._check_workers()
             time.sleep(0.0005)
         if self._errors:
             raise RuntimeError(f"requests failed: {self._errors[:3]}")

@@ -13,11 +13,12 @@ pytestmark = pytest.mark.gpu
 FRI = {"pow_bits": 6, "log_blowup_factor": 1, "log_last_layer_degree_bound": 1, "n_queries": 8, "fold_step": 1}
 
 
-def test_each_proof_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("whole_proofs", [False, True], ids=["batched_commits", "whole_proofs"])
+def test_each_proof_is_the_same_alone_and_in_the_mix(whole_proofs):
     reqs = make_requests(7, 40, col_choices=(4, 16, 32), log_range=(6, 11))
     # a budget that fits about two requests forces admissions to wait for completions
     budget = 2 * max(device_bytes(r.n_cols, r.log_n) for r in reqs)
-    server = ProofServer(12, FRI, n_workers=2, hbm_budget=budget)
+    server = ProofServer(12, FRI, n_workers=2, hbm_budget=budget, whole_proofs=whole_proofs)
     try:
         mixed = server.serve(reqs, realtime=False)
     finally:
