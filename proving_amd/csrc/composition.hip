@@ -33,7 +33,10 @@ __global__ __launch_bounds__(kThreads) void k_wide_fib_composition(const uint32_
                                                                    uint32_t* const* __restrict__ acc) {
   const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= (1u << eval_log)) return;
-  uint32_t a = cols[0][r], b = cols[1][r];
+  // the squares of the two previous columns: each column is squared once, for the next two
+  // constraints
+  const uint32_t a = cols[0][r], b = cols[1][r];
+  uint32_t sa = sh::mmul(a, a), sb = sh::mmul(b, b);
   uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   const uint32_t m = n_cols - 2;
   uint32_t j = 0;
@@ -42,14 +45,14 @@ __global__ __launch_bounds__(kThreads) void k_wide_fib_composition(const uint32_
 #pragma unroll 3
     for (; j < end; j++) {
       const uint32_t c = cols[j + 2][r];
-      const uint32_t k = sh::msub(c, sh::madd(sh::mmul(a, a), sh::mmul(b, b)));
+      const uint32_t k = sh::msub(c, sh::madd(sa, sb));
       const uint4 wj = *(const uint4*)&w[4 * j];
       s0 += (uint64_t)k * wj.x;
       s1 += (uint64_t)k * wj.y;
       s2 += (uint64_t)k * wj.z;
       s3 += (uint64_t)k * wj.w;
-      a = b;
-      b = c;
+      sa = sb;
+      sb = sh::mmul(c, c);
     }
     // three products of < 2^62 each on top of a folded value (< 2^34) stay below 2^64
     s0 = fold64(s0);
@@ -75,14 +78,15 @@ __global__ __launch_bounds__(kThreads) void k_wide_fib_trace(const uint32_t* __r
                                                              uint32_t n_cols, uint32_t* const* __restrict__ out) {
   const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
   if (r >= n_rows) return;
-  uint32_t a = c0[r], b = c1[r];
+  const uint32_t a = c0[r], b = c1[r];
   out[0][r] = a;
   out[1][r] = b;
+  uint32_t sa = sh::mmul(a, a), sb = sh::mmul(b, b);  // each column squared once
   for (uint32_t k = 2; k < n_cols; k++) {
-    const uint32_t c = sh::madd(sh::mmul(a, a), sh::mmul(b, b));
+    const uint32_t c = sh::madd(sa, sb);
     out[k][r] = c;
-    a = b;
-    b = c;
+    sa = sb;
+    sb = sh::mmul(c, c);
   }
 }
 
