@@ -39,26 +39,38 @@ __global__ __launch_bounds__(kThreads) void k_wide_fib_composition(const uint32_
   uint32_t sa = sh::mmul(a, a), sb = sh::mmul(b, b);
   uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   const uint32_t m = n_cols - 2;
-  uint32_t j = 0;
-  while (j < m) {
-    const uint32_t end = j + 3 < m ? j + 3 : m;
-#pragma unroll 3
-    for (; j < end; j++) {
-      const uint32_t c = cols[j + 2][r];
-      const uint32_t k = sh::msub(c, sh::madd(sa, sb));
-      const uint4 wj = *(const uint4*)&w[4 * j];
-      s0 += (uint64_t)k * wj.x;
-      s1 += (uint64_t)k * wj.y;
-      s2 += (uint64_t)k * wj.z;
-      s3 += (uint64_t)k * wj.w;
-      sa = sb;
-      sb = sh::mmul(c, c);
-    }
-    // three products of < 2^62 each on top of a folded value (< 2^34) stay below 2^64
+  auto step = [&](uint32_t c, uint32_t jj) {
+    const uint32_t k = sh::msub(c, sh::madd(sa, sb));
+    const uint4 wj = *(const uint4*)&w[4 * jj];
+    s0 += (uint64_t)k * wj.x;
+    s1 += (uint64_t)k * wj.y;
+    s2 += (uint64_t)k * wj.z;
+    s3 += (uint64_t)k * wj.w;
+    sa = sb;
+    sb = sh::mmul(c, c);
+  };
+  // three products of < 2^62 each on top of a folded value (< 2^34) stay below 2^64
+  auto fold_all = [&]() {
     s0 = fold64(s0);
     s1 = fold64(s1);
     s2 = fold64(s2);
     s3 = fold64(s3);
+  };
+  uint32_t j = 0;
+  // six columns per step, their loads issued together
+  for (; j + 6 <= m; j += 6) {
+    uint32_t c[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) c[u] = cols[j + 2 + u][r];
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+      step(c[u], j + u);
+      if (u == 2 || u == 5) fold_all();
+    }
+  }
+  for (; j < m; j++) {
+    step(cols[j + 2][r], j);
+    fold_all();
   }
   const uint32_t d = denom_inv[r >> trace_log];
   auto finish = [&](uint64_t s, uint32_t* out) {
