@@ -62,21 +62,35 @@ __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __rest
   for (int j = 0; j < NB; j++)
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[j][c] = 0;
-  uint32_t k = 0;
-  for (uint32_t col = 0; col < n_cols; col++) {
-    const uint32_t v = cols[col][r];
+  auto mac = [&](uint32_t v, uint32_t col) {
     const uint32_t* cf = coef + (size_t)col * NB * 4;
 #pragma unroll
     for (int j = 0; j < NB; j++)
 #pragma unroll
       for (int c = 0; c < 4; c++) acc[j][c] += (uint64_t)cf[j * 4 + c] * v;
-    if (++k == 3) {
-      k = 0;
+  };
+  // three products of < 2^62 on a folded value stay below 2^64
+  auto fold_all = [&]() {
 #pragma unroll
-      for (int j = 0; j < NB; j++)
+    for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
+      for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
+  };
+  uint32_t col = 0;
+  // six columns per step, their loads issued together
+  for (; col + 6 <= n_cols; col += 6) {
+    uint32_t v[6];
+#pragma unroll
+    for (int u = 0; u < 6; u++) v[u] = cols[col + u][r];
+#pragma unroll
+    for (int u = 0; u < 6; u++) {
+      mac(v[u], col + u);
+      if (u == 2 || u == 5) fold_all();
     }
+  }
+  for (; col < n_cols; col++) {
+    mac(cols[col][r], col);
+    fold_all();
   }
   sh::qm31 S[NB];
 #pragma unroll
