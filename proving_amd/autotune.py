@@ -26,10 +26,18 @@ def plan(top, shape):
     """The pass plan sh_fft_* derive from a shape for a transform of `top` layers: the block pass
     and the strided passes (lo, hi, tile log), as in plan_passes (fft.hip)."""
     block_log, stride_layers, tile_log, _ = shape
+    rem = 0
+    if top > block_log:
+        rem = (top - block_log) % stride_layers
+        if rem and rem <= 2 and block_log + rem <= 13:
+            block_log += rem  # a remainder of one or two layers joins the block pass
+            rem = 0
     bh = min(top, block_log)
     passes, lo = [(0, bh, 0)], bh
     while lo < top:
         g = min(top - lo, stride_layers)
+        if rem and rem <= 2 and stride_layers < top - lo < 2 * stride_layers:
+            g = (top - lo + 1) // 2  # otherwise the last two strided passes share the layers
         passes.append((lo, lo + g, min(lo, tile_log)))
         lo += g
     return tuple(passes)

@@ -37,11 +37,27 @@ struct FftPass {
 // of 2^tile_log groups.
 std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
   std::vector<FftPass> v;
-  uint32_t bh = top < t.block_log ? top : t.block_log;
+  // One table entry serves both directions of a size: an LDE plans its 2^(m-1) computed layers,
+  // an interpolation all m, so an entry tuned for one can leave the other a last strided pass of
+  // one or two layers (a whole pass over the array whose workgroups are mostly idle).  Such a
+  // remainder goes into the block pass while the block stays within 2^13 words.
+  // Otherwise, with two or more strided passes, the last two share their layers evenly.
+  uint32_t block = t.block_log;
+  uint32_t rem = 0;
+  if (top > block) {
+    rem = (top - block) % t.stride_layers;
+    if (rem != 0 && rem <= 2 && block + rem <= 13) {
+      block += rem;
+      rem = 0;
+    }
+  }
+  uint32_t bh = top < block ? top : block;
   v.push_back({0, bh, 0});
   uint32_t l = bh;
   while (l < top) {
     uint32_t g = top - l < t.stride_layers ? top - l : t.stride_layers;
+    if (rem != 0 && rem <= 2 && top - l > t.stride_layers && top - l < 2 * t.stride_layers)
+      g = (top - l + 1) / 2;  // the last two passes: ceil and floor of their layers
     uint32_t tl = l < t.tile_log ? l : t.tile_log;
     v.push_back({l, l + g, tl});
     l += g;
