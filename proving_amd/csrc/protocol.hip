@@ -112,8 +112,11 @@ std::vector<std::pair<uint32_t, uint32_t>> merkle_witness_nodes(uint32_t L, std:
   std::sort(queries.begin(), queries.end());
   queries.erase(std::unique(queries.begin(), queries.end()), queries.end());
   std::vector<std::pair<uint32_t, uint32_t>> need;
+  need.reserve(queries.size() * L);
+  std::vector<uint32_t> next;
+  next.reserve(queries.size());
   for (int layer = (int)L - 1; layer >= 0; layer--) {
-    std::vector<uint32_t> next;
+    next.clear();
     for (size_t i = 0; i < queries.size();) {
       uint32_t first = queries[i];
       size_t len = (i + 1 < queries.size() && (first ^ 1u) == queries[i + 1]) ? 2 : 1;
@@ -121,7 +124,7 @@ std::vector<std::pair<uint32_t, uint32_t>> merkle_witness_nodes(uint32_t L, std:
       next.push_back(first >> 1);
       i += len;
     }
-    queries = next;
+    queries.swap(next);
   }
   return need;
 }
@@ -828,6 +831,9 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   std::vector<const uint32_t*> gc;
   std::vector<uint32_t> gr;
   std::vector<const uint8_t*> gn;
+  gc.reserve(4ull * q.size() * layers_all.size() << fs);
+  gr.reserve(4ull * q.size() * layers_all.size() << fs);
+  gn.reserve(32ull * q.size() * layers_all.size());
   std::vector<uint32_t> lq = q;
   for (size_t li = 0; li < layers_all.size(); li++) {
     const FriLayer& L = *layers_all[li];
