@@ -66,6 +66,38 @@ def test_evaluate_matches_cpu(ctx, backend, n, m):
     tree.free()
 
 
+# pass shapes whose strided layers leave a remainder of one or two layers: the planner folds it into
+# the block pass (block + remainder <= 13) or splits the last two strided passes
+@pytest.mark.parametrize("kind,n,m,shape", [
+    ("interpolate", 15, 15, (11, 3, 3, 3)),   # 4 strided layers: remainder 1 -> block 12
+    ("interpolate", 17, 17, (11, 4, 4, 3)),   # remainder 2 -> block 13
+    ("interpolate", 18, 18, (13, 4, 4, 4)),   # remainder 1, block full -> passes of 3 and 2
+    ("interpolate", 16, 16, (13, 2, 3, 3)),   # remainder 1 with 2-layer passes
+    ("evaluate", 16, 17, (12, 3, 3, 3)),      # the LDE plans its 16 computed layers
+    ("evaluate", 18, 19, (13, 4, 5, 4)),
+])
+def test_pass_plans_with_remainders(ctx, backend, kind, n, m, shape):
+    rng = np.random.default_rng(7 * n + m)
+    saved = backend.fft_tuning(m)
+    backend.set_fft_tuning(m, *shape)
+    try:
+        tree = backend.precompute_canonic_twiddles(m)
+        tw, itw = O.canonic_twiddles(m)
+        vals = rand_col(rng, n)
+        col = BaseColumn.from_cpu(ctx, vals)
+        if kind == "interpolate":
+            backend.interpolate(col, tree)
+            np.testing.assert_array_equal(col.to_cpu(), O.interpolate(itw, vals, n))
+        else:
+            out = backend.evaluate(col, m, tree)
+            np.testing.assert_array_equal(out.to_cpu(), O.evaluate(tw, vals, m))
+            out.free()
+        col.free()
+        tree.free()
+    finally:
+        backend.set_fft_tuning(m, *saved)
+
+
 def test_round_trip_closed_form(ctx, backend):
     """Interpolating values 0..2^n (natural order) gives a polynomial with f(domain.at(i)) = i:
     evaluate(interpolate(x)) == x, and the coefficients match the oracle."""
