@@ -260,6 +260,23 @@ def fold_circle_into_line(src, log_size, alpha, half_initial=None):
     return dst
 
 
+def fri_decompose(src):
+    """FriOps::decompose (crates/stwo/src/prover/backend/cpu/fri.rs:39-62) with
+    decomposition_coefficient (:134-146) for a QM31 evaluation in bit-reversed order, blowup 2:
+    lambda = (sum of the first half - sum of the second half) / N, coordinate by coordinate, and
+    g = f - lambda on the first half, f + lambda on the second.  Returns (g as (4, N), lambda)."""
+    c = np.asarray(src, dtype=np.int64)
+    n = c.shape[1]
+    h = n // 2
+    inv_n = m31_inv(n)
+    lam = [int((c[k, :h].sum() - c[k, h:].sum()) % P) * inv_n % P for k in range(4)]
+    g = c.copy()
+    col = np.array(lam, np.int64)[:, None]
+    g[:, :h] -= col
+    g[:, h:] += col
+    return (g % P).astype(np.uint32), lam
+
+
 # ---- quotients / accumulation ---------------------------------------------------
 def line_coeffs(point, value, alpha):
     out = np.zeros(12, np.uint32)

@@ -73,21 +73,14 @@ def test_eval_by_folding_on_gpu(ctx, backend, golden):
     assert got == list(backend.eval_at_point(coeffs, pt))
 
 
-def test_decompose(ctx, backend):
-    rng = np.random.default_rng(2)
-    log = 9
+@pytest.mark.parametrize("log", [5, 9, 16])
+def test_decompose(ctx, backend, log):
+    rng = np.random.default_rng(2 + log)
     src = rnd(rng, (4, 1 << log))
     g, lam = backend.decompose(SecureColumnByCoords.from_cpu(ctx, src))
-    # lambda = (sum of first half - sum of second half) / N (cpu/fri.rs:118-146)
-    a = src[:, : 1 << (log - 1)].astype(np.int64).sum(axis=1) % P
-    b = src[:, 1 << (log - 1):].astype(np.int64).sum(axis=1) % P
-    inv = O.m31_inv(1 << log)
-    exp_lam = [int((a[c] - b[c]) % P) * inv % P for c in range(4)]
+    exp_g, exp_lam = O.fri_decompose(src)
     assert list(lam) == exp_lam
-    exp = src.astype(np.int64)
-    exp[:, : 1 << (log - 1)] -= np.array(exp_lam, np.int64)[:, None]
-    exp[:, 1 << (log - 1):] += np.array(exp_lam, np.int64)[:, None]
-    np.testing.assert_array_equal(g.to_cpu(), (exp % P).astype(np.uint32))
+    np.testing.assert_array_equal(g.to_cpu(), exp_g)
 
 
 def _samples(rng, polys, pts, alpha, periodic_first=False):

@@ -460,3 +460,37 @@ def test_split_at_mid_identity(L):
     assert lhs == [(a + b) % P for a, b in zip(lv, fr)]
     # a dropped factor fails: p(z) != p_left(z) + p_right(z) in general
     assert lhs != [(a + b) % P for a, b in zip(lv, rv)]
+
+
+@pytest.mark.parametrize("log", [5, 6, 8, 11])
+def test_decompose_splits_off_the_component_outside_the_fft_space(log):
+    """FriOps::decompose pinned by what it is for (the reference's
+    decompose_coeff_out_fft_space_test, crates/stwo/src/prover/backend/cpu/fri.rs:163-210): the
+    circle basis element at index 2^(log-1) lies outside the FFT space of log - 1 and takes the
+    value +c on the first half of the bit-reversed CanonicCoset(log) and -c on the second, so
+    p + kappa * that element (p in the FFT space) decomposes into exactly the evaluations of p
+    and lambda = kappa * c.  A wrong sign or divisor in lambda breaks both equalities."""
+    rng = np.random.default_rng(log)
+    n, h = 1 << log, 1 << (log - 1)
+    tw, itw = O.canonic_twiddles(log)
+    e = np.zeros(n, np.uint32)
+    e[h] = 1
+    ev = O.evaluate(tw, e, log)
+    c = int(ev[0])
+    assert c != 0 and np.all(ev[:h] == c) and np.all(ev[h:] == P - c)
+    p = rng.integers(0, P, (4, h), dtype=np.uint32)
+    p[0] = 0  # the reference's case: only the out-of-space coefficient
+    kappa = [int(x) for x in rng.integers(1, P, 4)]
+    f = np.empty((4, n), np.uint32)
+    fft_part = np.empty((4, n), np.uint32)
+    for k in range(4):
+        coeffs = np.zeros(n, np.uint32)
+        coeffs[:h] = p[k]
+        fft_part[k] = O.evaluate(tw, coeffs, log)
+        coeffs[h] = kappa[k]
+        f[k] = O.evaluate(tw, coeffs, log)
+    g, lam = O.fri_decompose(f)
+    assert lam == [kappa[k] * c % P for k in range(4)]
+    np.testing.assert_array_equal(g, fft_part)
+    for k in range(4):
+        assert not O.interpolate(itw, g[k], log)[h:].any()
