@@ -360,10 +360,21 @@ extern "C" int sh_download(sh_ctx* ctx, void* h_dst, const void* d_src, uint64_t
 }
 
 // Fill kernel (not hipMemsetAsync, so the call stays safe inside a captured graph).
+// Byte fill: 16-byte stores over the 16-byte-aligned middle, single bytes for the unaligned head
+// and the tail (each under 16 bytes, so the first 16 lanes of the grid cover them).
 __global__ void k_fill_u8(uint8_t* p, uint8_t v, uint64_t n) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = v;
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t lead = (16 - ((uintptr_t)p & 15)) & 15;
+  const uint64_t head = lead < n ? lead : n;
+  const uint64_t nq = (n - head) >> 4;
+  const uint32_t w = v * 0x01010101u;
+  const uint4 x = make_uint4(w, w, w, w);
+  uint4* q = (uint4*)(p + head);
+  for (uint64_t i = tid; i < nq; i += stride) q[i] = x;
+  const uint64_t done = head + (nq << 4);
+  if (tid < head) p[tid] = v;
+  if (tid < n - done) p[done + tid] = v;
 }
 
 extern "C" int sh_copy(sh_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes) {
@@ -376,7 +387,8 @@ extern "C" int sh_copy(sh_ctx* ctx, void* d_dst, const void* d_src, uint64_t byt
 extern "C" int sh_memset(sh_ctx* ctx, void* d_dst, int value, uint64_t bytes) {
   if (!ctx) return SH_ERR_INVALID;
   if (bytes == 0) return SH_OK;
-  uint64_t blocks = (bytes + 255) / 256;
+  uint64_t blocks = (bytes / 16 + 255) / 256;
+  if (blocks < 1) blocks = 1;
   if (blocks > 65536) blocks = 65536;
   k_fill_u8<<<(unsigned)blocks, 256, 0, ctx->stream>>>((uint8_t*)d_dst, (uint8_t)value, bytes);
   return sh_launch_check(ctx, "fill");

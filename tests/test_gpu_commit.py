@@ -5,6 +5,8 @@ benchmark configurations, against the oracle, bit-exact.
 * 512 columns of mixed sizes (256 @ 2^12, 128 @ 2^16, 96 @ 2^18, 32 @ 2^20), seed 1: root of the
   lifted tree and of the mixed-degree tree (commit_on_layer), plus spot columns.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -96,3 +98,18 @@ def test_pool_reuses_freed_buffers(ctx, backend):
     ctx.check(ctx.lib.sh_pool_trim(ctx.h), "pool_trim")
     _, _, _, r3 = backend.commit_trace(evals, 1, tree)
     assert r3 == root
+
+
+@pytest.mark.parametrize("off,n,val", [(0, 1 << 14, 0), (1, 1, 7), (3, 29, 0xAB), (5, 4096 + 11, 0xFF),
+                                       (16, 16, 1), (12, 1 << 20, 0x5A)])
+def test_memset_bytes(ctx, off, n, val):
+    """sh_memset (the zero-fill of accumulators and counters): exactly bytes [off, off + n) take the
+    value, at any alignment and length (16-byte stores over the aligned middle, bytes at the ends)."""
+    rng = np.random.default_rng(off + n)
+    host = rng.integers(0, 1 << 31, 1 << 19, dtype=np.uint32)
+    col = BaseColumn.from_cpu(ctx, host)
+    ctx.check(ctx.lib.sh_memset(ctx.h, ctypes.c_void_p(col.ptr + off), val, n), "sh_memset")
+    exp = host.view(np.uint8).copy()
+    exp[off:off + n] = val
+    np.testing.assert_array_equal(col.to_cpu().view(np.uint8), exp)
+    col.free()
