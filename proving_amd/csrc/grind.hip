@@ -182,10 +182,14 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
     const uint64_t want = std::max<uint64_t>(64, ((1ull << pow_bits) / kChunk) * 4);
     const unsigned groups = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_chunks, want), kMaxGroups);
     unsigned int* d_ctl = (unsigned int*)d_best;
+    // the control words' initial value comes from the staging arena (staged once, then a device
+    // copy per launch), not from pageable host memory
+    const uint32_t init[4] = {0xffffffffu, 0xffffffffu, 0u, 0u};
+    void* d_init;
+    SH_TRY(sh_stage(ctx, init, sizeof(init), &d_init));
     SpanGuard span(ctx, "grind");
     for (uint64_t base = 0;; base += n_chunks * kChunk) {
-      const uint32_t init[4] = {0xffffffffu, 0xffffffffu, 0u, 0u};
-      SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ctl, init, 16, hipMemcpyHostToDevice, ctx->stream));
+      SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ctl, d_init, 16, hipMemcpyDeviceToDevice, ctx->stream));
       k_grind32<<<groups, 256, 0, ctx->stream>>>(g, mask, base, (uint32_t)n_chunks, d_ctl);
       SH_TRY(sh_launch_check(ctx, "grind"));
       unsigned long long best = 0;
