@@ -66,8 +66,11 @@ typedef struct {
  * evaluation on CanonicCoset(log_size) (4 device coordinate arrays).  Advances the channel.
  * Writes the FriProof in CairoSerde felt encoding (crates/cairo-serialize/src/serialize.rs:66-90)
  * to felts (each felt < 2^64) and the sorted, deduplicated query positions to
- * query_positions (at most n_queries).  fold_step must be 1 (the production setting; packed
- * leaves are not implemented). */
+ * query_positions (at most n_queries).  fold_step 1-4 is supported: with fold_step > 1 a layer
+ * of at least 4 values is committed with packed leaves (crates/stwo/src/prover/fri.rs:295-297),
+ * each leaf holding 4 consecutive QM31 values (LOG_PACKED_LEAF_SIZE = 2,
+ * crates/stwo/src/core/vcs_lifted/verifier.rs:16), and each layer is folded fold_step times
+ * (the last fold may be shorter). */
 int sh_fri_prove(sh_ctx *ctx, const sh_twiddles *tw, sh_channel *ch, const sh_fri_params *params,
                  const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
                  uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);

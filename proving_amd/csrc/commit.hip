@@ -24,6 +24,11 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
                          uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready);
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers);
 
+static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
+                         uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
+                         uint32_t* const* d_ldes, uint8_t* const* d_layers, const std::vector<uint32_t>& classes,
+                         uint32_t L);
+
 extern "C" int sh_commit_trace(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
                                uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n,
                                uint32_t log_blowup, uint32_t* const* d_ldes, uint8_t* const* d_layers,
@@ -39,10 +44,32 @@ extern "C" int sh_commit_trace(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
   const uint32_t L = classes.back() + log_blowup;
 
-  // fork: every auxiliary stream starts after the work already on the main stream
+  // fork: the auxiliary stream starts after the work already on the main stream
   hipEvent_t fork = sh_sync_event(ctx, 0);
   SH_CHECK_HIP(ctx, hipEventRecord(fork, ctx->main_stream));
-  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamWaitEvent(a, fork, 0));
+  SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->aux[0], fork, 0));
+  const int rc = commit_forked(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, classes, L);
+  if (rc != SH_OK) {
+    // join: buffers freed after this call may be reused at once, so nothing enqueued on the
+    // auxiliary stream may still run unordered after the main stream's work
+    hipEvent_t join = sh_sync_event(ctx, 1 + classes.size());
+    if (hipEventRecord(join, ctx->aux[0]) == hipSuccess) (void)hipStreamWaitEvent(ctx->main_stream, join, 0);
+    return rc;
+  }
+  if (root) {
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
+    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return SH_OK;
+}
+
+// The part of sh_commit_trace after the fork: FFT classes on aux[0], leaves and layers on the main
+// stream.  Every leaf group waits for its class's event, so on success the last group has joined
+// the auxiliary stream back into the main one.
+static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
+                         uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
+                         uint32_t* const* d_ldes, uint8_t* const* d_layers, const std::vector<uint32_t>& classes,
+                         uint32_t L) {
   std::vector<hipEvent_t> ready(classes.size());
   for (size_t k = 0; k < classes.size(); k++) {
     const uint32_t m = classes[k];
@@ -74,10 +101,5 @@ extern "C" int sh_commit_trace(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
     logs[i] = log_sizes[order[i]] + log_blowup;
   }
   SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), n, L, (uint32_t*)d_layers[L], ready.data()));
-  SH_TRY(sh_build_layers_impl(ctx, L, (uint32_t* const*)d_layers));
-  if (root) {
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
-    SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  return SH_OK;
+  return sh_build_layers_impl(ctx, L, (uint32_t* const*)d_layers);
 }

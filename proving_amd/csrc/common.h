@@ -27,7 +27,10 @@ struct sh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;     // the stream every entry point orders its work on
   hipStream_t main_stream = nullptr;
-  hipStream_t aux[3] = {nullptr, nullptr, nullptr};  // fork/join helpers inside one call
+  // fork/join helper inside one call (the batched commit runs its FFT chain on it); a context
+  // holds only the streams it uses, so several contexts in one process stay within the device's
+  // few hardware queues
+  hipStream_t aux[1] = {nullptr};
   std::string last_error;
   // pinned host arena + device mirror for per-call tables; identical tables are
   // reused (content-addressed), so steady-state calls issue no copies

@@ -63,8 +63,15 @@ extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, 
     return sh_fail(ctx, SH_ERR_OOM, "graph arena allocation failed");
   }
   // the context's streams must be idle before its state is swapped
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
-  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
+  {
+    hipError_t e = hipStreamSynchronize(ctx->main_stream);
+    for (auto a : ctx->aux)
+      if (e == hipSuccess) e = hipStreamSynchronize(a);
+    if (e != hipSuccess) {
+      release(g);
+      return sh_fail(ctx, SH_ERR_HIP, "stream synchronisation before capture failed: %s", hipGetErrorString(e));
+    }
+  }
   const bool prof = ctx->profile;
   ctx->profile = false;
   swap_state(ctx, g);

@@ -109,39 +109,132 @@ __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __rest
   }
 }
 
-// Per-accumulation data for the combine kernel.
-struct AccSpec {
-  uint32_t n_acc;
-  uint32_t sub_log;
+// Per-accumulation constants of the combine kernel (host-computed, 16 words each):
+//   den_k(p) = (Re px_k - p.x) * Im py_k - (Re py_k - p.y) * Im px_k
+//            = c_k - p.x * Im py_k + p.y * Im px_k,   c_k = Re px_k * Im py_k - Re py_k * Im px_k
+// (crates/stwo/src/core/pcs/quotients.rs:233-256), all CM31; and first_linear_k (f0 + f1 i,
+// f2 + f3 i) as -f and the two imaginary parts, so that every product below is an addition.
+struct CombineK {
+  uint32_t c[2];      // c_k
+  uint32_t npiy[2];   // -Im py_k
+  uint32_t pix[2];    // Im px_k
+  uint32_t nfl[4];    // -first_linear_k
+  uint32_t flim[2];   // f1, f3
+  uint32_t plog;      // log size of partial numerator k
+  uint32_t pad[3];
 };
+static_assert(sizeof(CombineK) == 64, "CombineK layout");
 
+
+// (u64) a * b + c reduced to canonical, for a, b, c < P: the sum is at most P^2 - P, whose two
+// 31-bit halves add to at most 2P - 2.
+__device__ __forceinline__ uint32_t mad_red(uint32_t a, uint32_t b, uint32_t c) {
+  const uint64_t v = (uint64_t)a * b + c;
+  return sh::reduce_once(((uint32_t)v & sh::P) + (uint32_t)(v >> 31));
+}
+// v += a * b with v kept below 2^64: at most four products of two canonical words between folds.
+__device__ __forceinline__ void mac64(uint64_t& v, uint32_t a, uint32_t b) { v += (uint64_t)a * b; }
+
+// compute_quotients_and_combine's row loop (crates/stwo/src/prover/backend/cpu/quotients.rs:
+// 51-101): q[row] = sum_k (partial_k[lift(row)] - first_linear_k * p.y) / den_k(p).  Each lane
+// takes RB rows (stride 256, so every load and store of a wave is coalesced) and inverts the
+// denominators of its rows for NK accumulations at once: den^-1 = conj(den) / |den|^2 with the
+// M31 norms batch-inverted (Montgomery's trick; the SIMD backend batch-inverts its denominators
+// too, crates/stwo/src/prover/backend/simd/quotients.rs:255-283): one exponentiation per lane per
+// NK accumulations instead of one per row and accumulation.  Products are summed lazily in 64 bits.
+template <int NK, int RB>
 __global__ __launch_bounds__(256) void k_combine(const uint32_t* const* __restrict__ parts,
-                                                 const uint32_t* __restrict__ acc_data, AccSpec spec,
+                                                 const CombineK* __restrict__ kc, uint32_t n_acc, uint32_t sub_log,
                                                  sh_point_tab half, uint32_t* const* __restrict__ out) {
-  const uint32_t rows = 1u << spec.sub_log;
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  sh::mpt dp = sh_circle_domain_point(half, spec.sub_log - 1, sh::bitrev(r, spec.sub_log));
-  sh::qm31 q = sh::qzero();
-  for (uint32_t k = 0; k < spec.n_acc; k++) {
-    // acc_data per k: point x (4), point y (4), first_linear (4), partial log (1) -> 13 words
-    const uint32_t* d = acc_data + 13 * k;
-    sh::cm31 prx = {d[0], d[1]}, pix = {d[2], d[3]}, pry = {d[4], d[5]}, piy = {d[6], d[7]};
-    sh::qm31 fl = sh::qload(d + 8);
-    uint32_t plog = d[12];
-    sh::cm31 den = sh::csub(sh::cmul(sh::csub(prx, {dp.x, 0}), piy), sh::cmul(sh::csub(pry, {dp.y, 0}), pix));
-    sh::cm31 dinv = sh::cinv(den);
-    uint32_t ratio = spec.sub_log - plog;
-    uint32_t li = ((r >> (ratio + 1)) << 1) + (r & 1);
-    const uint32_t* const* pk = parts + 4 * k;
-    sh::qm31 pv = {{pk[0][li], pk[1][li]}, {pk[2][li], pk[3][li]}};
-    sh::qm31 full = sh::qsub(pv, sh::qmulm(fl, dp.y));
-    q = sh::qadd(q, sh::qmulc(full, dinv));
+  const uint32_t rows = 1u << sub_log;
+  const uint32_t r0 = blockIdx.x * (256u * RB) + threadIdx.x;
+  uint32_t px[RB], py[RB];
+#pragma unroll
+  for (int j = 0; j < RB; j++) {
+    const uint32_t r = r0 + 256u * j;
+    sh::mpt dp = {1, 0};
+    if (r < rows) dp = sh_circle_domain_point(half, sub_log - 1, sh::bitrev(r, sub_log));
+    px[j] = dp.x;
+    py[j] = dp.y;
   }
-  out[0][r] = q.a.a;
-  out[1][r] = q.a.b;
-  out[2][r] = q.b.a;
-  out[3][r] = q.b.b;
+  uint64_t acc[RB][4];
+#pragma unroll
+  for (int j = 0; j < RB; j++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[j][c] = 0;
+  for (uint32_t k0 = 0; k0 < n_acc; k0 += NK) {
+    // denominators and their norms; an absent accumulation (the last chunk may be short) takes
+    // norm 1, and a zero norm is replaced by 1 with its inverse forced to 0
+    uint32_t d0[NK][RB], d1[NK][RB], nrm[NK][RB], pre[NK * RB];
+    uint32_t run = 1;
+#pragma unroll
+    for (int kk = 0; kk < NK; kk++) {
+      const bool has = k0 + kk < n_acc;
+      const CombineK K = kc[has ? k0 + kk : 0];
+#pragma unroll
+      for (int j = 0; j < RB; j++) {
+        const uint32_t a = mad_red(py[j], K.pix[0], mad_red(px[j], K.npiy[0], K.c[0]));
+        const uint32_t b = mad_red(py[j], K.pix[1], mad_red(px[j], K.npiy[1], K.c[1]));
+        d0[kk][j] = a;
+        d1[kk][j] = b;
+        const uint32_t n = mad_red(b, b, sh::mmul(a, a));
+        nrm[kk][j] = (!has || n == 0) ? 1u : n;
+        run = sh::mmul(run, nrm[kk][j]);
+        pre[kk * RB + j] = run;
+      }
+    }
+    uint32_t inv = sh::minv(run);
+#pragma unroll
+    for (int kk = NK - 1; kk >= 0; kk--) {
+      const bool has = k0 + kk < n_acc;
+      const CombineK K = kc[has ? k0 + kk : 0];
+      const uint32_t ratio = sub_log - K.plog;
+      const uint32_t* const* pk = parts + 4 * (has ? k0 + kk : 0);
+#pragma unroll
+      for (int j = RB - 1; j >= 0; j--) {
+        const int e = kk * RB + j;
+        const uint32_t a = d0[kk][j], b = d1[kk][j];
+        uint32_t ninv = e > 0 ? sh::mmul(inv, pre[e - 1]) : inv;  // 1 / norm of element e
+        inv = sh::mmul(inv, nrm[kk][j]);
+        if (a == 0 && b == 0) ninv = 0;
+        const uint32_t r = r0 + 256u * j;
+        if (!has || r >= rows) continue;
+        // den^-1 = (a - b i) / norm = (i0, -i1); g = p.y * den^-1 = (g0, g1)
+        const uint32_t i0 = sh::mmul(a, ninv), i1 = sh::mmul(b, ninv);
+        const uint32_t ni1 = sh::P - i1;  // -Im den^-1, in [1, P]
+        const uint32_t g0 = sh::mmul(py[j], i0), g1 = sh::mmul(py[j], ni1);
+        const uint32_t li = ((r >> (ratio + 1)) << 1) + (r & 1);
+        const uint32_t pv0 = pk[0][li], pv1 = pk[1][li], pv2 = pk[2][li], pv3 = pk[3][li];
+        // + partial * den^-1: each CM31 half (u0 + u1 i)(i0 - i1 i) = (u0 i0 + u1 i1) + (u1 i0 - u0 i1) i
+        mac64(acc[j][0], pv0, i0);
+        mac64(acc[j][0], pv1, i1);
+        mac64(acc[j][1], pv1, i0);
+        mac64(acc[j][1], pv0, ni1);
+        mac64(acc[j][2], pv2, i0);
+        mac64(acc[j][2], pv3, i1);
+        mac64(acc[j][3], pv3, i0);
+        mac64(acc[j][3], pv2, ni1);
+        // - first_linear * g: each CM31 half -(f0 + f1 i)(g0 + g1 i) = (-f0 g0 + f1 g1) + (-f0 g1 - f1 g0) i
+        mac64(acc[j][0], K.nfl[0], g0);
+        mac64(acc[j][0], K.flim[0], g1);
+        mac64(acc[j][1], K.nfl[0], g1);
+        mac64(acc[j][1], K.nfl[1], g0);
+        mac64(acc[j][2], K.nfl[2], g0);
+        mac64(acc[j][2], K.flim[1], g1);
+        mac64(acc[j][3], K.nfl[2], g1);
+        mac64(acc[j][3], K.nfl[3], g0);
+#pragma unroll
+        for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < RB; j++) {
+    const uint32_t r = r0 + 256u * j;
+    if (r >= rows) continue;
+#pragma unroll
+    for (int c = 0; c < 4; c++) out[c][r] = lazy_final(acc[j][c]);
+  }
 }
 
 // out[i] = sum_k col_k[lift_k(i)] for every coordinate.
@@ -303,20 +396,34 @@ extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* 
   const uint32_t s = L - b;
   SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(L + 1), L - 1, tw->root_initial_index, tw->root_log_size),
              "domain of log size %u is not in the twiddle tower", L);
-  std::vector<uint32_t> acc(13 * std::max<uint64_t>(n_acc, 1));
+  std::vector<CombineK> kc(std::max<uint64_t>(n_acc, 1));
   for (uint64_t k = 0; k < n_acc; k++) {
     SH_REQUIRE(ctx, partial_logs[k] >= 1 && partial_logs[k] <= s, "partial numerator log size %u not in [1, %u]",
                partial_logs[k], s);
-    for (int w = 0; w < 8; w++) acc[13 * k + w] = points[8 * k + w];
-    for (int w = 0; w < 4; w++) acc[13 * k + 8 + w] = first_linear[4 * k + w];
-    acc[13 * k + 12] = partial_logs[k];
+    const uint32_t* pt = points + 8 * k;  // x = (prx, pix), y = (pry, piy) as CM31 pairs
+    const sh::cm31 prx = {pt[0], pt[1]}, pix = {pt[2], pt[3]}, pry = {pt[4], pt[5]}, piy = {pt[6], pt[7]};
+    const sh::cm31 c = sh::csub(sh::cmul(prx, piy), sh::cmul(pry, pix));
+    const sh::cm31 npiy = sh::cneg(piy);
+    CombineK& K = kc[k];
+    K.c[0] = c.a;
+    K.c[1] = c.b;
+    K.npiy[0] = npiy.a;
+    K.npiy[1] = npiy.b;
+    K.pix[0] = pix.a;
+    K.pix[1] = pix.b;
+    const uint32_t* fl = first_linear + 4 * k;
+    for (int w = 0; w < 4; w++) K.nfl[w] = sh::mneg(fl[w]);
+    K.flim[0] = fl[1];
+    K.flim[1] = fl[3];
+    K.plog = partial_logs[k];
+    K.pad[0] = K.pad[1] = K.pad[2] = 0;
   }
   // subdomain: CircleDomain(Coset(half_odds(L-1).initial, s - 1)) (domain.rs:72-82 split)
   const uint32_t half_init = sh::subgroup_gen(L + 1);
   sh_point_tab half;
   SH_TRY(sh_point_tab_make(ctx, half_init, s - 1, &half));
   void *d_acc, *d_parts;
-  SH_TRY(sh_stage(ctx, acc.data(), acc.size() * 4, &d_acc));
+  SH_TRY(sh_stage(ctx, kc.data(), kc.size() * sizeof(CombineK), &d_acc));
   std::vector<const void*> parts(4 * std::max<uint64_t>(n_acc, 1), nullptr);
   for (uint64_t i = 0; i < 4 * n_acc; i++) parts[i] = d_partials[i];
   SH_TRY(sh_stage(ctx, parts.data(), parts.size() * sizeof(void*), &d_parts));
@@ -329,9 +436,20 @@ extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* 
   uint32_t* const* d_qtab = (uint32_t* const*)d_q;
   {
     SpanGuard span(ctx, "quotients_combine");
-    AccSpec spec = {(uint32_t)n_acc, s};
-    k_combine<<<((1u << s) + 255) / 256, 256, 0, ctx->stream>>>((const uint32_t* const*)d_parts,
-                                                                 (const uint32_t*)d_acc, spec, half, d_qtab);
+    const uint32_t* const* pt = (const uint32_t* const*)d_parts;
+    const CombineK* kt = (const CombineK*)d_acc;
+    const uint32_t na = (uint32_t)n_acc;
+    // accumulations per batch inversion: all of them up to 4 (every inverted norm is used); rows
+    // per lane: 8 for one accumulation, 4 otherwise (the register budget of 3 waves per SIMD)
+    auto grid = [&](int rb) { return (unsigned)(((1ull << s) + 256ull * rb - 1) / (256ull * rb)); };
+    if (n_acc <= 1)
+      k_combine<1, 8><<<grid(8), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
+    else if (n_acc == 2)
+      k_combine<2, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
+    else if (n_acc == 3)
+      k_combine<3, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
+    else
+      k_combine<4, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
     SH_TRY(sh_launch_check(ctx, "combine"));
   }
   // interpolate on the subdomain (its twiddles are prefixes of the domain-L layers), evaluate on L

@@ -19,8 +19,88 @@ from . import field as F
 from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, FriParams, HashLayerSet, SecureColumnByCoords,
                       _logs, _ptr_array, fri_prove, fri_quotients_arrays, tree_decommit)
 
+P = F.P
 OODS_POW_BITS = 16  # crates/stwo/src/core/proof_of_work.rs:8
 COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
+
+
+class ConstraintsNotSatisfied(Exception):
+    """ProvingError::ConstraintsNotSatisfied (crates/stwo/src/prover/mod.rs:137-140): the
+    composition value rebuilt from the sampled composition coordinates differs from the
+    constraints evaluated on the sampled trace values at the OODS point."""
+
+
+def _from_partial_evals(e):
+    """SecureField::from_partial_evals (crates/stwo/src/core/fields/qm31.rs): e0 + e1 i + e2 u +
+    e3 iu, each e_t a QM31."""
+    out = [0, 0, 0, 0]
+    for t in range(4):
+        basis = [0, 0, 0, 0]
+        basis[t] = 1
+        out = F.qadd(out, F.qmul(e[t], basis))
+    return out
+
+
+def composition_oods_eval(comp_samples, oods_point, max_log_degree_bound):
+    """StarkProof::extract_composition_oods_eval (crates/stwo/src/core/proof.rs:27-53): the eight
+    sampled coordinate values of the split composition polynomial give left + x * right, with x
+    the OODS point doubled max_log_degree_bound - 1 times (the split_at_mid factor)."""
+    left = _from_partial_evals(comp_samples[:4])
+    right = _from_partial_evals(comp_samples[4:])
+    x = F.qpoint_repeated_double(list(oods_point), max_log_degree_bound - 1)[:4]
+    return F.qadd(left, F.qmul(x, right))
+
+
+def _qmul_rows(x, y):
+    """QM31 products of the rows of two (n, 4) uint64 arrays of canonical words (u^2 = 2 + i)."""
+    def cmul(a0, a1, b0, b1):
+        return (a0 * b0 % P + P - a1 * b1 % P) % P, (a0 * b1 % P + a1 * b0 % P) % P
+    ac0, ac1 = cmul(x[:, 0], x[:, 1], y[:, 0], y[:, 1])
+    bd0, bd1 = cmul(x[:, 2], x[:, 3], y[:, 2], y[:, 3])
+    ad0, ad1 = cmul(x[:, 0], x[:, 1], y[:, 2], y[:, 3])
+    bc0, bc1 = cmul(x[:, 2], x[:, 3], y[:, 0], y[:, 1])
+    r0 = (ac0 + 2 * bd0 + P - bd1) % P    # + (2 + i) * bd
+    r1 = (ac1 + bd0 + 2 * bd1) % P
+    return np.stack([r0, r1, (ad0 + bc0) % P, (ad1 + bc1) % P], axis=1)
+
+
+def wide_fib_constraints_at_point(mask_values, alpha, oods_point, log_n, powers=None):
+    """Components::eval_composition_polynomial_at_point (crates/stwo/src/core/air/components.rs)
+    for one WideFibonacciEval component (crates/examples/src/wide_fibonacci/mod.rs:87-98):
+    the constraints c_{k+2} - (c_k^2 + c_{k+1}^2) in order, accumulated by Horner's rule in alpha
+    (PointEvaluationAccumulator, crates/stwo/src/core/air/accumulation.rs:18-33), i.e. constraint
+    j weighted by alpha^(m-1-j) for m constraints, each divided by the vanishing polynomial of the
+    trace coset CanonicCoset(log_n) at the point (crates/constraint-framework/src/component.rs:
+    220-243).  powers[j] = alpha^j when given (the prover already holds them)."""
+    coset_initial = 1 << (31 - (log_n + 1))  # CanonicCoset(n).coset = Coset::odds(n)
+    dinv = F.qinv(F.coset_vanishing_q(coset_initial, log_n, oods_point))
+    v = np.asarray(mask_values, dtype=np.uint64).reshape(-1, 4)
+    m = len(v) - 2
+    if m <= 0:
+        return [0, 0, 0, 0]
+    if powers is None:
+        pw = [[1, 0, 0, 0]]
+        for _ in range(m - 1):
+            pw.append(F.qmul(pw[-1], alpha))
+        powers = pw
+    w = np.asarray(powers, dtype=np.uint64).reshape(-1, 4)[:m][::-1]   # alpha^(m-1-j)
+    a, b, c = v[:m], v[1:m + 1], v[2:]
+    cons = (c + 2 * P - _qmul_rows(a, a) - _qmul_rows(b, b)) % P
+    terms = _qmul_rows(w, cons)
+    total = [int(t) for t in terms.sum(axis=0) % P]
+    return F.qmul(dinv, total)
+
+
+def check_wide_fib_oods(sampled_values, alpha, oods_point, log_n, log_blowup, powers=None):
+    """The sanity check closing prove_ex (crates/stwo/src/prover/mod.rs:123-132): raises
+    ConstraintsNotSatisfied when the composition value at the OODS point, rebuilt from the
+    composition samples, differs from the constraints evaluated on the trace samples.
+    sampled_values: [preprocessed, trace (one value per column), composition (8 values)]."""
+    max_log_degree_bound = (log_n + log_blowup) - log_blowup  # the lifting size minus the blowup
+    lhs = composition_oods_eval([c[0] for c in sampled_values[2]], oods_point, max_log_degree_bound)
+    rhs = wide_fib_constraints_at_point([c[0] for c in sampled_values[1]], alpha, oods_point, log_n, powers)
+    if [int(v) for v in lhs] != [int(v) for v in rhs]:
+        raise ConstraintsNotSatisfied("Constraints not satisfied.")
 
 
 class CommitmentTree:
@@ -102,11 +182,12 @@ class CommitmentSchemeProver:
         ldes = self.be.evaluate_polynomials(coeffs, b, self.tw) if coeffs else []
         return self._commit(coeffs, ldes, channel)
 
-    def prove_values(self, sampled_points, channel, oods_proof_of_work):
+    def prove_values(self, sampled_points, channel, oods_proof_of_work, check_samples=None):
         """prove_values (crates/stwo/src/prover/pcs/mod.rs:224-315). sampled_points[tree][col] is
-        a list of QM31 circle points (8 words). Returns (felts of the CommitmentSchemeProof as a uint64
-        array, the
-        proof as a dict)."""
+        a list of QM31 circle points (8 words). check_samples, when given, is called with the
+        sampled values (per tree and column) while the quotient kernels run; an exception from it
+        ends the proof.  Returns (felts of the CommitmentSchemeProof as a uint64 array, the proof
+        as a dict)."""
         L = self.trees[-1].lifting_log_size
         b = self.fri["log_blowup_factor"]
         ctx = self.be.ctx
@@ -163,6 +244,12 @@ class CommitmentSchemeProver:
             sampled_values.append([vl[o[i]:o[i + 1]] for i in range(n)])
             blocks.append((counts[c:c + n].tolist(), values[o[0]:o[-1]]))
             c += n
+        if check_samples is not None:
+            try:
+                check_samples(sampled_values)
+            except BaseException:
+                quotients.free()   # stream-ordered: the kernels enqueued on it finish first
+                raise
         params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
                            self.fri["fold_step"])
         fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients, as_array=True)
@@ -300,7 +387,11 @@ def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
     channel.mix_u64(oods_pow)
     z = [int(x) for x in channel.draw_random_point()]
     sampled_points = [[], [[z] for _ in range(n_cols)], [[z] for _ in range(8)]]
-    felts, proof = scheme.prove_values(sampled_points, channel, oods_pow)
+    # the sanity check closing prove_ex (crates/stwo/src/prover/mod.rs:123-132), run on the host
+    # while the quotient kernels run (it reads only the sampled values)
+    felts, proof = scheme.prove_values(
+        sampled_points, channel, oods_pow,
+        check_samples=lambda sv: check_wide_fib_oods(sv, alpha, z, log_n, b, powers[::-1]))
     proof["composition_random_coeff"] = alpha
     proof["oods_point"] = z
     return felts, proof

@@ -107,6 +107,13 @@ class ProofServer:
                 self._errors.append((req.index, repr(e)))
             finally:
                 if scheme is not None:
+                    # kernels still queued on this worker's stream may read the request's buffers
+                    # (after a failure the success-path sync above did not run): wait for them
+                    # before the buffers go back to the pool they came from
+                    try:
+                        ctx.sync()
+                    except Exception:
+                        pass
                     scheme.trees = None         # release this request's buffers (back to the pools)
                 req.done = time.perf_counter() - self._t0
                 with self._lock:

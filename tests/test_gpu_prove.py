@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 from oracle import protocol as PR
 from proving_amd.backend import BaseColumn
-from proving_amd.prover import prove_wide_fibonacci
+from proving_amd.prover import ConstraintsNotSatisfied, prove_wide_fibonacci
 from proving_amd.workloads import WIDE_FIB, wide_fib_trace
 
 pytestmark = pytest.mark.gpu
@@ -37,14 +37,15 @@ def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n,
 
 
 def test_wide_fib_bad_trace_rejected(ctx, backend):
+    """A trace that breaks one transition constraint: the prover itself refuses it
+    (ProvingError::ConstraintsNotSatisfied, crates/stwo/src/prover/mod.rs:123-132)."""
     n_cols, log_n = 16, 8
     trace = wide_fib_trace(n_cols, log_n, 5)
     trace[7][100] = (int(trace[7][100]) + 1) % P
     fri = params()
     tree = backend.precompute_canonic_twiddles(log_n + 1)
-    felts, _, _ = prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri)
-    with pytest.raises(AssertionError):
-        PR.wide_fib_verify(felts.tolist(), n_cols, log_n, fri)
+    with pytest.raises(ConstraintsNotSatisfied):
+        prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri)
 
 
 def test_wide_fib_full_size_verifies(ctx, backend):
