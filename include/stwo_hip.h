@@ -235,6 +235,18 @@ int sh_compute_quotients_and_combine(sh_ctx *ctx, const sh_twiddles *tw, const u
                                      const uint32_t *partial_logs, const uint32_t *points,
                                      const uint32_t *first_linear, uint64_t n_acc,
                                      uint32_t lifting_log_size, uint32_t log_blowup, uint32_t *const *d_out);
+/* sh_compute_quotients_and_combine with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The subdomain quotient lives in a static device array
+ * of 32 MiB per device (4 coordinate arrays of 2^21 words), so lifting_log_size - log_blowup must
+ * be at most 21 (SH_ERR_INVALID otherwise).  Every call on one device shares that array:
+ * calls that overlap on one device (other contexts or streams) race.  Same arguments and
+ * results as sh_compute_quotients_and_combine. */
+int sh_compute_quotients_and_combine_noalloc(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_partials,
+                                             const uint32_t *partial_logs, const uint32_t *points,
+                                             const uint32_t *first_linear, uint64_t n_acc,
+                                             uint32_t lifting_log_size, uint32_t log_blowup,
+                                             uint32_t *const *d_out);
 
 /* ---------------------------------------------------------------- AccumulationOps */
 /* AccumulationOps::lift_and_accumulate (crates/stwo/src/prover/air/accumulation.rs:198-200;

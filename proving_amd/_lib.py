@@ -60,6 +60,8 @@ SIGNATURES = {
                                          c_u64, c_vp, c_vp]),
     "sh_compute_quotients_and_combine": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_u32,
                                                  c_vp]),
+    "sh_compute_quotients_and_combine_noalloc": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32,
+                                                         c_u32, c_vp]),
     "sh_lift_and_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
     "sh_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u32]),
     "sh_grind": (c_int, [c_vp, c_vp, c_u32, c_u64p]),

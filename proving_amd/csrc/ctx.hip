@@ -10,6 +10,8 @@
 
 static const uint64_t kRingBytes = 16ull << 20;
 
+int sh_static_scratch_init(sh_ctx* ctx);  // quotients.hip
+
 int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...) {
   if (ctx) {
     char buf[1024];
@@ -47,6 +49,10 @@ extern "C" int sh_ctx_create(int device, sh_ctx** out) {
     return SH_ERR_HIP;
   }
   ctx->ring_size = kRingBytes;
+  if (sh_static_scratch_init(ctx) != SH_OK) {
+    sh_ctx_destroy(ctx);
+    return SH_ERR_HIP;
+  }
   // Built-in pass shapes: the autotuner's picks for the uneven-commit size
   // classes on MI355X (interpolations of 2^12..2^20, evaluations of 2^13..2^21),
   // block pass of 2^12 elsewhere.  sh_autotune_fft refines them per workload.

@@ -384,11 +384,34 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
   return SH_OK;
 }
 
-extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw,
-                                                const uint32_t* const* d_partials, const uint32_t* partial_logs,
-                                                const uint32_t* points, const uint32_t* first_linear,
-                                                uint64_t n_acc, uint32_t lifting_log_size, uint32_t log_blowup,
-                                                uint32_t* const* d_out) {
+// The allocation-free variant's scratch: the subdomain quotient (4 coordinate arrays of up to
+// 2^kStaticQuotLog words, 32 MiB), one copy per device for the life of the process.  Host code never
+// names it directly: sh_static_scratch_init resolves each device's address once.
+constexpr uint32_t kStaticQuotLog = 21;
+static __device__ __align__(256) uint32_t scratch[4u << kStaticQuotLog];
+constexpr int kMaxDevices = 64;
+static uint32_t* g_scratch_ptr[kMaxDevices];  // per device: the address of `scratch` on it
+static std::mutex g_scratch_mu;
+
+// Called by sh_ctx_create with ctx->device current: resolves that device's copy of `scratch`.
+int sh_static_scratch_init(sh_ctx* ctx) {
+  SH_REQUIRE(ctx, ctx->device >= 0 && ctx->device < kMaxDevices, "device %d beyond the static scratch table",
+             ctx->device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device]) return SH_OK;
+  void* ptr = nullptr;
+  SH_CHECK_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  SH_REQUIRE(ctx, ptr != nullptr, "hipGetSymbolAddress returned no address for the static scratch");
+  g_scratch_ptr[ctx->device] = (uint32_t*)ptr;
+  return SH_OK;
+}
+
+// q_scratch: 4 << (L - b) words for the subdomain quotient, or nullptr to use the context's
+// (lazily allocated) scratch slot.
+static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_partials,
+                                 const uint32_t* partial_logs, const uint32_t* points, const uint32_t* first_linear,
+                                 uint64_t n_acc, uint32_t lifting_log_size, uint32_t log_blowup,
+                                 uint32_t* const* d_out, uint32_t* q_scratch) {
   if (!ctx || !tw || !d_out || (n_acc && (!d_partials || !partial_logs || !points || !first_linear)))
     return SH_ERR_INVALID;
   const uint32_t L = lifting_log_size, b = log_blowup;
@@ -427,9 +450,12 @@ extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* 
   std::vector<const void*> parts(4 * std::max<uint64_t>(n_acc, 1), nullptr);
   for (uint64_t i = 0; i < 4 * n_acc; i++) parts[i] = d_partials[i];
   SH_TRY(sh_stage(ctx, parts.data(), parts.size() * sizeof(void*), &d_parts));
-  void* scr;
-  SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 16ull << s, &scr));
-  uint32_t* q = (uint32_t*)scr;
+  uint32_t* q = q_scratch;
+  if (!q) {
+    void* scr;
+    SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 16ull << s, &scr));
+    q = (uint32_t*)scr;
+  }
   const void* qcols[8] = {q, q + (1u << s), q + (2u << s), q + (3u << s), d_out[0], d_out[1], d_out[2], d_out[3]};
   void* d_q;
   SH_TRY(sh_stage(ctx, qcols, sizeof(qcols), &d_q));
@@ -456,6 +482,37 @@ extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* 
   SH_TRY(sh_fft_interpolate_batch(ctx, tw, (const uint32_t* const*)d_qtab, d_qtab, 4, s, L, half_init));
   SH_TRY(sh_fft_evaluate_batch(ctx, tw, (const uint32_t* const*)d_qtab, d_qtab + 4, 4, s, L, L, half_init));
   return SH_OK;
+}
+
+extern "C" int sh_compute_quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw,
+                                                const uint32_t* const* d_partials, const uint32_t* partial_logs,
+                                                const uint32_t* points, const uint32_t* first_linear,
+                                                uint64_t n_acc, uint32_t lifting_log_size, uint32_t log_blowup,
+                                                uint32_t* const* d_out) {
+  return quotients_and_combine(ctx, tw, d_partials, partial_logs, points, first_linear, n_acc, lifting_log_size,
+                               log_blowup, d_out, nullptr);
+}
+
+// The same with the subdomain quotient in the device's static scratch: no device allocation.
+// Every call on one device shares that scratch (which keeps the last call's contents; nothing
+// in it needs to start at zero, every word is written before it is read), so calls that overlap
+// on one device race.
+extern "C" int sh_compute_quotients_and_combine_noalloc(sh_ctx* ctx, const sh_twiddles* tw,
+                                                        const uint32_t* const* d_partials,
+                                                        const uint32_t* partial_logs, const uint32_t* points,
+                                                        const uint32_t* first_linear, uint64_t n_acc,
+                                                        uint32_t lifting_log_size, uint32_t log_blowup,
+                                                        uint32_t* const* d_out) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, lifting_log_size >= log_blowup + 1 && lifting_log_size - log_blowup <= kStaticQuotLog,
+             "quotient subdomain of log %u exceeds the static scratch (at most 2^%u rows per coordinate)",
+             lifting_log_size - log_blowup, kStaticQuotLog);
+  uint32_t* q = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device] : nullptr;
+  SH_REQUIRE(ctx, q != nullptr, "the static scratch of device %d was not resolved at context creation",
+             ctx->device);
+  // the four coordinate arrays packed at the subdomain's size (4 << s words in all)
+  return quotients_and_combine(ctx, tw, d_partials, partial_logs, points, first_linear, n_acc, lifting_log_size,
+                               log_blowup, d_out, q);
 }
 
 extern "C" int sh_lift_and_accumulate(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
