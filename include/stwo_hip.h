@@ -80,13 +80,14 @@ int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
 
 /* Tuning of the FFT pass shape for transforms of log size log_size (no
  * counterpart in the reference; the SIMD backend's split is fixed at compile
- * time): contiguous block pass of 2^block_log words, strided passes of
- * stride_layers layers over 2^tile_log groups, register stages of at most
- * max_stage layers.  Defaults are the autotuner's picks measured on MI355X
- * for the benchmark's sizes (block 2^12 or 2^13, 5-8 layers, 32 groups,
- * stages of 3-5).  Results never depend on it. */
+ * time): contiguous block pass of 2^block_log words (1..13), strided passes of
+ * stride_layers layers (1..8), and the rows per lane of the block and the
+ * strided passes, 2^r_block and 2^r_stride quads of 4 positions (1..4: more
+ * rows per lane run more layers per LDS round trip, fewer rows put more lanes
+ * on a small transform).  Defaults are the autotuner's picks on MI355X for the
+ * benchmark's sizes.  Results never depend on it. */
 int sh_set_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
-                      uint32_t tile_log, uint32_t max_stage);
+                      uint32_t r_block, uint32_t r_stride);
 int sh_get_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t out[4]);
 /* Autotuner (no counterpart in the reference): for the column sizes of a
  * workload, times candidate pass shapes of every interpolation and

@@ -24,8 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def plan(top, shape):
     """The pass plan sh_fft_* derive from a shape for a transform of `top` layers: the block pass
-    and the strided passes (lo, hi, tile log), as in plan_passes (fft.hip)."""
-    block_log, stride_layers, tile_log, _ = shape
+    and the strided passes (lo, hi), as in plan_passes (fft.hip)."""
+    block_log, stride_layers, _, _ = shape
     rem = 0
     if top > block_log:
         rem = (top - block_log) % stride_layers
@@ -38,7 +38,7 @@ def plan(top, shape):
         g = min(top - lo, stride_layers)
         if rem and rem <= 2 and stride_layers < top - lo < 2 * stride_layers:
             g = (top - lo + 1) // 2  # otherwise the last two strided passes share the layers
-        passes.append((lo, lo + g, min(lo, tile_log)))
+        passes.append((lo, lo + g))
         lo += g
     return tuple(passes)
 
@@ -83,8 +83,8 @@ def main():
     be.autotune_fft(logs, b, tree)
     timer = CommitTimer(ctx, be, tree, evals, b)
     sizes = sorted(set(logs) | {lg + b for lg in logs})
-    shapes = [(blk, st, tl, ms) for blk in (11, 12, 13) for st in (5, 6, 7, 8) for tl in (3, 4, 5)
-              for ms in (3, 4, 5) if st + tl <= 13]
+    shapes = [(blk, st, rb, rs) for blk in (11, 12, 13) for st in (5, 6, 7, 8) for rb in (1, 2, 3)
+              for rs in (1, 2, 3)]
     cur = timer()
     print(f"autotuner picks: commit step {cur:.4f} ms", flush=True)
     for sweep in range(args.sweeps):
@@ -93,7 +93,8 @@ def main():
             keep = be.fft_tuning(m)
             seen, best_t, best_ms = set(), keep, cur
             for t in shapes:
-                key = (plan(m, t), t[3])
+                p = plan(m, t)
+                key = (p, t[2], t[3] if len(p) > 1 else 0)  # r_stride only matters with strided passes
                 if key in seen:
                     continue
                 seen.add(key)
@@ -116,7 +117,7 @@ def main():
     print(f"final commit step {final:.4f} ms: {table}", flush=True)
     doc = {"device": "gfx950", "workload": "commit of 512 mixed columns, blowup 1, captured graph",
            "commit_step_ms": round(final, 4), "shapes": table,
-           "fields": "block_log, stride_layers, tile_log, max_stage"}
+           "fields": "block_log, stride_layers, r_block, r_stride"}
     with open(args.out, "w") as f:
         json.dump(doc, f, indent=1)
     print(f"wrote {args.out}", flush=True)

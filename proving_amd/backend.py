@@ -492,8 +492,8 @@ class HipBackend:
         self._chk(self.lib.sh_get_fft_tuning(self.h, log_size, out), "get_fft_tuning")
         return tuple(out)
 
-    def set_fft_tuning(self, log_size, block_log, stride_layers, tile_log, max_stage):
-        self._chk(self.lib.sh_set_fft_tuning(self.h, log_size, block_log, stride_layers, tile_log, max_stage),
+    def set_fft_tuning(self, log_size, block_log, stride_layers, r_block, r_stride):
+        self._chk(self.lib.sh_set_fft_tuning(self.h, log_size, block_log, stride_layers, r_block, r_stride),
                   "set_fft_tuning")
 
     # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------

@@ -64,8 +64,10 @@ struct sh_ctx {
   uint64_t pool_bytes = 0;
   // FFT pass shape per transform log size (see fft.hip)
   struct FftTune {
-    uint8_t block_log, stride_layers, tile_log, max_stage;
+    uint8_t block_log, stride_layers;
+    uint8_t r_block, r_stride;  // rows bits per lane of the quad-layout passes (fft2.hip)
   } fft_tune[32];
+  bool fft2 = true;  // quad-layout pass kernels where a specialisation exists
 };
 
 // Every staged copy is complete (the caller has synchronised the context's streams).

@@ -1,5 +1,6 @@
 // Context, device memory, staging ring, profiling spans and column utilities.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -49,20 +50,18 @@ extern "C" int sh_ctx_create(int device, sh_ctx** out) {
     return SH_ERR_HIP;
   }
   ctx->ring_size = kRingBytes;
+  {
+    // SH_FFT_IMPL=0 keeps every transform on the generic pass kernel (for A/B timing)
+    const char* impl = getenv("SH_FFT_IMPL");
+    ctx->fft2 = !(impl && impl[0] == '0');
+  }
   if (sh_static_scratch_init(ctx) != SH_OK) {
     sh_ctx_destroy(ctx);
     return SH_ERR_HIP;
   }
-  // Built-in pass shapes: the autotuner's picks for the uneven-commit size
-  // classes on MI355X (interpolations of 2^12..2^20, evaluations of 2^13..2^21),
-  // block pass of 2^12 elsewhere.  sh_autotune_fft refines them per workload.
-  for (uint32_t m = 0; m < 32; m++) ctx->fft_tune[m] = {(uint8_t)(m <= 13 ? 13 : 12), 8, 5, 4};
-  ctx->fft_tune[13] = {12, 8, 4, 5};
-  ctx->fft_tune[16] = {11, 8, 5, 3};
-  ctx->fft_tune[17] = {11, 5, 5, 3};
-  ctx->fft_tune[18] = {12, 7, 5, 5};
-  ctx->fft_tune[19] = {12, 6, 5, 4};
-  ctx->fft_tune[20] = {12, 8, 5, 5};
+  // Built-in pass shapes (the autotuner refines them per workload): a block pass of 2^13 words
+  // up to that size, else 2^12 and strided passes of 8 layers; 8 quads per lane.
+  for (uint32_t m = 0; m < 32; m++) ctx->fft_tune[m] = {(uint8_t)(m <= 13 ? 13 : 12), 8, 3, 3};
   *out = ctx;
   return SH_OK;
 }
@@ -480,14 +479,14 @@ bool is_doubling_of(uint32_t init, uint32_t log, uint32_t root_init, uint32_t ro
 }  // namespace sh
 
 extern "C" int sh_set_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
-                                 uint32_t tile_log, uint32_t max_stage) {
+                                 uint32_t r_block, uint32_t r_stride) {
   if (!ctx) return SH_ERR_INVALID;
   SH_REQUIRE(ctx, log_size < 32, "log size %u out of range", log_size);
   SH_REQUIRE(ctx, block_log >= 1 && block_log <= 13, "block_log %u not in [1, 13]", block_log);
-  SH_REQUIRE(ctx, tile_log <= 6 && stride_layers >= 1 && stride_layers + tile_log <= 13,
-             "strided pass of %u layers x 2^%u groups exceeds 32 KiB of LDS", stride_layers, tile_log);
-  SH_REQUIRE(ctx, max_stage >= 1 && max_stage <= 5, "max_stage %u not in [1, 5]", max_stage);
-  ctx->fft_tune[log_size] = {(uint8_t)block_log, (uint8_t)stride_layers, (uint8_t)tile_log, (uint8_t)max_stage};
+  SH_REQUIRE(ctx, stride_layers >= 1 && stride_layers <= 8, "stride_layers %u not in [1, 8]", stride_layers);
+  SH_REQUIRE(ctx, r_block >= 1 && r_block <= 4 && r_stride >= 1 && r_stride <= 4,
+             "rows per lane 2^%u / 2^%u not in [2, 16]", r_block, r_stride);
+  ctx->fft_tune[log_size] = {(uint8_t)block_log, (uint8_t)stride_layers, (uint8_t)r_block, (uint8_t)r_stride};
   return SH_OK;
 }
 
@@ -496,7 +495,7 @@ extern "C" int sh_get_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t out[4]
   const auto& t = ctx->fft_tune[log_size];
   out[0] = t.block_log;
   out[1] = t.stride_layers;
-  out[2] = t.tile_log;
-  out[3] = t.max_stage;
+  out[2] = t.r_block;
+  out[3] = t.r_stride;
   return SH_OK;
 }

@@ -33,8 +33,7 @@ struct FftPass {
 };
 
 // Layers [0, top) split into passes, ascending: a contiguous block pass of up
-// to 2^block_log words, then strided passes of stride_layers layers over tiles
-// of 2^tile_log groups.
+// to 2^block_log words, then strided passes of stride_layers layers.
 std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
   std::vector<FftPass> v;
   // One table entry serves both directions of a size: an LDE plans its 2^(m-1) computed layers,
@@ -58,7 +57,8 @@ std::vector<FftPass> plan_passes(uint32_t top, const sh_ctx::FftTune& t) {
     uint32_t g = top - l < t.stride_layers ? top - l : t.stride_layers;
     if (rem != 0 && rem <= 2 && top - l > t.stride_layers && top - l < 2 * t.stride_layers)
       g = (top - l + 1) / 2;  // the last two passes: ceil and floor of their layers
-    uint32_t tl = l < t.tile_log ? l : t.tile_log;
+    // tile of the generic kernel (the fallback): up to 32 groups, within 2^13 words
+    const uint32_t tl = std::min<uint32_t>(std::min<uint32_t>(l, 5u), 13u - g);
     v.push_back({l, l + g, tl});
     l += g;
   }
@@ -410,6 +410,32 @@ int check_domain(sh_ctx* ctx, const sh_twiddles* tw, uint32_t m) {
 
 }  // namespace
 
+int sh_fft2_block(sh_ctx* ctx, bool inverse, uint32_t g, uint32_t r, const uint32_t* const* d_src,
+                  uint32_t* const* d_dst, uint32_t n_cols, uint32_t m, uint32_t src_mask, const uint32_t* tw,
+                  uint32_t K, uint32_t Pl, uint32_t scale);
+int sh_fft2_strided(sh_ctx* ctx, bool inverse, uint32_t g, uint32_t r, uint32_t lo, const uint32_t* const* d_src,
+                    uint32_t* const* d_dst, uint32_t n_cols, uint32_t m, uint32_t src_mask, const uint32_t* tw,
+                    uint32_t K, uint32_t Pl, uint32_t scale);
+
+// One pass with the quad-layout kernels of fft2.hip (the tuned rows-per-lane first, then the
+// others); SH_ERR_NOT_FOUND when none of them has this shape.
+static int fft2_pass(sh_ctx* ctx, bool inverse, const FftPass& p, const sh_ctx::FftTune& t,
+                     const uint32_t* const* s, uint32_t* const* d, uint32_t n_cols, uint32_t m, uint32_t src_mask,
+                     const uint32_t* tw, uint32_t K, uint32_t Pl, uint32_t scale) {
+  if (!ctx->fft2) return SH_ERR_NOT_FOUND;
+  const uint32_t g = p.hi - p.lo;
+  const uint32_t first = p.lo == 0 ? t.r_block : t.r_stride;
+  const uint32_t order[5] = {first, 3, 2, 4, 1};
+  for (uint32_t k = 0; k < 5; k++) {
+    if (k > 0 && order[k] == first) continue;
+    const int rc = p.lo == 0 ? sh_fft2_block(ctx, inverse, g, order[k], s, d, n_cols, m, src_mask, tw, K, Pl, scale)
+                             : sh_fft2_strided(ctx, inverse, g, order[k], p.lo, s, d, n_cols, m, src_mask, tw, K, Pl,
+                                               scale);
+    if (rc != SH_ERR_NOT_FOUND) return rc;
+  }
+  return SH_ERR_NOT_FOUND;
+}
+
 // Interpolation of n_cols columns of log size m, all passes; src/dst are
 // device tables of column pointers (src may equal dst: in place).
 // Pl and half_init describe the domain: canonic domains pass Pl = m.
@@ -445,11 +471,17 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
     size_t lds = 4u << (G + p.tl);
     const uint32_t* const* s = k == 0 ? d_src : (const uint32_t* const*)d_dst;
     SpanGuard span(ctx, "fft_ifft_pass");
-    auto kern = tune.max_stage >= 5 ? k_fft_pass<true, 5> : tune.max_stage == 4 ? k_fft_pass<true, 4>
-                                                                                 : k_fft_pass<true, 3>;
+    const int rc2 = fft2_pass(ctx, true, p, tune, s, d_dst, n_cols, m, 0xffffffffu, tw->itw, tw->root_log_size, Pl,
+                              k + 1 == passes.size() ? scale : 0u);
+    if (rc2 != SH_ERR_NOT_FOUND) {
+      SH_TRY(rc2);
+      continue;
+    }
+    // the generic kernel: shapes with no quad-layout specialisation (blocks below 2^9 words)
+    auto kern = k_fft_pass<true, 4>;
     kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi,
                                                p.tl, k + 1 == passes.size() ? scale : 0u,
-                                               std::min<uint32_t>(tune.max_stage, 5));
+                                               4u);
     SH_TRY(sh_launch_check(ctx, "ifft_pass"));
   }
   return SH_OK;
@@ -485,11 +517,16 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
     bool first = k == (int)passes.size() - 1;
     const uint32_t* const* s = first ? d_src : (const uint32_t* const*)d_dst;
     SpanGuard span(ctx, "fft_rfft_pass");
-    auto kern = tune.max_stage >= 5 ? k_fft_pass<false, 5> : tune.max_stage == 4 ? k_fft_pass<false, 4>
-                                                                                  : k_fft_pass<false, 3>;
+    const int rc2 = fft2_pass(ctx, false, p, tune, s, d_dst, n_cols, m, first ? src_mask : 0xffffffffu, tw->tw,
+                              tw->root_log_size, Pl, 0u);
+    if (rc2 != SH_ERR_NOT_FOUND) {
+      SH_TRY(rc2);
+      continue;
+    }
+    auto kern = k_fft_pass<false, 4>;
     kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, first ? src_mask : 0xffffffffu, tw->tw,
                                                tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u,
-                                               std::min<uint32_t>(tune.max_stage, 5));
+                                               4u);
     SH_TRY(sh_launch_check(ctx, "rfft_pass"));
   }
   return SH_OK;
@@ -603,10 +640,9 @@ extern "C" int sh_autotune_fft(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   }
   std::vector<sh_ctx::FftTune> cands;
   for (uint8_t blk : {11, 12, 13})
-    for (uint8_t st : {3, 4, 5, 6, 7, 8})
-      for (uint8_t tl : {3, 4, 5})
-        for (uint8_t ms : {3, 4, 5})
-          if (st + tl <= 13) cands.push_back({blk, st, tl, ms});
+    for (uint8_t st : {5, 6, 7, 8})
+      for (uint8_t rb : {1, 2, 3})
+        for (uint8_t rs : {1, 2, 3}) cands.push_back({blk, st, rb, rs});
   hipEvent_t e0, e1;
   SH_CHECK_HIP(ctx, hipEventCreate(&e0));
   SH_CHECK_HIP(ctx, hipEventCreate(&e1));
@@ -636,7 +672,7 @@ extern "C" int sh_autotune_fft(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
       float best = 1e30f;
       sh_ctx::FftTune best_t = ctx->fft_tune[m];
       for (const auto& t : cands) {
-        if (t.block_log >= m && (t.stride_layers != 8 || t.tile_log != 5)) continue;  // same plan
+        if (t.block_log >= m && (t.stride_layers != 8 || t.r_stride != 3)) continue;  // same plan
         ctx->fft_tune[m] = t;
         float total = 1e30f;  // the fastest of 4 timed runs (the first, untimed, warms caches)
         for (int rep = 0; rep < 5 && rc == SH_OK; rep++) {

@@ -74,7 +74,7 @@ def test_evaluate_matches_cpu(ctx, backend, n, m):
     ("interpolate", 18, 18, (13, 4, 4, 4)),   # remainder 1, block full -> passes of 3 and 2
     ("interpolate", 16, 16, (13, 2, 3, 3)),   # remainder 1 with 2-layer passes
     ("evaluate", 16, 17, (12, 3, 3, 3)),      # the LDE plans its 16 computed layers
-    ("evaluate", 18, 19, (13, 4, 5, 4)),
+    ("evaluate", 18, 19, (13, 4, 2, 4)),
 ])
 def test_pass_plans_with_remainders(ctx, backend, kind, n, m, shape):
     rng = np.random.default_rng(7 * n + m)
@@ -96,6 +96,24 @@ def test_pass_plans_with_remainders(ctx, backend, kind, n, m, shape):
         tree.free()
     finally:
         backend.set_fft_tuning(m, *saved)
+
+
+# every family of the quad-layout pass kernels: block passes of 2^9..2^13 words and strided passes of
+# 1..8 layers, each with 2..16 rows per lane (shape = block_log, stride_layers, r_block, r_stride)
+QUAD_CASES = [(kind, n, m, (blk, st, r, r))
+              for r in (1, 2, 3, 4)
+              for kind, n, m, blk, st in [("interpolate", 9, 9, 13, 8), ("evaluate", 10, 11, 13, 8),
+                                          ("interpolate", 11, 11, 13, 8), ("evaluate", 12, 13, 12, 8),
+                                          ("interpolate", 13, 13, 13, 8), ("interpolate", 14, 14, 13, 1),
+                                          ("evaluate", 15, 16, 13, 2), ("interpolate", 16, 16, 13, 3),
+                                          ("evaluate", 17, 18, 13, 4), ("interpolate", 18, 18, 13, 5),
+                                          ("evaluate", 19, 20, 13, 6), ("interpolate", 20, 20, 13, 7),
+                                          ("evaluate", 19, 20, 11, 8)]]
+
+
+@pytest.mark.parametrize("kind,n,m,shape", QUAD_CASES)
+def test_quad_pass_kernels(ctx, backend, kind, n, m, shape):
+    test_pass_plans_with_remainders(ctx, backend, kind, n, m, shape)
 
 
 def test_round_trip_closed_form(ctx, backend):
