@@ -31,14 +31,16 @@ constexpr int kPadShift = 4;
 __host__ __device__ constexpr uint32_t padq(uint32_t q) { return q + (q >> kPadShift); }
 __device__ __forceinline__ lds_quad* lq(uint32_t byte_addr) { return (lds_quad*)(size_t)byte_addr; }
 
+// Butterflies with the twiddle given doubled (t2 = 2t): every twiddle is doubled once where it is
+// loaded and then serves 4 * 2^b butterflies, each one instruction cheaper (sh::mmul_dbl).
 template <bool INV>
-__device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t t) {
+__device__ __forceinline__ void bfly(uint32_t& a, uint32_t& b, uint32_t t2) {
   if constexpr (INV) {  // crates/stwo/src/core/fft.rs:14-21
     const uint32_t s = sh::madd(a, b);
-    b = sh::mmul(sh::msub(a, b), t);
+    b = sh::mmul_dbl(sh::msub(a, b), t2);
     a = s;
   } else {  // crates/stwo/src/core/fft.rs:5-12
-    const uint32_t m = sh::mmul(b, t);
+    const uint32_t m = sh::mmul_dbl(b, t2);
     b = sh::msub(a, m);
     a = sh::madd(a, m);
   }
@@ -115,7 +117,8 @@ __device__ __forceinline__ uint32_t lane_quad(uint32_t tau, uint32_t qs, uint32_
 }
 
 __device__ __forceinline__ void scale_rows(uint32_t* v, int n, uint32_t scale) {
-  for (int i = 0; i < n; i++) v[i] = sh::mmul(v[i], scale);
+  const uint32_t s2 = scale << 1;
+  for (int i = 0; i < n; i++) v[i] = sh::mmul_dbl(v[i], s2);
 }
 
 // ---------------------------------------------------------------- block pass
@@ -146,7 +149,7 @@ __device__ __forceinline__ void block_stage(uint32_t lds_base, uint32_t base, co
         const uint32_t hb = wpos >> (i + 1);
         const int off = (1 << RK) - (1 << (RK - b));
 #pragma unroll
-        for (int eh = 0; eh < (1 << (RK - 1 - b)); eh++) t[off + eh] = tab[hb + eh];
+        for (int eh = 0; eh < (1 << (RK - 1 - b)); eh++) t[off + eh] = tab[hb + eh] << 1;
       }
       t[(1 << RK) - 1] = 0;
       if constexpr (K == 0) {
@@ -160,12 +163,12 @@ __device__ __forceinline__ void block_stage(uint32_t lds_base, uint32_t base, co
 #pragma unroll
         for (int e = 0; e < (1 << RK); e += 2) {
           const uint2 xy = *(const uint2*)(tab1 + hq + e);
-          t1[e] = xy.x;
-          t1[e + 1] = xy.y;
-          ta[e] = xy.y;
-          tb[e] = sh::mneg(xy.y);
-          ta[e + 1] = sh::mneg(xy.x);
-          tb[e + 1] = xy.x;
+          t1[e] = xy.x << 1;  // doubled, as every twiddle here
+          t1[e + 1] = xy.y << 1;
+          ta[e] = xy.y << 1;
+          tb[e] = sh::mneg(xy.y) << 1;
+          ta[e + 1] = sh::mneg(xy.x) << 1;
+          tb[e + 1] = xy.x << 1;
         }
         if constexpr (INV) {
           quad_layers<true, RK>(v, t1, ta, tb);
@@ -221,12 +224,13 @@ __global__ __launch_bounds__(1 << (G - 2 - R)) void k_fft_block(const uint32_t* 
 #pragma unroll
   for (int k = 0; k < NQ / NT; k++) r[k] = *lq(lds_base + 16u * padq(threadIdx.x + k * NT));
   if (scale) {
+    const uint32_t s2 = scale << 1;
 #pragma unroll
     for (int k = 0; k < NQ / NT; k++) {
-      r[k].x = sh::mmul(r[k].x, scale);
-      r[k].y = sh::mmul(r[k].y, scale);
-      r[k].z = sh::mmul(r[k].z, scale);
-      r[k].w = sh::mmul(r[k].w, scale);
+      r[k].x = sh::mmul_dbl(r[k].x, s2);
+      r[k].y = sh::mmul_dbl(r[k].y, s2);
+      r[k].z = sh::mmul_dbl(r[k].z, s2);
+      r[k].w = sh::mmul_dbl(r[k].w, s2);
     }
   }
 #pragma unroll
@@ -275,7 +279,7 @@ __device__ __forceinline__ void strided_stage(uint32_t lds_base, const uint32_t*
       const uint32_t hb = (g_hi << (G - S - b - 1)) + (j0 >> (S + b + 1));
       const int off = (1 << RK) - (1 << (RK - b));
 #pragma unroll
-      for (int eh = 0; eh < (1 << (RK - 1 - b)); eh++) t[off + eh] = tab[hb + eh];
+      for (int eh = 0; eh < (1 << (RK - 1 - b)); eh++) t[off + eh] = tab[hb + eh] << 1;
     }
     t[(1 << RK) - 1] = 0;
     row_layers<INV, RK>(v, t);

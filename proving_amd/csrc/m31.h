@@ -34,6 +34,14 @@ SH_HD uint32_t mmul(uint32_t a, uint32_t b) {
   uint32_t hi = (uint32_t)(p >> 31);
   return reduce_once(lo + hi);
 }
+// a*b mod P with the second factor given doubled (b2 = 2b < 2^32, b < P), the SIMD backend's
+// twiddle form (crates/stwo/src/prover/backend/simd/circle.rs:452-459): the 64-bit product 2ab
+// has ab >> 31 as its high word and (ab mod 2^31) << 1 as its low word, so the split costs one
+// shift instead of a mask and a funnel shift.
+SH_HD uint32_t mmul_dbl(uint32_t a, uint32_t b2) {
+  const uint64_t p = (uint64_t)a * b2;
+  return reduce_once((uint32_t)(p >> 32) + ((uint32_t)p >> 1));
+}
 SH_HD uint32_t mpow(uint32_t a, uint32_t e) {
   uint32_t r = 1;
   while (e) {
