@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 39.32  # 256 CU x 64 lanes x 2.4 GHz, 32-bit integer ops
 B2_OPS_PER_COMPRESSION = 960  # 10 rounds x 8 G x 12 ops (3-input adds counted once)
+VALU_OPS_PER_BUTTERFLY = 11  # M31 butterfly: doubled-twiddle multiply 5, add 3, subtract 3
 
 
 def log(msg):
@@ -371,8 +372,10 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    # untimed warmup: a GPU idle before the timed region starts at lower clocks, and a 20-step
+    # region right after idling measures 8-10% slower than the same steps under sustained load
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", choices=["uneven", "plain", "fri_deep", "prove", "serve"], default="uneven",
                     help="uneven (default, the headline): the 512-column batched commit; plain: 64 x 2^14; "
                          "fri_deep: the FRI + DEEP-quotient phase on 200 x 2^21; prove: a full proof of the "
@@ -401,7 +404,7 @@ def main():
               "single MI355X and has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
-    from proving_amd.backend import BaseColumn, ColumnSet, Context, HashLayerSet, HipBackend
+    from proving_amd.backend import BaseColumn, ColumnSet, Context, HashLayer, HashLayerSet, HipBackend
     from proving_amd.workloads import PLAIN_COMMIT, UNEVEN_COMMIT, cells as count_cells, splitmix64_columns
 
     if args.copy_calibration:
@@ -484,6 +487,20 @@ def main():
 
     variants = {"graph_batched_ms_per_step": round(ms_per_step, 4),
                 "eager_batched_ms_per_step": round(sec_eager * 1e3 / args.steps, 4)}
+
+    # (2b) the same pipeline with the mixed-degree tree that injects the smaller columns at their
+    # matching layer (MerkleOps::commit_on_layer), also captured as one graph
+    mlayers = HashLayerSet(ctx, L)
+    _, _, _, mroot0 = be.commit_trace(evals, b, tree, coeffs, ldes, mlayers, mixed=True)
+    mgraph = be.commit_trace_graph(evals, b, tree, coeffs, ldes, mlayers, mixed=True)
+    for _ in range(args.warmup):
+        mgraph.launch()
+    sec_m = timed(mgraph.launch, args.steps)
+    assert mgraph.root() == mroot0, "mixed-degree graph replay differs from the eager mixed commit"
+    mgraph.free()
+    variants["mixed_graph_ms_per_step"] = round(sec_m * 1e3 / args.steps, 4)
+    variants["mixed_root"] = mroot0.hex()
+    log(f"[bench] graph batched, mixed-degree tree: {sec_m * 1e3 / args.steps:.3f} ms/step")
     if not args.no_plain:
         # (3) host-inclusive: the columns start in (pageable) host memory and cross PCIe first
         h2d_steps = max(1, args.steps // 4)
@@ -516,6 +533,20 @@ def main():
         variants["plain_per_column_ms_per_step"] = round(sec_plain * 1e3 / plain_steps, 4)
         variants["graph_vs_plain"] = round(sec_plain / plain_steps / (sec / args.steps), 2)
 
+    # (5) the reference's own Merkle benchmark shape (crates/stwo/benches/merkle.rs:12-26): one
+    # commit_on_layer over 256 columns of 2^16 zeros with no children
+    zcols = [BaseColumn.from_cpu(ctx, np.zeros(1 << 16, np.uint32)) for _ in range(256)]
+    zout = HashLayer(ctx, 16)
+    be.commit_on_layer(16, None, zcols, out=zout)
+    for _ in range(args.warmup):
+        be.commit_on_layer(16, None, zcols, out=zout)
+    sec_z = timed(lambda: be.commit_on_layer(16, None, zcols, out=zout), args.steps)
+    variants["commit_on_layer_256x2p16_ms"] = round(sec_z * 1e3 / args.steps, 4)
+    variants["commit_on_layer_256x2p16_elements_per_s"] = round((256 << 16) * args.steps / sec_z, 1)
+    log(f"[bench] commit_on_layer 256 x 2^16 (the reference's Merkle bench shape): "
+        f"{sec_z * 1e3 / args.steps:.3f} ms")
+    del zcols, zout
+
     work_per = algorithmic_work(logs, b)
     kernels = {}
     for k, v in prof.items():
@@ -535,20 +566,22 @@ def main():
     bytes_per_launch = work_per[dom]["bytes"] / n_launch
     avg_s = prof[dom]["ms"] / prof[dom]["n"] / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
-    # the FFT passes are bound by integer VALU issue, not HBM: 12 VALU instructions per M31
-    # butterfly (multiply 6, add 3, subtract 3) against the 32-bit VALU peak
+    # the FFT passes are bound by integer VALU issue, not HBM (the counters in
+    # profiles/commit_uneven512_hbm_traffic.md): an M31 butterfly is 11 VALU instructions of
+    # arithmetic (multiply by a doubled twiddle 5, add 3, subtract 3), priced against the 32-bit
+    # VALU peak; the HBM fraction of the same launches is reported beside it
     bfly = {"fft_ifft_pass": sum(lg << (lg - 1) for lg in logs),
             "fft_rfft_pass": sum(lg << (lg + b - 1) for lg in logs)}
-    valu_ops = 12 * bfly[dom] / n_launch
+    valu_ops = VALU_OPS_PER_BUTTERFLY * bfly[dom] / n_launch
     valu_tops = valu_ops / avg_s / 1e12
     tuning_now = {str(m): list(be.fft_tuning(m)) for m in sorted(set(logs) | {lg + b for lg in logs})}
     traffic, same_shapes = profiled_traffic(dom, tuning_now) if args.workload == "uneven" else (None, None)
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
-    roofline["valu"] = {"ops_per_butterfly": 12, "butterflies_per_launch": bfly[dom] / n_launch,
-                        "achieved_tops": round(valu_tops, 2), "peak_tops": VALU_PEAK_TOPS,
-                        "frac": round(valu_tops / VALU_PEAK_TOPS, 4)}
+    roofline = {"kernel": dom, "bound": "valu", "achieved": round(valu_tops, 2), "peak": VALU_PEAK_TOPS,
+                "unit": "TOP/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 4), "traffic": traffic,
+                "ops_per_butterfly": VALU_OPS_PER_BUTTERFLY, "butterflies_per_launch": bfly[dom] / n_launch,
+                "avg_launch_us": round(avg_s * 1e6, 2)}
+    roofline["hbm"] = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": bytes_per_launch}
     if traffic is not None:
         roofline["traffic_source"] = ("profiles/commit_uneven512_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + "
                                       "WRITE_SIZE x 1.0 per dispatch)")

@@ -172,6 +172,17 @@ int sh_commit_trace(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d
                     uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
                     uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
                     uint8_t root[32]);
+/* The same with the mixed-degree tree: TreeBuilder-style interpolate + LDE as above, then
+ * MerkleProver::commit (crates/stwo/src/prover/vcs/prover.rs:39-59 over MerkleOps::
+ * commit_on_layer, crates/stwo/src/prover/vcs/ops.rs:24-29), which injects each column at the
+ * layer of its LDE size (d_layers[k] receives layer k, k = 0..max(log_size) + log_blowup).
+ * Batched: the size classes' FFTs run largest first on a helper stream, and each column level of
+ * the tree starts when its class is ready (one launch per column level, the column-free levels
+ * above it reduced in the same launch).  root may be NULL (no sync). */
+int sh_commit_trace_mixed(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_evals,
+                          uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
+                          uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
+                          uint8_t root[32]);
 
 /* ---------------------------------------------------------------- OOD sampling */
 /* PolyOps::eval_at_point (crates/stwo/src/prover/poly/circle/ops.rs:44-48;
@@ -296,6 +307,12 @@ int sh_commit_trace_graph_create(sh_ctx *ctx, const sh_twiddles *tw, const uint3
                                  uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
                                  uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
                                  sh_graph **out);
+/* hipGraph of sh_commit_trace_mixed with these exact arguments (no counterpart in the
+ * reference); the same rules as sh_commit_trace_graph_create. */
+int sh_commit_trace_mixed_graph_create(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_evals,
+                                       uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
+                                       uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
+                                       sh_graph **out);
 int sh_graph_launch(sh_ctx *ctx, sh_graph *g);
 int sh_graph_destroy(sh_ctx *ctx, sh_graph *g);
 

@@ -50,6 +50,7 @@ SIGNATURES = {
     "sh_merkle_commit_on_layer": (c_int, [c_vp, c_u32, c_vp, c_vp, c_u64, c_vp]),
     "sh_merkle_commit_mixed": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "sh_commit_trace": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp]),
+    "sh_commit_trace_mixed": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp, c_vp]),
     "sh_eval_at_point": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
     "sh_barycentric_weights": (c_int, [c_vp, c_u32, c_vp, c_vp]),
     "sh_barycentric_eval_at_point": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
@@ -69,6 +70,8 @@ SIGNATURES = {
     "sh_get_fft_tuning": (c_int, [c_vp, c_u32, c_vp]),
     "sh_autotune_fft": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32]),
     "sh_commit_trace_graph_create": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp,
+                                             ctypes.POINTER(c_vp)]),
+    "sh_commit_trace_mixed_graph_create": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp,
                                              ctypes.POINTER(c_vp)]),
     "sh_graph_launch": (c_int, [c_vp, c_vp]),
     "sh_graph_destroy": (c_int, [c_vp, c_vp]),

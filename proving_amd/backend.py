@@ -446,8 +446,8 @@ class HipBackend:
         return layers, (bytes(root) if want_root else None)
 
     # ---- MerkleOps (crates/stwo/src/prover/vcs/ops.rs:8-29) ------------------------------------
-    def commit_on_layer(self, log_size, prev, cols):
-        out = HashLayer(self.ctx, log_size)
+    def commit_on_layer(self, log_size, prev, cols, out=None):
+        out = out if out is not None else HashLayer(self.ctx, log_size)
         self._chk(self.lib.sh_merkle_commit_on_layer(self.h, log_size, prev.ptr if prev else None,
                                                      _ptrs(cols), len(cols), out.ptr), "commit_on_layer")
         return out
@@ -460,12 +460,13 @@ class HipBackend:
                                                   len(cols), _ptrs(layers), root), "commit_mixed")
         return layers, bytes(root)
 
-    def commit_trace_graph(self, evals, log_blowup, twiddles, coeffs, ldes, layers):
-        """Captures commit_trace on these buffers as a HIP graph; returns a CommitGraph."""
+    def commit_trace_graph(self, evals, log_blowup, twiddles, coeffs, ldes, layers, mixed=False):
+        """Captures commit_trace (mixed: its mixed-degree tree variant) on these buffers as a HIP
+        graph; returns a CommitGraph."""
         h = ctypes.c_void_p()
-        self._chk(self.lib.sh_commit_trace_graph_create(
-            self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s(_logs(evals)), len(evals),
-            log_blowup, _ptrs(ldes), _ptrs(layers), ctypes.byref(h)), "commit_trace_graph_create")
+        fn = self.lib.sh_commit_trace_mixed_graph_create if mixed else self.lib.sh_commit_trace_graph_create
+        self._chk(fn(self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s(_logs(evals)), len(evals),
+                     log_blowup, _ptrs(ldes), _ptrs(layers), ctypes.byref(h)), "commit_trace_graph_create")
         g = CommitGraph(self.ctx, h, layers)
         g.buffers = (list(evals), list(coeffs), list(ldes), twiddles)  # the graph reads them on every launch
         return g
@@ -498,16 +499,18 @@ class HipBackend:
 
     # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------
     def commit_trace(self, evals, log_blowup, twiddles, coeffs=None, ldes=None, layers=None,
-                     want_root=True):
+                     want_root=True, mixed=False):
+        """Batched interpolate + LDE + Merkle commitment (sh_commit_trace); mixed=True builds the
+        mixed-degree tree (commit_on_layer) instead of the lifted one (sh_commit_trace_mixed)."""
         logs = np.asarray(_logs(evals), np.uint32)
         coeffs = coeffs if coeffs is not None else ColumnSet(self.ctx, logs)
         ldes = ldes if ldes is not None else ColumnSet(self.ctx, logs + log_blowup)
         L = int(logs.max()) + log_blowup
         layers = layers if layers is not None else HashLayerSet(self.ctx, L)
         root = (ctypes.c_uint8 * 32)() if want_root else None
-        self._chk(self.lib.sh_commit_trace(self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs),
-                                           _u32s(_logs(evals)), len(evals), log_blowup,
-                                           _ptrs(ldes), _ptrs(layers), root), "commit_trace")
+        fn = self.lib.sh_commit_trace_mixed if mixed else self.lib.sh_commit_trace
+        self._chk(fn(self.h, twiddles.h, _ptrs(evals), _ptrs(coeffs), _u32s(_logs(evals)), len(evals), log_blowup,
+                     _ptrs(ldes), _ptrs(layers), root), "commit_trace")
         return coeffs, ldes, layers, (bytes(root) if want_root else None)
 
 

@@ -1,4 +1,4 @@
-// HIP-graph capture of the batched trace commitment.
+// HIP-graph capture of the batched trace commitment (lifted or mixed-degree tree).
 //
 // A captured commit replays every launch of sh_commit_trace (both streams,
 // with their fork/join dependencies) as one graph launch, removing per-launch
@@ -48,10 +48,11 @@ void release(sh_graph* g) {
 }
 }  // namespace
 
-extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
-                                            uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n,
-                                            uint32_t log_blowup, uint32_t* const* d_ldes,
-                                            uint8_t* const* d_layers, sh_graph** out) {
+// Captures one batched commit (the lifted tree, or the mixed-degree tree when mixed) as a graph.
+static int commit_graph_create(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
+                               uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
+                               uint32_t* const* d_ldes, uint8_t* const* d_layers, bool mixed, sh_graph** out) {
+  auto commit = mixed ? sh_commit_trace_mixed : sh_commit_trace;
   if (!ctx || !out) return SH_ERR_INVALID;
   *out = nullptr;
   SH_REQUIRE(ctx, ctx->stream == ctx->main_stream, "graph capture must start on the main stream");
@@ -75,7 +76,7 @@ extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, 
   const bool prof = ctx->profile;
   ctx->profile = false;
   swap_state(ctx, g);
-  int rc = sh_commit_trace(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, nullptr);
+  int rc = commit(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, nullptr);
   if (rc == SH_OK) {
     hipError_t e = hipStreamSynchronize(ctx->main_stream);
     for (auto a : ctx->aux)
@@ -88,7 +89,7 @@ extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, 
     if (e != hipSuccess) {
       rc = sh_fail(ctx, SH_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     } else {
-      rc = sh_commit_trace(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, nullptr);
+      rc = commit(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, nullptr);
       hipGraph_t graph = nullptr;
       hipError_t e2 = hipStreamEndCapture(ctx->main_stream, &graph);
       if (rc == SH_OK && e2 != hipSuccess)
@@ -108,6 +109,21 @@ extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, 
   }
   *out = g;
   return SH_OK;
+}
+
+extern "C" int sh_commit_trace_graph_create(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
+                                            uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n,
+                                            uint32_t log_blowup, uint32_t* const* d_ldes,
+                                            uint8_t* const* d_layers, sh_graph** out) {
+  return commit_graph_create(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, false, out);
+}
+
+extern "C" int sh_commit_trace_mixed_graph_create(sh_ctx* ctx, const sh_twiddles* tw,
+                                                  const uint32_t* const* d_evals, uint32_t* const* d_coeffs,
+                                                  const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
+                                                  uint32_t* const* d_ldes, uint8_t* const* d_layers,
+                                                  sh_graph** out) {
+  return commit_graph_create(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, true, out);
 }
 
 extern "C" int sh_graph_launch(sh_ctx* ctx, sh_graph* g) {

@@ -245,46 +245,107 @@ __global__ __launch_bounds__(256) void k_next_layers_quad(const uint32_t* __rest
   }
 }
 
-// Mixed-degree node (crates/stwo/src/core/vcs/blake2_merkle.rs:13-29):
-// Blake2s(prev[2i] || prev[2i+1] || LE32(col_0[i]) || ... ).
-__global__ __launch_bounds__(kThreads) void k_commit_on_layer(uint32_t log_size, const uint32_t* prev,
-                                                              const uint32_t* const* cols, uint32_t cnt,
-                                                              uint32_t* out) {
-  __shared__ uint32_t msg_lds[kThreads * kMsgStride];
-  const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
-  const bool active = r < (1u << log_size);
-  uint32_t* lm = msg_lds + threadIdx.x * kMsgStride;
-  uint32_t h[8];
-  sh::b2_init(h);
-  uint32_t pend = 0, t = 0;
-  if (prev) {
-    if (active)
-      for (int j = 0; j < 16; j++) lm[j] = prev[16 * r + j];
-    pend = 16;
-  }
-  for (uint32_t k = 0; k < cnt; k++) {
-    if (pend == 16) {
-      t += 64;
-      if (active) {
-        uint32_t m[16];
-        load_msg(lm, m);
+// Mixed-degree levels in one launch: level `log_size` (one lane per node) is
+// Blake2s(prev[2i] || prev[2i+1] || LE32(col_0[i]) || ...) (crates/stwo/src/core/vcs/
+// blake2_merkle.rs:13-29); each 16-word block of column values is loaded straight into registers
+// (16 independent loads, coalesced across lanes) while the previous block is compressed.  Then
+// up to kMixedExtra column-free levels above it are reduced inside the workgroup through LDS.
+constexpr uint32_t kMixedExtra = 3;
+struct MixedArgs {
+  const uint32_t* prev;         // level log_size + 1, or nullptr at the bottom of the tree
+  const uint32_t* const* cols;  // this level's columns (device table), cnt of them
+  uint32_t cnt;
+  uint32_t log_size;
+  uint32_t extra;                   // column-free levels computed above this one
+  uint32_t* out[kMixedExtra + 1];   // out[l]: level log_size - l
+};
+
+__global__ __launch_bounds__(kThreads) void k_mixed_levels(MixedArgs a) {
+  __shared__ uint32_t node[kThreads * 8];
+  const uint32_t n0 = 1u << a.log_size;
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  if (i < n0) {
+    uint32_t h[8], m[16];
+    sh::b2_init(h);
+    bool full = false;  // m holds a full block, compressed once more data is known to follow
+    uint32_t t = 0;
+    if (a.prev) {
+      const uint4* s = (const uint4*)(a.prev + 16ull * i);
+      const uint4 q0 = s[0], q1 = s[1], q2 = s[2], q3 = s[3];
+      m[0] = q0.x; m[1] = q0.y; m[2] = q0.z; m[3] = q0.w;
+      m[4] = q1.x; m[5] = q1.y; m[6] = q1.z; m[7] = q1.w;
+      m[8] = q2.x; m[9] = q2.y; m[10] = q2.z; m[11] = q2.w;
+      m[12] = q3.x; m[13] = q3.y; m[14] = q3.z; m[15] = q3.w;
+      full = true;
+    }
+    uint32_t k = 0;
+    for (; k + 16 <= a.cnt; k += 16) {
+      uint32_t nx[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) nx[j] = a.cols[k + j][i];
+      if (full) {
+        t += 64;
         sh::b2_compress(h, m, t, false);
       }
-      pend = 0;
+#pragma unroll
+      for (int j = 0; j < 16; j++) m[j] = nx[j];
+      full = true;
     }
-    const uint32_t* col = cols[k];
-    if (active) lm[pend] = col[r];
-    pend++;
+    const uint32_t rem = a.cnt - k;
+    if (rem) {
+      uint32_t nx[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) nx[j] = (uint32_t)j < rem ? a.cols[k + j][i] : 0u;
+      if (full) {
+        t += 64;
+        sh::b2_compress(h, m, t, false);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; j++) m[j] = nx[j];
+      t += 4 * rem;
+    } else if (full) {
+      t += 64;
+    } else {  // no children and no columns: the hash of nothing
+#pragma unroll
+      for (int j = 0; j < 16; j++) m[j] = 0;
+    }
+    sh::b2_compress(h, m, t, true);
+    uint4* o = (uint4*)(a.out[0] + 8ull * i);
+    o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    *(uint4*)&node[threadIdx.x * 8] = make_uint4(h[0], h[1], h[2], h[3]);
+    *(uint4*)&node[threadIdx.x * 8 + 4] = make_uint4(h[4], h[5], h[6], h[7]);
   }
-  if (!active) return;
-  t += 4 * pend;
-  for (uint32_t j = pend; j < 16; j++) lm[j] = 0;
-  uint32_t m[16];
-  load_msg(lm, m);
-  sh::b2_compress(h, m, t, true);
-  uint4* o = (uint4*)(out + 8 * r);
-  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
-  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+  uint32_t width = kThreads;
+  for (uint32_t l = 1; l <= a.extra; l++) {
+    __syncthreads();
+    width >>= 1;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gi = blockIdx.x * width + tid;
+    uint32_t h[8];
+    const bool mine = tid < width && gi < (n0 >> l);
+    if (mine) {
+      uint32_t m[16];
+#pragma unroll
+      for (int w = 0; w < 16; w += 4) {
+        const uint4 q = *(const uint4*)&node[tid * 16 + w];
+        m[w] = q.x;
+        m[w + 1] = q.y;
+        m[w + 2] = q.z;
+        m[w + 3] = q.w;
+      }
+      sh::b2_init(h);
+      sh::b2_compress(h, m, 64, true);
+    }
+    __syncthreads();
+    if (mine) {
+      *(uint4*)&node[tid * 8] = make_uint4(h[0], h[1], h[2], h[3]);
+      *(uint4*)&node[tid * 8 + 4] = make_uint4(h[4], h[5], h[6], h[7]);
+      uint4* o = (uint4*)(a.out[l] + 8ull * gi);
+      o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+      o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+    }
+  }
 }
 
 }  // namespace
@@ -455,16 +516,20 @@ extern "C" int sh_merkle_commit_lifted(sh_ctx* ctx, const uint32_t* const* d_col
 int sh_commit_on_layer_impl(sh_ctx* ctx, uint32_t log_size, const uint32_t* d_prev,
                             const uint32_t* const* h_cols, uint64_t n, uint32_t* d_out) {
   SH_REQUIRE(ctx, log_size <= 30, "layer log size %u too large", log_size);
-  const uint32_t* const* d_cols = nullptr;
+  MixedArgs a = {};
+  a.prev = d_prev;
+  a.cnt = (uint32_t)n;
+  a.log_size = log_size;
+  a.extra = 0;
+  a.out[0] = d_out;
   if (n) {
     void* d_tab;
     SH_TRY(sh_stage(ctx, h_cols, n * sizeof(void*), &d_tab));
-    d_cols = (const uint32_t* const*)d_tab;
+    a.cols = (const uint32_t* const*)d_tab;
   }
-  uint64_t rows = 1ull << log_size;
+  const uint64_t rows = 1ull << log_size;
   SpanGuard span(ctx, "merkle_commit_on_layer");
-  k_commit_on_layer<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(
-      log_size, d_prev, d_cols, (uint32_t)n, d_out);
+  k_mixed_levels<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(a);
   return sh_launch_check(ctx, "commit_on_layer");
 }
 
@@ -472,6 +537,60 @@ extern "C" int sh_merkle_commit_on_layer(sh_ctx* ctx, uint32_t log_size, const u
                                          const uint32_t* const* d_cols, uint64_t n, uint8_t* d_out) {
   if (!ctx || !d_out || (n && !d_cols)) return SH_ERR_INVALID;
   return sh_commit_on_layer_impl(ctx, log_size, (const uint32_t*)d_prev, d_cols, n, (uint32_t*)d_out);
+}
+
+// The mixed-degree tree (MerkleProver::commit, crates/stwo/src/prover/vcs/prover.rs:39-59) over
+// columns already sorted descending by size: one k_mixed_levels launch per level that carries
+// columns (plus the bottom level), each also reducing up to kMixedExtra column-free levels above
+// it, and the column-free top of the tree through sh_build_layers_impl.  level_ready, when given,
+// holds one event per distinct column size (descending): a level waits for its columns' event.
+int sh_merkle_mixed_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* logs, uint64_t n,
+                         uint8_t* const* d_layers, const hipEvent_t* level_ready) {
+  for (uint64_t i = 0; i < n; i++) {
+    SH_REQUIRE(ctx, logs[i] <= 30, "column log size %u too large", logs[i]);
+    if (i) SH_REQUIRE(ctx, logs[i] <= logs[i - 1], "columns must be sorted descending by size");
+  }
+  const uint32_t max_log = n ? logs[0] : 0;
+  uint64_t pos = 0;
+  size_t cls = 0;
+  int lv = (int)max_log;
+  const uint32_t* prev = nullptr;
+  while (lv >= 0) {
+    const uint64_t first = pos;
+    while (pos < n && logs[pos] == (uint32_t)lv) pos++;
+    const uint64_t cnt = pos - first;
+    if (cnt == 0 && prev && pos == n) {
+      // every remaining level is column-free: the plain inner layers
+      return sh_build_layers_impl(ctx, (uint32_t)lv + 1, (uint32_t* const*)d_layers);
+    }
+    // column-free levels directly above this one, up to the next level with columns
+    uint32_t free_above = 0;
+    const uint32_t next_log = pos < n ? logs[pos] : 0;
+    const uint32_t limit = pos < n ? (uint32_t)lv - next_log - 1 : 0;  // all free when no columns remain
+    free_above = std::min<uint32_t>(std::min<uint32_t>(limit, kMixedExtra), (uint32_t)lv);
+    MixedArgs a = {};
+    a.prev = prev;
+    a.cnt = (uint32_t)cnt;
+    a.log_size = (uint32_t)lv;
+    a.extra = free_above;
+    if (cnt) {
+      void* d_tab;
+      SH_TRY(sh_stage(ctx, h_cols + first, cnt * sizeof(void*), &d_tab));
+      a.cols = (const uint32_t* const*)d_tab;
+    }
+    for (uint32_t l = 0; l <= free_above; l++) a.out[l] = (uint32_t*)d_layers[lv - (int)l];
+    if (cnt && level_ready) SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream, level_ready[cls], 0));
+    if (cnt) cls++;
+    {
+      SpanGuard span(ctx, "merkle_mixed_levels");
+      const uint64_t nodes = 1ull << lv;
+      k_mixed_levels<<<(unsigned)((nodes + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(a);
+      SH_TRY(sh_launch_check(ctx, "mixed_levels"));
+    }
+    prev = (const uint32_t*)d_layers[lv - (int)free_above];
+    lv -= (int)free_above + 1;
+  }
+  return SH_OK;
 }
 
 extern "C" int sh_merkle_commit_mixed(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
@@ -482,16 +601,13 @@ extern "C" int sh_merkle_commit_mixed(sh_ctx* ctx, const uint32_t* const* d_cols
   for (uint64_t i = 0; i < n; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](uint64_t x, uint64_t y) { return log_sizes[x] > log_sizes[y]; });
-  uint32_t max_log = n ? log_sizes[order[0]] : 0;
-  uint64_t pos = 0;
-  const uint32_t* prev = nullptr;
-  for (int ls = (int)max_log; ls >= 0; ls--) {
-    std::vector<const uint32_t*> layer_cols;
-    while (pos < n && log_sizes[order[pos]] == (uint32_t)ls) layer_cols.push_back(d_cols[order[pos++]]);
-    SH_TRY(sh_commit_on_layer_impl(ctx, (uint32_t)ls, prev, layer_cols.data(), layer_cols.size(),
-                                   (uint32_t*)d_layers[ls]));
-    prev = (const uint32_t*)d_layers[ls];
+  std::vector<const uint32_t*> cols(n);
+  std::vector<uint32_t> logs(n);
+  for (uint64_t i = 0; i < n; i++) {
+    cols[i] = d_cols[order[i]];
+    logs[i] = log_sizes[order[i]];
   }
+  SH_TRY(sh_merkle_mixed_impl(ctx, cols.data(), logs.data(), n, d_layers, nullptr));
   if (root) {
     SH_CHECK_HIP(ctx, hipMemcpyAsync(root, d_layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
     SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));

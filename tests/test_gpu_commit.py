@@ -53,8 +53,17 @@ def test_commit_mixed_512_columns_root(ctx, backend):
     exp = O.merkle_commit_lifted(exp_ldes, 21)
     assert root == exp[0][0].tobytes()
     # the mixed-degree tree that injects smaller columns at their matching layer
+    exp_mixed = O.merkle_commit_mixed(exp_ldes)
     _, mroot = backend.commit_mixed(ldes)
-    assert mroot == O.merkle_commit_mixed(exp_ldes)[0][0].tobytes()
+    assert mroot == exp_mixed[0][0].tobytes()
+    # the batched commit with the mixed-degree tree (FFT classes largest first, each column level
+    # started when its class is ready): same LDEs, the oracle's mixed root and layers
+    co2, ld2, lay2, mroot2 = backend.commit_trace(evals, 1, tree, mixed=True)
+    assert mroot2 == exp_mixed[0][0].tobytes()
+    for i in (0, 255, 256, 383, 384, 479, 480, 511):
+        np.testing.assert_array_equal(ld2[i].to_cpu(), exp_ldes[i])
+    for k in (21, 20, 19, 17, 13, 12, 6, 1):
+        np.testing.assert_array_equal(lay2[k].to_cpu(), exp_mixed[k], err_msg=f"layer {k}")
 
 
 def test_commit_repeatable(ctx, backend):

@@ -43,3 +43,33 @@ def test_graph_replays_match_eager(ctx, backend):
     for lde, exp in zip(ldes[::7], exp_ldes[::7]):
         np.testing.assert_array_equal(lde.to_cpu(), exp)
     g.free()
+
+
+def test_mixed_graph_replays_match_eager(ctx, backend):
+    """The captured batched commit with the mixed-degree tree: replays equal the eager commit,
+    and with new inputs the oracle's mixed-degree root."""
+    logs = [10] * 24 + [13] * 6 + [15] * 2 + [9] * 3
+    b = 1
+    L = max(logs) + b
+    tree = backend.precompute_canonic_twiddles(L)
+    evals = [BaseColumn.from_cpu(ctx, c) for c in splitmix64_columns(21, logs)]
+    coeffs = [BaseColumn(ctx, lg) for lg in logs]
+    ldes = [BaseColumn(ctx, lg + b) for lg in logs]
+    layers = [HashLayer(ctx, k) for k in range(L + 1)]
+    _, _, _, eager_root = backend.commit_trace(evals, b, tree, coeffs, ldes, layers, mixed=True)
+    g = backend.commit_trace_graph(evals, b, tree, coeffs, ldes, layers, mixed=True)
+    other = [BaseColumn.from_cpu(ctx, c) for c in splitmix64_columns(22, [12] * 4)]
+    for rep in range(4):
+        layers[0].upload(np.zeros(32, np.uint8))
+        g.launch()
+        assert g.root() == eager_root, rep
+        backend.commit_trace(other, b, tree, mixed=True)
+        ctx.sync()
+    host2 = splitmix64_columns(23, logs)
+    for e, c in zip(evals, host2):
+        e.upload(c)
+    g.launch()
+    tw, itw = O.canonic_twiddles(L)
+    exp_ldes = [O.evaluate(tw, O.interpolate(itw, c, lg), lg + b) for c, lg in zip(host2, logs)]
+    assert g.root() == O.merkle_commit_mixed(exp_ldes)[0][0].tobytes()
+    g.free()

@@ -115,3 +115,18 @@ def test_mixed_tree_matches_oracle(ctx, backend):
     assert root == exp[0][0].tobytes()
     for k in (0, 5, 12, 13, 17, 20):
         np.testing.assert_array_equal(layers[k].to_cpu(), exp[k])
+
+
+def test_mixed_tree_long_gaps_every_layer(ctx, backend):
+    """Column levels separated by 4-5 column-free levels (more than one launch reduces), up to 40
+    columns on one level, a single column at the top level: every layer equals the oracle's."""
+    rng = np.random.default_rng(9)
+    logs = [20] * 3 + [14] * 40 + [9] * 17 + [3] * 2 + [1]
+    host = [rng.integers(0, P, 1 << lg, dtype=np.uint32) for lg in logs]
+    order = rng.permutation(len(logs))  # the tree sorts the columns itself
+    cols = [BaseColumn.from_cpu(ctx, host[i]) for i in order]
+    layers, root = backend.commit_mixed(cols)
+    exp = O.merkle_commit_mixed([host[i] for i in order])
+    assert root == exp[0][0].tobytes()
+    for k in range(21):
+        np.testing.assert_array_equal(layers[k].to_cpu(), exp[k], err_msg=f"layer {k}")
