@@ -29,7 +29,7 @@ int sh_merkle_mixed_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
 static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
                          uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
                          uint32_t* const* d_ldes, uint8_t* const* d_layers, const std::vector<uint32_t>& classes,
-                         uint32_t L, bool mixed);
+                         uint32_t L, bool mixed, hipStream_t fft_stream);
 
 // The lifted tree takes the size classes ascending (its leaf groups are absorbed smallest first);
 // the mixed-degree tree descending (it is built from the largest columns' level down).
@@ -47,17 +47,25 @@ static int commit_pipeline(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* c
   const uint32_t L = classes.back() + log_blowup;
   if (mixed) std::reverse(classes.begin(), classes.end());
 
-  // fork: the auxiliary stream starts after the work already on the main stream
-  hipEvent_t fork = sh_sync_event(ctx, 0);
-  SH_CHECK_HIP(ctx, hipEventRecord(fork, ctx->main_stream));
-  SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->aux[0], fork, 0));
-  const int rc =
-      commit_forked(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, classes, L, mixed);
+  // fork (several size classes: a class's tree work overlaps the next class's FFTs): the
+  // auxiliary stream starts after the work already on the main stream.  A single class has
+  // nothing to overlap and stays on the main stream.
+  hipStream_t fft_stream = ctx->main_stream;
+  if (classes.size() > 1) {
+    SH_TRY(sh_aux_stream(ctx, &fft_stream));
+    hipEvent_t fork = sh_sync_event(ctx, 0);
+    SH_CHECK_HIP(ctx, hipEventRecord(fork, ctx->main_stream));
+    SH_CHECK_HIP(ctx, hipStreamWaitEvent(fft_stream, fork, 0));
+  }
+  const int rc = commit_forked(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, classes, L,
+                               mixed, fft_stream);
   if (rc != SH_OK) {
     // join: buffers freed after this call may be reused at once, so nothing enqueued on the
     // auxiliary stream may still run unordered after the main stream's work
-    hipEvent_t join = sh_sync_event(ctx, 1 + classes.size());
-    if (hipEventRecord(join, ctx->aux[0]) == hipSuccess) (void)hipStreamWaitEvent(ctx->main_stream, join, 0);
+    if (fft_stream != ctx->main_stream) {
+      hipEvent_t join = sh_sync_event(ctx, 1 + classes.size());
+      if (hipEventRecord(join, fft_stream) == hipSuccess) (void)hipStreamWaitEvent(ctx->main_stream, join, 0);
+    }
     return rc;
   }
   if (root) {
@@ -81,13 +89,13 @@ extern "C" int sh_commit_trace_mixed(sh_ctx* ctx, const sh_twiddles* tw, const u
   return commit_pipeline(ctx, tw, d_evals, d_coeffs, log_sizes, n, log_blowup, d_ldes, d_layers, root, true);
 }
 
-// The part of sh_commit_trace after the fork: FFT classes on aux[0], the tree on the main stream.
+// The part of sh_commit_trace after the fork: FFT classes on fft_stream, the tree on the main stream.
 // Every leaf group (lifted) or column level (mixed) waits for its class's event, and the last
 // class's event is waited on by the tree, so on success the auxiliary stream has joined back.
 static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_evals,
                          uint32_t* const* d_coeffs, const uint32_t* log_sizes, uint64_t n, uint32_t log_blowup,
                          uint32_t* const* d_ldes, uint8_t* const* d_layers, const std::vector<uint32_t>& classes,
-                         uint32_t L, bool mixed) {
+                         uint32_t L, bool mixed, hipStream_t fft_stream) {
   std::vector<hipEvent_t> ready(classes.size());
   for (size_t k = 0; k < classes.size(); k++) {
     const uint32_t m = classes[k];
@@ -100,7 +108,7 @@ static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* con
         ld.push_back(d_ldes[i]);
       }
     std::vector<uint32_t> cl(ev.size(), m), dl(ev.size(), m + log_blowup);
-    StreamScope scope(ctx, ctx->aux[0]);
+    StreamScope scope(ctx, fft_stream);
     SH_TRY(sh_interpolate_grouped(ctx, tw, ev.data(), co.data(), cl.data(), ev.size()));
     SH_TRY(sh_evaluate_grouped(ctx, tw, (const uint32_t* const*)co.data(), cl.data(), dl.data(), ld.data(),
                                ev.size()));

@@ -27,9 +27,9 @@ struct sh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;     // the stream every entry point orders its work on
   hipStream_t main_stream = nullptr;
-  // fork/join helper inside one call (the batched commit runs its FFT chain on it); a context
-  // holds only the streams it uses, so several contexts in one process stay within the device's
-  // few hardware queues
+  // fork/join helper inside one call (the batched commit of several column sizes runs its FFT
+  // chain on it), created on first use: a context holds only the streams it uses, so several
+  // contexts in one process stay within the device's few hardware queues
   hipStream_t aux[1] = {nullptr};
   std::string last_error;
   // pinned host arena + device mirror for per-call tables; identical tables are
@@ -115,6 +115,8 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev);
 hipEvent_t sh_sync_event(sh_ctx* ctx, size_t i);
 // Ensures scratch buffer `slot` holds at least `bytes` and returns it.
 int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out);
+// The context's auxiliary stream (created on first use).
+int sh_aux_stream(sh_ctx* ctx, hipStream_t* out);
 // Pooled device allocation for buffers that outlive one launch (see sh_ctx::pool).
 int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out);
 void sh_dev_free(sh_ctx* ctx, void* p);

@@ -41,7 +41,6 @@ extern "C" int sh_ctx_create(int device, sh_ctx** out) {
   sh_ctx* ctx = new sh_ctx();
   ctx->device = device;
   bool ok = hipStreamCreateWithFlags(&ctx->main_stream, hipStreamNonBlocking) == hipSuccess;
-  for (auto& a : ctx->aux) ok = ok && hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess;
   ctx->stream = ctx->main_stream;
   if (!ok || hipHostMalloc((void**)&ctx->h_ring, kRingBytes, hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void**)&ctx->d_ring, kRingBytes) != hipSuccess ||
@@ -136,7 +135,8 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
              "a per-call table was not staged before graph capture (run the call eagerly first)");
   if (ctx->ring_off + need > ctx->ring_size) {
     SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
-    for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
+    for (auto a : ctx->aux)
+      if (a) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
     ctx->ring_off = 0;
     ctx->table_cache.clear();
     sh_stage_all_complete(ctx);
@@ -179,7 +179,8 @@ int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out) {
   if (s.bytes < bytes) {
     if (s.ptr) {
       SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
-      for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
+      for (auto a : ctx->aux)
+      if (a) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
       SH_CHECK_HIP(ctx, hipFree(s.ptr));
       s.ptr = nullptr;
       s.bytes = 0;
@@ -191,6 +192,14 @@ int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out) {
     s.bytes = bytes;
   }
   *out = s.ptr;
+  return SH_OK;
+}
+
+// The auxiliary stream, created on first use: a context that never forks (every single-size
+// commit, every other entry point) holds one stream.
+int sh_aux_stream(sh_ctx* ctx, hipStream_t* out) {
+  if (!ctx->aux[0]) SH_CHECK_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux[0], hipStreamNonBlocking));
+  *out = ctx->aux[0];
   return SH_OK;
 }
 
@@ -282,7 +291,8 @@ static constexpr uint64_t kPoolCapBytes = 64ull << 30;  // cached bytes kept for
 
 static int pool_release(sh_ctx* ctx) {  // caller holds pool_mu
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->main_stream));
-  for (auto a : ctx->aux) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
+  for (auto a : ctx->aux)
+    if (a) SH_CHECK_HIP(ctx, hipStreamSynchronize(a));
   for (auto& kv : ctx->pool)
     for (void* p : kv.second) SH_CHECK_HIP(ctx, hipFree(p));
   ctx->pool.clear();
@@ -317,7 +327,8 @@ void sh_dev_free(sh_ctx* ctx, void* p) {
   auto it = ctx->pool_live.find(p);
   if (it == ctx->pool_live.end()) {  // not ours: free it the plain way
     hipStreamSynchronize(ctx->main_stream);
-    for (auto a : ctx->aux) hipStreamSynchronize(a);
+    for (auto a : ctx->aux)
+      if (a) hipStreamSynchronize(a);
     hipFree(p);
     return;
   }
@@ -325,7 +336,8 @@ void sh_dev_free(sh_ctx* ctx, void* p) {
   ctx->pool_live.erase(it);
   if (ctx->pool_bytes + cls > kPoolCapBytes) {
     hipStreamSynchronize(ctx->main_stream);
-    for (auto a : ctx->aux) hipStreamSynchronize(a);
+    for (auto a : ctx->aux)
+      if (a) hipStreamSynchronize(a);
     hipFree(p);
     return;
   }

@@ -67,7 +67,7 @@ static int commit_graph_create(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   {
     hipError_t e = hipStreamSynchronize(ctx->main_stream);
     for (auto a : ctx->aux)
-      if (e == hipSuccess) e = hipStreamSynchronize(a);
+      if (a && e == hipSuccess) e = hipStreamSynchronize(a);
     if (e != hipSuccess) {
       release(g);
       return sh_fail(ctx, SH_ERR_HIP, "stream synchronisation before capture failed: %s", hipGetErrorString(e));
@@ -80,7 +80,7 @@ static int commit_graph_create(sh_ctx* ctx, const sh_twiddles* tw, const uint32_
   if (rc == SH_OK) {
     hipError_t e = hipStreamSynchronize(ctx->main_stream);
     for (auto a : ctx->aux)
-      if (e == hipSuccess) e = hipStreamSynchronize(a);
+      if (a && e == hipSuccess) e = hipStreamSynchronize(a);
     if (e != hipSuccess) rc = sh_fail(ctx, SH_ERR_HIP, "warm-up failed: %s", hipGetErrorString(e));
     sh_stage_all_complete(ctx);  // the capture below only reuses tables, with no event waits
   }

@@ -12,6 +12,7 @@ Every kernel runs on the device; the host handles the channel, a few QM31 scalar
 small proof vectors. There is no CPU fallback: without libstwo_hip.so nothing here imports.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -41,13 +42,21 @@ def _from_partial_evals(e):
     return out
 
 
+def _doubled_x(x, n):
+    """x-coordinate of a point doubled n times (x -> 2x^2 - 1 needs only x)."""
+    for _ in range(n):
+        x = F.double_x(x)
+    return x
+
+
 def composition_oods_eval(comp_samples, oods_point, max_log_degree_bound):
     """StarkProof::extract_composition_oods_eval (crates/stwo/src/core/proof.rs:27-53): the eight
     sampled coordinate values of the split composition polynomial give left + x * right, with x
-    the OODS point doubled max_log_degree_bound - 1 times (the split_at_mid factor)."""
+    the x-coordinate of the OODS point doubled max_log_degree_bound - 1 times (the split_at_mid
+    factor)."""
     left = _from_partial_evals(comp_samples[:4])
     right = _from_partial_evals(comp_samples[4:])
-    x = F.qpoint_repeated_double(list(oods_point), max_log_degree_bound - 1)[:4]
+    x = _doubled_x(list(oods_point[:4]), max_log_degree_bound - 1)
     return F.qadd(left, F.qmul(x, right))
 
 
@@ -101,6 +110,26 @@ def check_wide_fib_oods(sampled_values, alpha, oods_point, log_n, log_blowup, po
     rhs = wide_fib_constraints_at_point([c[0] for c in sampled_values[1]], alpha, oods_point, log_n, powers)
     if [int(v) for v in lhs] != [int(v) for v in rhs]:
         raise ConstraintsNotSatisfied("Constraints not satisfied.")
+
+
+class _Background:
+    """fn(*args) on a daemon thread; result() joins it and re-raises what it raised."""
+
+    def __init__(self, fn, *args):
+        self._exc = None
+
+        def run():
+            try:
+                fn(*args)
+            except BaseException as e:  # handed to the caller of result()
+                self._exc = e
+        self._t = threading.Thread(target=run, daemon=True)
+        self._t.start()
+
+    def result(self):
+        self._t.join()
+        if self._exc is not None:
+            raise self._exc
 
 
 class CommitmentTree:
@@ -244,12 +273,11 @@ class CommitmentSchemeProver:
             sampled_values.append([vl[o[i]:o[i + 1]] for i in range(n)])
             blocks.append((counts[c:c + n].tolist(), values[o[0]:o[-1]]))
             c += n
+        checker = None
         if check_samples is not None:
-            try:
-                check_samples(sampled_values)
-            except BaseException:
-                quotients.free()   # stream-ordered: the kernels enqueued on it finish first
-                raise
+            # on a host thread while the FRI driver runs (the ctypes calls release the GIL);
+            # its verdict is collected once the proof is assembled
+            checker = _Background(check_samples, sampled_values)
         params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
                            self.fri["fold_step"])
         fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients, as_array=True)
@@ -261,6 +289,8 @@ class CommitmentSchemeProver:
             qv, wit = tree.decommit(self.be, pp_positions if i == 0 else queries)
             queried_values.append(qv)
             decommitments.append(wit)
+        if checker is not None:
+            checker.result()  # re-raises the check's exception (ConstraintsNotSatisfied)
         proof = {"fri_params": dict(self.fri), "commitments": [t.root for t in self.trees],
                  "oods_proof_of_work": oods_proof_of_work, "sampled_values": sampled_values,
                  "sampled_value_blocks": blocks,

@@ -1,18 +1,16 @@
-"""A proof-serving loop on one GPU: continuous batching of wide-Fibonacci prove requests.
+"""A proof-serving loop on one GPU: continuous admission of wide-Fibonacci prove requests.
 
-Requests arrive over time (a Poisson stream in the benchmark). The scheduler admits waiting
-requests shortest first (by trace cells) when the estimated device footprint fits the HBM budget
-and a slot is free; every scheduling
-tick, the traces of all newly admitted requests are generated on the device and committed
-together (one interpolation and one evaluation launch chain per column size across all of
-them, then each request's own Merkle tree), and each request then finishes its proof
-(composition, OODS, DEEP quotients, FRI, decommitment) on one of several worker contexts, so the
-proofs of different requests overlap on the device. The twiddle tree stays resident. A proof is
-the same bytes whether it is made alone or in the mix.
+Requests arrive over time (a Poisson stream in the benchmark) and are admitted as worker slots
+and the HBM budget allow; the proofs of different requests overlap on the device, and each is
+the same bytes whether it is made alone or in the mix.  The default schedule (ProofServer's
+"lanes") keeps one lane in strict arrival order and lets small requests bypass it on a second
+lane while a large proof occupies the first; the "batched" schedule commits the traces of every
+admitted batch together (one interpolation and evaluation launch chain per column size across
+requests) and finishes the proofs on worker contexts.  Twiddle trees stay resident.
 
-Worker contexts are independent (own streams and buffer pools); buffers committed on the
-scheduler's context are handed to a worker after the scheduler's stream is synchronised, and
-are released only after the worker has synchronised on its last use.
+Worker contexts are independent (own stream and buffer pool); in the batched schedule, buffers
+committed on the scheduler's context are handed to a worker after the scheduler's stream is
+synchronised, and are released only after the worker has synchronised on its last use.
 """
 import queue
 import threading
@@ -53,41 +51,65 @@ def device_bytes(n_cols, log_n, log_blowup=1):
 
 
 class ProofServer:
-    """whole_proofs: each worker generates, commits and proves its requests on its own context
-    (the proofs of different requests overlap on the device, with no handoff between contexts);
-    otherwise the scheduler commits the traces of each admitted batch together on its context
-    and the workers finish the proofs."""
+    """The request loop.  Two scheduling modes, both admitting by the HBM budget and keeping the
+    twiddle tree resident:
+
+    * "lanes" (default): each worker context proves whole requests.  Worker 0 is the main lane
+      and takes requests strictly in arrival order; the other workers are bypass lanes that take
+      only small requests (at most small_cells trace cells), oldest first, and only while the main
+      lane is busy.  A small proof then runs on the device beside a large one instead of queueing
+      ahead of or behind it: the small ones' latency drops, and the large ones wait only behind
+      what arrived before them on the main lane.  One stream per worker (a context creates its
+      helper stream only for multi-size commits), so the process stays within the device's
+      hardware queues.
+    * "batched": the scheduler commits the traces of each admitted batch together on its own
+      context (one interpolation and evaluation launch chain per column size across requests)
+      and the workers finish the proofs, shortest job first with aging.
+
+    whole_proofs=True selects "lanes", False "batched" (the names the tests use)."""
 
     def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05,
-                 whole_proofs=False):
+                 whole_proofs=True, small_cells=1 << 23):
         self.fri = dict(fri)
+        self.mode = "lanes" if whole_proofs else "batched"
         self.whole_proofs = whole_proofs
-        self.ctx = Context(0)
-        self.be = HipBackend(self.ctx)
-        self.be.load_fft_tuning()  # the committed pass-shape table, when there is one
-        self.tw = self.be.precompute_canonic_twiddles(twiddle_log)   # resident for every request
+        self.twiddle_log = twiddle_log
+        self.small_cells = small_cells
         self.budget = hbm_budget
-        self.max_in_flight = max_in_flight or 2 * n_workers
         self.n_workers = n_workers
+        self.max_in_flight = max_in_flight or (n_workers if whole_proofs else 2 * n_workers)
         self.max_wait = max_wait
-        self._work = queue.Queue()
         self._lock = threading.Lock()
         self._reserved = 0
         self._in_flight = 0
         self._errors = []
         self._t0 = None
-        self._workers = [threading.Thread(target=self._worker, daemon=True) for _ in range(n_workers)]
+        self.ctx = self.be = self.tw = None
+        if self.mode == "batched":
+            self.ctx = Context(0)
+            self.be = HipBackend(self.ctx)
+            self.be.load_fft_tuning()  # the committed pass-shape table, when there is one
+            self.tw = self.be.precompute_canonic_twiddles(twiddle_log)   # resident for every request
+        self._queues = [queue.Queue() for _ in range(n_workers)]
+        self._busy = [False] * n_workers
+        self._ready = threading.Semaphore(0)
+        self._workers = [threading.Thread(target=self._worker, args=(k,), daemon=True) for k in range(n_workers)]
         for w in self._workers:
             w.start()
+        for _ in range(n_workers):  # every worker's context and twiddles exist before serving starts
+            while not self._ready.acquire(timeout=1.0):
+                self._check_workers()
 
-    # ---- worker: the rest of each proof on its own context ---------------------------------
-    def _worker(self):
+    # ---- worker: whole proofs (lanes) or the rest of each proof (batched) ---------------------
+    def _worker(self, k):
         ctx = Context(0)
         be = HipBackend(ctx)
         be.load_fft_tuning()
-        tw = be.precompute_canonic_twiddles(self.tw.root_log_size + 1) if self.whole_proofs else None
+        tw = be.precompute_canonic_twiddles(self.twiddle_log) if self.mode == "lanes" else None
+        ctx.sync()
+        self._ready.release()
         while True:
-            item = self._work.get()
+            item = self._queues[k].get()
             if item is None:
                 if tw is not None:
                     tw.free()
@@ -119,6 +141,7 @@ class ProofServer:
                 with self._lock:
                     self._reserved -= cost
                     self._in_flight -= 1
+                    self._busy[k] = False
 
     def _check_workers(self):
         """A worker thread that died would leave its requests in flight forever."""
@@ -126,17 +149,53 @@ class ProofServer:
             raise RuntimeError(f"a worker thread stopped; errors so far: {self._errors[:3]}")
 
     def close(self):
-        for _ in self._workers:
-            self._work.put(None)
+        for q in self._queues:
+            q.put(None)
         for w in self._workers:
             w.join()
-        self.tw.free()
-        self.ctx.close()
+        if self.tw is not None:
+            self.tw.free()
+        if self.ctx is not None:
+            self.ctx.close()
 
-    # ---- scheduler ---------------------------------------------------------------------------
-    def _admit(self, batch):
+    # ---- scheduling ----------------------------------------------------------------------------
+    def _cells(self, r):
+        return r.n_cols << r.log_n
+
+    def _dispatch_lanes(self, waiting, now):
+        """Hands requests to idle lanes: the main lane takes the oldest request of any size, a
+        bypass lane the oldest small one while the main lane is busy.  Returns the dispatched."""
+        b = self.fri["log_blowup_factor"]
+        out = []
+        with self._lock:
+            for k in range(self.n_workers):
+                if self._busy[k] or not waiting:
+                    continue
+                if k == 0:
+                    cand = 0
+                elif self._busy[0]:
+                    cand = next((i for i, r in enumerate(waiting) if self._cells(r) <= self.small_cells), None)
+                    if cand is None:
+                        continue
+                else:
+                    continue  # the main lane is idle: it takes the next request itself
+                r = waiting[cand]
+                cost = device_bytes(r.n_cols, r.log_n, b)
+                if self._reserved + cost > self.budget and self._in_flight > 0:
+                    continue
+                self._reserved += cost
+                self._in_flight += 1
+                self._busy[k] = True
+                r.admitted = now
+                waiting.pop(cand)
+                out.append((k, r, cost))
+        for k, r, cost in out:
+            self._queues[k].put((r, None, None, cost))
+        return out
+
+    def _admit_batched(self, batch):
         """Generate and commit the traces of newly admitted requests together, then hand each to
-        the workers."""
+        the workers (round robin)."""
         b = self.fri["log_blowup_factor"]
         traces = []
         for r in batch:
@@ -146,13 +205,14 @@ class ProofServer:
         self.ctx.sync()  # the workers' contexts read these buffers
         for r, t in zip(batch, trees):
             scheme, channel = start_wide_fibonacci(self.be, self.tw, None, r.log_n, self.fri, committed_trace=t)
-            self._work.put((r, scheme, channel, device_bytes(r.n_cols, r.log_n, b)))
+            k = r.index % self.n_workers
+            self._queues[k].put((r, scheme, channel, device_bytes(r.n_cols, r.log_n, b)))
 
     def serve(self, requests, realtime=True):
         """Runs the request stream; arrivals are honoured in wall-clock time when realtime.
         Returns the requests with their timestamps and proof felts."""
         pending = sorted(requests, key=lambda r: r.arrival)
-        waiting = []  # arrived, not admitted
+        waiting = []  # arrived, not admitted, in arrival order
         self._t0 = time.perf_counter()
         i = 0
         b = self.fri["log_blowup_factor"]
@@ -161,9 +221,14 @@ class ProofServer:
             while i < len(pending) and (pending[i].arrival <= now or not realtime):
                 waiting.append(pending[i])
                 i += 1
-            # shortest job first (by trace cells), so a small request does not queue behind a
-            # large one; a request waiting longer than max_wait goes first (no starvation)
-            waiting.sort(key=lambda r: (now - r.arrival < self.max_wait, r.n_cols << r.log_n, r.arrival))
+            if self.mode == "lanes":
+                if not self._dispatch_lanes(waiting, now):
+                    self._check_workers()
+                    time.sleep(0.0002)
+                continue
+            # batched: shortest job first (by trace cells); a request waiting longer than
+            # max_wait goes first (no starvation)
+            waiting.sort(key=lambda r: (now - r.arrival < self.max_wait, self._cells(r), r.arrival))
             batch = []
             with self._lock:
                 while waiting:
@@ -176,11 +241,8 @@ class ProofServer:
                     self._in_flight += 1
                     r.admitted = now
                     batch.append(waiting.pop(0))
-            if batch and self.whole_proofs:
-                for r in batch:
-                    self._work.put((r, None, None, device_bytes(r.n_cols, r.log_n, b)))
-            elif batch:
-                self._admit(batch)
+            if batch:
+                self._admit_batched(batch)
             else:
                 self._check_workers()
                 time.sleep(0.0005)
@@ -189,7 +251,7 @@ class ProofServer:
                 if self._in_flight == 0:
                     break
             self._check_workers()
-            time.sleep(0.0005)
+            time.sleep(0.0002)
         if self._errors:
             raise RuntimeError(f"requests failed: {self._errors[:3]}")
         return requests
