@@ -196,6 +196,11 @@ int sh_eval_at_point(sh_ctx *ctx, const uint32_t *d_coeffs, uint32_t log_size, c
  * out[4 * (i * n_points + j) ..] (host).  Each column is read once per 4 points. */
 int sh_eval_at_points(sh_ctx *ctx, const uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n_cols,
                       const uint32_t *points, uint32_t n_points, uint32_t *out);
+/* compute_samples (crates/stwo/src/prover/pcs/mod.rs:176-222) at one point held in device memory
+ * (d_point, 8 words a device channel drew): column i (2^log_sizes[i] coefficients) evaluated at
+ * the point doubled doublings[i] times goes to d_values[4i..4i+4] (device).  No host wait. */
+int sh_eval_at_dpoint(sh_ctx *ctx, const uint32_t *const *d_coeffs, const uint32_t *log_sizes,
+                      const uint32_t *doublings, uint64_t n_cols, const uint32_t *d_point, uint32_t *d_values);
 /* PolyOps::barycentric_weights_into (crates/stwo/src/prover/poly/circle/ops.rs:50-62;
  * SIMD crates/stwo/src/prover/backend/simd/circle.rs:165-300): weights for
  * CanonicCoset(log_size) and the point, written to 4 coordinate arrays. */
@@ -276,6 +281,11 @@ int sh_accumulate(sh_ctx *ctx, uint32_t *const *d_col, const uint32_t *const *d_
  * the smallest nonce whose proof-of-work hash over the channel digest has at
  * least pow_bits trailing zero bits. */
 int sh_grind(sh_ctx *ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t *nonce);
+/* GrindOps::grind on a device channel (d_ch: 16 device words, stwo_prover.h) followed by
+ * mix_u64(nonce) (crates/stwo/src/prover/mod.rs:74-81): the nonce lands as a little-endian u64 in
+ * the first 8 bytes of d_ctl (16 device bytes); the host does not wait.  pow_bits <= 32; one
+ * search over 2^(pow_bits + 12) nonces (a nonce reading as all ones means none was found). */
+int sh_grind_dchannel(sh_ctx *ctx, uint32_t *d_ch, uint32_t pow_bits, uint32_t *d_ctl);
 
 /* ---------------------------------------------------------------- composition */
 /* ComponentProver::evaluate_constraint_quotients_on_domain
@@ -289,6 +299,11 @@ int sh_grind(sh_ctx *ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t 
 int sh_wide_fibonacci_composition(sh_ctx *ctx, const uint32_t *const *d_cols, uint64_t n_cols,
                                   uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
                                   const uint32_t *powers, uint32_t *const d_accum[4]);
+/* sh_wide_fibonacci_composition with the powers in device memory (d_powers, 4 words each, as
+ * sh_secure_powers_dev writes them). */
+int sh_wide_fibonacci_composition_dev(sh_ctx *ctx, const uint32_t *const *d_cols, uint64_t n_cols,
+                                      uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
+                                      const uint32_t *d_powers, uint32_t *const d_accum[4]);
 
 /* generate_trace of the wide-Fibonacci AIR (crates/examples/src/wide_fibonacci/mod.rs:27-48)
  * on the device from its two input columns d_c0, d_c1 (2^log_size rows): d_cols[0..n_cols)

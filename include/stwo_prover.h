@@ -38,6 +38,23 @@ void sh_channel_draw_queries(sh_channel *ch, uint32_t log_domain_size, uint32_t 
  * out[4k..4k+4] = felt^k for k < n (host arithmetic). */
 void sh_generate_secure_powers(const uint32_t felt[4], uint64_t n, uint32_t *out);
 
+/* The same channel kept in device memory (16 words: the digest as 8 little-endian words, then
+ * n_draws), so that a proof mixes its roots, nonces and sampled values and draws its randomness
+ * without the host waiting for any of them.  Each call is stream-ordered on the context stream. */
+/* The device channel set to a host state. */
+int sh_dchannel_load(sh_ctx *ctx, uint32_t *d_ch, const uint8_t digest[32], uint32_t n_draws);
+/* mix_felts / mix_u32s / mix_u64 (crates/stwo/src/core/channel/blake2s.rs:42-67) and mix_root over
+ * n device words. */
+int sh_dchannel_mix(sh_ctx *ctx, uint32_t *d_ch, const uint32_t *d_words, uint64_t n);
+/* draw_secure_felts (crates/stwo/src/core/channel/blake2s.rs:69-78): n felts into d_out (4 words
+ * each). */
+int sh_dchannel_draw_secure_felts(sh_ctx *ctx, uint32_t *d_ch, uint32_t n, uint32_t *d_out);
+/* CirclePoint::get_random_point (crates/stwo/src/core/circle.rs:150-160) into d_out (8 words). */
+int sh_dchannel_draw_random_point(sh_ctx *ctx, uint32_t *d_ch, uint32_t *d_out);
+/* generate_secure_powers (crates/stwo/src/prover/air/accumulation.rs:188-196) of a device felt:
+ * d_out[4k..4k+4] = felt^k for k < n, or felt^(n-1-k) when reversed. */
+int sh_secure_powers_dev(sh_ctx *ctx, const uint32_t *d_felt, uint64_t n, int reversed, uint32_t *d_out);
+
 /* ---------------------------------------------------------------- Merkle decommitment */
 /* MerkleProverLifted::decommit hash witness (crates/stwo/src/prover/vcs_lifted/prover.rs:97-162)
  * for a lifted tree of height L whose layers are d_layers[0..L] (layer k: 2^k hashes).
@@ -74,6 +91,14 @@ typedef struct {
 int sh_fri_prove(sh_ctx *ctx, const sh_twiddles *tw, sh_channel *ch, const sh_fri_params *params,
                  const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
                  uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);
+/* sh_fri_prove on a proof whose channel is the device channel d_ch: the driver starts from it
+ * without the host reading it, and ch_out receives the channel's state at the end of the proof
+ * (as sh_fri_prove leaves ch).  n_extra words at d_extra come back to h_extra with the commit
+ * phase's own download (the proof's device-drawn values, read in the same round trip). */
+int sh_fri_prove_dch(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *d_ch, sh_channel *ch_out,
+                     const uint32_t *d_extra, uint64_t n_extra, uint32_t *h_extra, const sh_fri_params *params,
+                     const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
+                     uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);
 
 /* ---------------------------------------------------------------- DEEP quotients */
 /* compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160): columns are the
@@ -87,6 +112,14 @@ int sh_fri_quotients(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *
                      const uint32_t *sample_points, const uint32_t *sample_values,
                      const uint32_t random_coeff[4], uint32_t lifting_log_size, uint32_t log_blowup,
                      uint32_t *const d_out[4]);
+/* compute_fri_quotients when every column has one sample and all share one point held in device
+ * memory with the values and the random coefficient (a device channel's draws): column i is
+ * sampled at d_point (8 words) with value d_values[4i..4i+4]; the line coefficients, per-size
+ * sums and combine constants are computed on the device.  Same output as sh_fri_quotients. */
+int sh_fri_quotients_dpoint(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *const *d_cols,
+                            const uint32_t *col_log_sizes, uint64_t n_cols, const uint32_t *d_point,
+                            const uint32_t *d_values, const uint32_t *d_random_coeff, uint32_t lifting_log_size,
+                            uint32_t log_blowup, uint32_t *const d_out[4]);
 
 #ifdef __cplusplus
 }

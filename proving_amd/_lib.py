@@ -42,7 +42,9 @@ SIGNATURES = {
     "sh_evaluate_polynomials": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_split_at_mid": (c_int, [c_vp, c_vp, c_u32, c_vp, c_vp]),
     "sh_eval_at_points": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_u32, c_vp]),
+    "sh_eval_at_dpoint": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "sh_wide_fibonacci_composition": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
+    "sh_wide_fibonacci_composition_dev": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
     "sh_wide_fibonacci_trace": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u64, c_vp]),
     "sh_merkle_build_leaves": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_merkle_build_next_layer": (c_int, [c_vp, c_vp, c_u32, c_vp]),
@@ -66,6 +68,7 @@ SIGNATURES = {
     "sh_lift_and_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
     "sh_accumulate": (c_int, [c_vp, c_vp, c_vp, c_u32]),
     "sh_grind": (c_int, [c_vp, c_vp, c_u32, c_u64p]),
+    "sh_grind_dchannel": (c_int, [c_vp, c_vp, c_u32, c_vp]),
     "sh_set_fft_tuning": (c_int, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32]),
     "sh_get_fft_tuning": (c_int, [c_vp, c_u32, c_vp]),
     "sh_autotune_fft": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32]),
@@ -85,11 +88,19 @@ SIGNATURES = {
     "sh_channel_draw_random_point": (None, [c_vp, c_vp]),
     "sh_channel_draw_queries": (None, [c_vp, c_u32, c_u32, c_vp]),
     "sh_generate_secure_powers": (None, [c_vp, c_u64, c_vp]),
+    "sh_dchannel_load": (c_int, [c_vp, c_vp, c_vp, c_u32]),
+    "sh_dchannel_mix": (c_int, [c_vp, c_vp, c_vp, c_u64]),
+    "sh_dchannel_draw_secure_felts": (c_int, [c_vp, c_vp, c_u32, c_vp]),
+    "sh_dchannel_draw_random_point": (c_int, [c_vp, c_vp, c_vp]),
+    "sh_secure_powers_dev": (c_int, [c_vp, c_vp, c_u64, c_int, c_vp]),
     "sh_merkle_decommit": (c_int, [c_vp, c_vp, c_u32, c_vp, c_u64, c_vp, c_u64, c_u64p]),
     "sh_gather": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp]),
     "sh_tree_decommit": (c_int, [c_vp, c_vp, c_vp, c_u64, c_vp, c_u32, c_vp, c_u64, c_vp, c_vp, c_u64, c_u64p]),
     "sh_fri_prove": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),
     "sh_fri_quotients": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
+    "sh_fri_quotients_dpoint": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
+    "sh_fri_prove_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64,
+                                 c_u64p, c_vp, c_u64p]),
 }
 
 _lib = None

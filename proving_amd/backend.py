@@ -660,6 +660,25 @@ class _Ops:
                   "wide_fibonacci_trace")
         return cols
 
+    def secure_powers_dev(self, felt_ptr, n_powers, reversed_, out_ptr):
+        """generate_secure_powers of a device felt into device memory (reversed: felt^(n-1-k))."""
+        self._chk(self.lib.sh_secure_powers_dev(self.h, felt_ptr, n_powers, int(bool(reversed_)), out_ptr),
+                  "secure_powers_dev")
+
+    def wide_fibonacci_composition_dev(self, cols, trace_log, eval_log, powers_ptr, accum, log_expansion=0):
+        """wide_fibonacci_composition with the constraint powers in device memory."""
+        self._chk(self.lib.sh_wide_fibonacci_composition_dev(self.h, _ptrs(cols), len(cols), trace_log, eval_log,
+                                                             log_expansion, powers_ptr, _ptrs(accum.columns)),
+                  "wide_fibonacci_composition_dev")
+        return accum
+
+    def eval_at_dpoint(self, coeffs, doublings, point_ptr, values_ptr):
+        """compute_samples at one device point: column i at the point doubled doublings[i] times
+        into values_ptr[4i..4i+4] (device)."""
+        dbl = np.ascontiguousarray(doublings, dtype=np.uint32)
+        self._chk(self.lib.sh_eval_at_dpoint(self.h, _ptrs(coeffs), _u32s(_logs(coeffs)), dbl.ctypes.data,
+                                             len(coeffs), point_ptr, values_ptr), "eval_at_dpoint")
+
     # ---- GrindOps (crates/stwo/src/core/proof_of_work.rs:10-14) -------------------------------
     def grind(self, digest: bytes, pow_bits):
         d = (ctypes.c_uint8 * 32)(*digest)
@@ -721,6 +740,46 @@ class Channel:
         out = (ctypes.c_uint32 * max(1, n))()
         self.lib.sh_channel_draw_queries(ctypes.byref(self.s), log_domain_size, n, out)
         return list(out)[:n]
+
+
+class DeviceChannel:
+    """Blake2sChannel held in device memory (16 words: the digest, then n_draws), advanced by
+    stream-ordered launches on the backend's context: a proof mixes its roots, nonces and sampled
+    values and draws its randomness without the host waiting (dchannel.hip)."""
+
+    def __init__(self, backend, channel=None):
+        self.be = backend
+        self.buf = DeviceBuffer(backend.ctx, 64)
+        st = channel.s if channel is not None else ChannelState()
+        backend._chk(backend.lib.sh_dchannel_load(backend.h, self.buf.ptr, st.digest, st.n_draws), "dchannel_load")
+
+    @property
+    def ptr(self):
+        return self.buf.ptr
+
+    def mix_words(self, ptr, n_words):
+        """mix_u32s / mix_felts / mix_root over n_words device words at ptr."""
+        self.be._chk(self.be.lib.sh_dchannel_mix(self.be.h, self.buf.ptr, ptr, n_words), "dchannel_mix")
+
+    def draw_secure_felts(self, n, out_ptr):
+        self.be._chk(self.be.lib.sh_dchannel_draw_secure_felts(self.be.h, self.buf.ptr, n, out_ptr),
+                     "dchannel_draw_secure_felts")
+
+    def draw_random_point(self, out_ptr):
+        self.be._chk(self.be.lib.sh_dchannel_draw_random_point(self.be.h, self.buf.ptr, out_ptr),
+                     "dchannel_draw_random_point")
+
+    def grind(self, pow_bits, ctl_ptr):
+        """GrindOps::grind + mix_u64(nonce); the nonce lands at ctl_ptr (2 words, little-endian)."""
+        self.be._chk(self.be.lib.sh_grind_dchannel(self.be.h, self.buf.ptr, pow_bits, ctl_ptr), "grind_dchannel")
+
+    def state(self):
+        """(digest bytes, n_draws), read back (waits for the stream)."""
+        w = self.buf.download(np.uint32, 16)
+        return w[:8].tobytes(), int(w[8])
+
+    def free(self):
+        self.buf.free()
 
 
 class FriParams(ctypes.Structure):
@@ -797,6 +856,36 @@ def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22, as_array=
                                           ctypes.byref(nf), qp.ctypes.data, ctypes.byref(nq)), "fri_prove")
     out = felts[:nf.value].copy() if as_array else felts[:nf.value].tolist()
     return out, qp[:nq.value].tolist()
+
+
+def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extra=None, cap=1 << 22):
+    """fri_prove on the device channel: channel_out (Channel) receives the final state.  extra:
+    (device pointer, n words) read back with the commit phase's download.  Returns (felts as a
+    uint64 array, sorted query positions, the extra words)."""
+    buf = getattr(backend, "_fri_felt_buf", None)
+    if buf is None or len(buf) < cap:
+        buf = backend._fri_felt_buf = np.empty(cap, np.uint64)
+    nf = ctypes.c_uint64()
+    qp = np.empty(max(1, params.n_queries), np.uint32)
+    nq = ctypes.c_uint64()
+    e_ptr, e_n = extra if extra is not None else (None, 0)
+    h_extra = np.empty(max(1, e_n), np.uint32)
+    backend._chk(backend.lib.sh_fri_prove_dch(backend.h, twiddles.h, dchannel.ptr, ctypes.byref(channel_out.s),
+                                              e_ptr, e_n, h_extra.ctypes.data, ctypes.byref(params),
+                                              _ptrs(column.columns), column.log_size, buf.ctypes.data, cap,
+                                              ctypes.byref(nf), qp.ctypes.data, ctypes.byref(nq)), "fri_prove_dch")
+    return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:e_n]
+
+
+def fri_quotients_dpoint(backend, twiddles, columns, point_ptr, values_ptr, random_coeff_ptr, lifting_log_size,
+                         log_blowup, out=None):
+    """fri_quotients when every column has one sample at the one device point point_ptr, its
+    value at values_ptr[4i..], and the random coefficient in device memory."""
+    out = out or SecureColumnByCoords(backend.ctx, lifting_log_size)
+    backend._chk(backend.lib.sh_fri_quotients_dpoint(
+        backend.h, twiddles.h, _ptrs(columns), _u32s(_logs(columns)), len(columns), point_ptr, values_ptr,
+        random_coeff_ptr, lifting_log_size, log_blowup, _ptrs(out.columns)), "fri_quotients_dpoint")
+    return out
 
 
 def fri_quotients(backend, twiddles, columns, samples, random_coeff, lifting_log_size, log_blowup, out=None):

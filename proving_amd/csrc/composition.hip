@@ -128,9 +128,11 @@ static uint32_t coset_vanishing_idx(uint32_t coset_initial, uint32_t coset_log, 
   return sh::index_to_point_host(sh::idx_reduce(q)).x;
 }
 
-extern "C" int sh_wide_fibonacci_composition(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
-                                             uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
-                                             const uint32_t* powers, uint32_t* const d_accum[4]) {
+// powers: host words (staged here) or, when powers_on_device, a device array the stream has
+// already produced (4 words per constraint).
+static int composition(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols, uint32_t trace_log,
+                       uint32_t eval_log, uint32_t log_expansion, const uint32_t* powers, bool powers_on_device,
+                       uint32_t* const d_accum[4]) {
   if (!ctx || !d_cols || !powers || !d_accum) return SH_ERR_INVALID;
   SH_REQUIRE(ctx, n_cols >= 3 && n_cols <= 4096, "wide Fibonacci needs 3..4096 columns (got %llu)",
              (unsigned long long)n_cols);
@@ -162,7 +164,10 @@ extern "C" int sh_wide_fibonacci_composition(sh_ctx* ctx, const uint32_t* const*
   // staged tables: column pointers, weights (4 words per constraint), denominators
   void *d_ptrs, *d_w, *d_den, *d_acc;
   SH_TRY(sh_stage(ctx, d_cols, n_cols * sizeof(void*), &d_ptrs));
-  SH_TRY(sh_stage(ctx, powers, 16 * (n_cols - 2), &d_w));
+  if (powers_on_device)
+    d_w = (void*)powers;
+  else
+    SH_TRY(sh_stage(ctx, powers, 16 * (n_cols - 2), &d_w));
   SH_TRY(sh_stage(ctx, denom.data(), 4ull * n_denom, &d_den));
   SH_TRY(sh_stage(ctx, d_accum, 4 * sizeof(void*), &d_acc));
   SpanGuard span(ctx, "composition");
@@ -171,4 +176,16 @@ extern "C" int sh_wide_fibonacci_composition(sh_ctx* ctx, const uint32_t* const*
       (const uint32_t* const*)d_ptrs, (uint32_t)n_cols, eval_log, trace_log, (const uint32_t*)d_w,
       (const uint32_t*)d_den, (uint32_t* const*)d_acc);
   return sh_launch_check(ctx, "wide_fib_composition");
+}
+
+extern "C" int sh_wide_fibonacci_composition(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
+                                             uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
+                                             const uint32_t* powers, uint32_t* const d_accum[4]) {
+  return composition(ctx, d_cols, n_cols, trace_log, eval_log, log_expansion, powers, false, d_accum);
+}
+
+extern "C" int sh_wide_fibonacci_composition_dev(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
+                                                 uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
+                                                 const uint32_t* d_powers, uint32_t* const d_accum[4]) {
+  return composition(ctx, d_cols, n_cols, trace_log, eval_log, log_expansion, d_powers, true, d_accum);
 }

@@ -91,8 +91,9 @@ __device__ __forceinline__ uint32_t grind_word0(const GrindPre& g, uint32_t lo, 
 }
 
 // ctl[0..1]: best nonce (u64, all ones when none), ctl[2]: next chunk to claim.
-__global__ __launch_bounds__(256) void k_grind32(GrindPre g, uint32_t mask, uint64_t base, uint32_t n_chunks,
-                                                 unsigned int* ctl) {
+__global__ __launch_bounds__(256) void k_grind32(const GrindPre* __restrict__ gp, uint32_t mask, uint64_t base,
+                                                 uint32_t n_chunks, unsigned int* ctl) {
+  const GrindPre g = *gp;
   unsigned long long* best = (unsigned long long*)ctl;
   unsigned int* next = ctl + 2;
   __shared__ uint32_t s_chunk;
@@ -153,7 +154,62 @@ __global__ __launch_bounds__(256) void k_grind(const uint32_t* __restrict__ pre_
   }
 }
 
+// The prefix state from a device channel's digest: prefixed = Blake2s(LE32(0x12345678) || digest
+// || LE32(pow_bits)) (one 40-byte block), then the nonce-independent part of round 0.  Also resets
+// the search control words (best nonce = none, next chunk = 0).
+__global__ void k_grind_prep(const uint32_t* __restrict__ ch, uint32_t pow_bits, GrindPre* __restrict__ out,
+                             unsigned int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t m[16], h[8];
+  m[0] = 0x12345678u;
+#pragma unroll
+  for (int i = 0; i < 8; i++) m[1 + i] = ch[i];
+  m[9] = pow_bits;
+#pragma unroll
+  for (int i = 10; i < 16; i++) m[i] = 0;
+  sh::b2_init(h);
+  sh::b2_compress(h, m, 40, true);
+  grind_prefix_state(h, out);
+  ctl[0] = 0xffffffffu;
+  ctl[1] = 0xffffffffu;
+  ctl[2] = 0;
+  ctl[3] = 0;
+}
+
+uint32_t grind_groups(uint32_t pow_bits, uint64_t n_chunks) {
+  // about four times the expected position of the first hit in flight at once: a small grind
+  // does not hash a whole device's worth of chunks before the first claims see the hit
+  const uint64_t want = std::max<uint64_t>(64, ((1ull << pow_bits) / kChunk) * 4);
+  return (uint32_t)std::min<uint64_t>(std::min<uint64_t>(n_chunks, want), kMaxGroups);
+}
+
 }  // namespace
+
+int sh_dch_mix_impl(sh_ctx* ctx, uint32_t* d_ch, const uint32_t* d_words, uint64_t n);
+
+// GrindOps::grind on a device channel (crates/stwo/src/core/proof_of_work.rs:10-14) followed by
+// mix_u64(nonce) (crates/stwo/src/prover/mod.rs:74-81, crates/stwo/src/prover/fri.rs:243-244),
+// with nothing returned to the host: d_ctl (16 device bytes) receives the nonce as a
+// little-endian u64 in its first 8 bytes.  One search launch over 2^(pow_bits + 12) nonces (the
+// chance of no valid nonce in it is below e^-4000); if none were found the nonce reads as all
+// ones, which no valid proof contains.
+extern "C" int sh_grind_dchannel(sh_ctx* ctx, uint32_t* d_ch, uint32_t pow_bits, uint32_t* d_ctl) {
+  if (!ctx || !d_ch || !d_ctl) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, pow_bits <= 32, "pow_bits %u above 32 has no device-channel search", pow_bits);
+  void* scr;
+  SH_TRY(sh_scratch_get(ctx, SCR_GRIND, 256, &scr));
+  GrindPre* d_pre = (GrindPre*)((uint8_t*)scr + 64);
+  SpanGuard span(ctx, "grind");
+  k_grind_prep<<<1, 64, 0, ctx->stream>>>(d_ch, pow_bits, d_pre, (unsigned int*)d_ctl);
+  SH_TRY(sh_launch_check(ctx, "grind_prep"));
+  const uint32_t mask = pow_bits == 32 ? 0xffffffffu : ((1u << pow_bits) - 1);
+  const uint32_t range_log = std::max<uint32_t>(pow_bits + 12, 16);
+  const uint64_t n_chunks = std::min<uint64_t>((1ull << range_log) / kChunk, 0xffffffffull);
+  k_grind32<<<grind_groups(pow_bits, n_chunks), 256, 0, ctx->stream>>>(d_pre, mask, 0, (uint32_t)n_chunks,
+                                                                      (unsigned int*)d_ctl);
+  SH_TRY(sh_launch_check(ctx, "grind"));
+  return sh_dch_mix_impl(ctx, d_ch, d_ctl, 2);
+}
 
 extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits, uint64_t* nonce) {
   if (!ctx || !digest || !nonce) return SH_ERR_INVALID;
@@ -177,10 +233,9 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
     const uint32_t mask = pow_bits == 32 ? 0xffffffffu : ((1u << pow_bits) - 1);
     const uint32_t range_log = std::max<uint32_t>(pow_bits + 6, 16);  // 2^range_log nonces per launch
     const uint64_t n_chunks = (1ull << range_log) / kChunk;
-    // about four times the expected position of the first hit in flight at once: a small grind
-    // does not hash a whole device's worth of chunks before the first claims see the hit
-    const uint64_t want = std::max<uint64_t>(64, ((1ull << pow_bits) / kChunk) * 4);
-    const unsigned groups = (unsigned)std::min<uint64_t>(std::min<uint64_t>(n_chunks, want), kMaxGroups);
+    const unsigned groups = grind_groups(pow_bits, n_chunks);
+    void* d_g;
+    SH_TRY(sh_stage(ctx, &g, sizeof(g), &d_g));
     unsigned int* d_ctl = (unsigned int*)d_best;
     // the control words' initial value comes from the staging arena (staged once, then a device
     // copy per launch), not from pageable host memory
@@ -190,7 +245,7 @@ extern "C" int sh_grind(sh_ctx* ctx, const uint8_t digest[32], uint32_t pow_bits
     SpanGuard span(ctx, "grind");
     for (uint64_t base = 0;; base += n_chunks * kChunk) {
       SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ctl, d_init, 16, hipMemcpyDeviceToDevice, ctx->stream));
-      k_grind32<<<groups, 256, 0, ctx->stream>>>(g, mask, base, (uint32_t)n_chunks, d_ctl);
+      k_grind32<<<groups, 256, 0, ctx->stream>>>((const GrindPre*)d_g, mask, base, (uint32_t)n_chunks, d_ctl);
       SH_TRY(sh_launch_check(ctx, "grind"));
       unsigned long long best = 0;
       SH_CHECK_HIP(ctx, hipMemcpyAsync(&best, d_best, 8, hipMemcpyDeviceToHost, ctx->stream));

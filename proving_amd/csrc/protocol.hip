@@ -608,10 +608,16 @@ extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, cons
 }
 
 // ------------------------------------------------------------------ FRI prover
-extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const sh_fri_params* p,
-                            const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
-                            uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
-  if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions) return SH_ERR_INVALID;
+// d_ch_in: when set, the proof's channel is this device channel (16 words, as dchannel.hip keeps
+// it): the driver starts from it, and ch receives its state (read back with the commit phase's
+// download) before the host replays the layers.  d_extra / h_extra: n_extra words of the caller's
+// device memory that the same download brings back.
+static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const uint32_t* d_ch_in,
+                          const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, const sh_fri_params* p,
+                          const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
+                          uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions || (n_extra && (!d_extra || !h_extra)))
+    return SH_ERR_INVALID;
   SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 4, "fold_step %u out of range [1, 4]", p->fold_step);
   SH_REQUIRE(ctx, p->log_blowup_factor >= 1 && p->log_blowup_factor <= 16 && p->log_last_layer_degree_bound <= 10,
              "invalid FRI parameters");
@@ -644,11 +650,14 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   // them from device memory.  The host replays the same channel operations afterwards, from the
   // downloaded roots, and checks that the alphas agree.
   const uint32_t max_layers = log_size + 1;
-  uint32_t* d_aux = nullptr;  // [ch: 16 words][alphas: 16 words per layer][roots: 8 words per layer]
-  TRY_CLEAN(sh_dev_alloc(ctx, 4ull * (16 + 24 * max_layers), (void**)&d_aux));
+  // [ch: 16 words][alphas: 16 words per layer][roots: 8 words per layer][initial ch: 16 words]
+  const uint64_t aux_words = 16 + 24 * max_layers + 16;
+  uint32_t* d_aux = nullptr;
+  TRY_CLEAN(sh_dev_alloc(ctx, 4 * aux_words, (void**)&d_aux));
   uint32_t* d_ch = d_aux;
   uint32_t* d_alphas = d_aux + 16;
   uint32_t* d_roots = d_alphas + 16 * max_layers;
+  uint32_t* d_init = d_roots + 8 * max_layers;
   auto cleanup_all = [&]() {
     cleanup();
     if (d_aux) sh_dev_free(ctx, d_aux);
@@ -663,10 +672,13 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
       return rc;           \
     }                      \
   } while (0)
-  std::vector<uint32_t> h_ch(16, 0);
-  memcpy(h_ch.data(), ch->digest, 32);
-  h_ch[8] = ch->n_draws;
-  {
+  if (d_ch_in) {
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ch, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
+    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_init, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    std::vector<uint32_t> h_ch(16, 0);
+    memcpy(h_ch.data(), ch->digest, 32);
+    h_ch[8] = ch->n_draws;
     void* staged;
     TRY_CLEAN(sh_stage(ctx, h_ch.data(), 64, &staged));
     SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ch, staged, 64, hipMemcpyDeviceToDevice, ctx->stream));
@@ -769,13 +781,18 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
     n_layers++;
   }
   // one download: the device channel, the alphas and roots of every layer, the last layer
-  std::vector<uint32_t> aux(16 + 24 * max_layers);
+  std::vector<uint32_t> aux(aux_words);
   std::vector<uint32_t> words(4ull << last_log);
   SH_CHECK_HIP(ctx, hipMemcpyAsync(aux.data(), d_aux, 4 * aux.size(), hipMemcpyDeviceToHost, ctx->stream));
   for (int c = 0; c < 4; c++)
     SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + ((uint64_t)c << last_log), cur.cols[c], 4ull << last_log,
                                      hipMemcpyDeviceToHost, ctx->stream));
+  if (n_extra) SH_CHECK_HIP(ctx, hipMemcpyAsync(h_extra, d_extra, 4 * n_extra, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (d_ch_in) {
+    memcpy(ch->digest, &aux[16 + 24 * max_layers], 32);
+    ch->n_draws = aux[16 + 24 * max_layers + 8];
+  }
   for (uint32_t l = 0; l < n_layers; l++) {
     DeviceTree& t = l == 0 ? first.tree : inner[l - 1].tree;
     memcpy(t.root.data(), &aux[16 + 16 * max_layers + 8 * l], 32);
@@ -898,6 +915,22 @@ extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, 
   memcpy(felts, f.v.data(), 8 * f.v.size());
   if (query_positions) memcpy(query_positions, q.data(), 4 * q.size());
   return SH_OK;
+}
+
+extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const sh_fri_params* p,
+                            const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
+                            uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
+  return fri_prove_impl(ctx, tw, ch, nullptr, nullptr, 0, nullptr, p, column, log_size, felts, cap, n_felts,
+                        query_positions, n_query_positions);
+}
+
+extern "C" int sh_fri_prove_dch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* d_ch, sh_channel* ch_out,
+                                const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, const sh_fri_params* p,
+                                const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
+                                uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (!d_ch) return SH_ERR_INVALID;
+  return fri_prove_impl(ctx, tw, ch_out, d_ch, d_extra, n_extra, h_extra, p, column, log_size, felts, cap, n_felts,
+                        query_positions, n_query_positions);
 }
 
 // ------------------------------------------------------------------ DEEP quotient driver

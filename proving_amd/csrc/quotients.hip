@@ -54,7 +54,8 @@ __device__ __forceinline__ uint32_t lazy_final(uint64_t v) {
 template <int NB>
 __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                    const uint32_t* __restrict__ coef, uint32_t rows,
-                                                   uint32_t* const* __restrict__ outs, OutSpec spec) {
+                                                   uint32_t* const* __restrict__ outs, OutSpec spec,
+                                                   const uint32_t* __restrict__ dev_B) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   uint64_t acc[NB][4];
@@ -101,7 +102,9 @@ __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __rest
 #pragma unroll
     for (int j = 1; j < NB; j++)
       if (spec.src[b] == (uint32_t)j) s = S[j];
-    sh::qm31 o = sh::qsub(sh::qmul(spec.w[b], s), spec.B[b]);
+    // dev_B: the constant terms in device memory (a driver that computed them on the device)
+    const sh::qm31 Bb = dev_B ? sh::qload(dev_B + 4 * b) : spec.B[b];
+    sh::qm31 o = sh::qsub(sh::qmul(spec.w[b], s), Bb);
     outs[4 * b + 0][r] = o.a.a;
     outs[4 * b + 1][r] = o.a.b;
     outs[4 * b + 2][r] = o.b.a;
@@ -340,10 +343,10 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
     uint32_t* const* ot = (uint32_t* const*)d_outs_tab;
     SpanGuard span(ctx, "quotients_accumulate_numerators");
     switch (nb) {
-      case 1: k_accum_num<1><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
-      case 2: k_accum_num<2><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
-      case 3: k_accum_num<3><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
-      default: k_accum_num<4><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec); break;
+      case 1: k_accum_num<1><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
+      case 2: k_accum_num<2><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
+      case 3: k_accum_num<3><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
+      default: k_accum_num<4><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
     }
     return sh_launch_check(ctx, "accumulate_numerators");
   };
@@ -406,6 +409,10 @@ int sh_static_scratch_init(sh_ctx* ctx) {
   return SH_OK;
 }
 
+static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_parts,
+                              const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
+                              uint32_t* q_scratch);
+
 // q_scratch: 4 << (L - b) words for the subdomain quotient, or nullptr to use the context's
 // (lazily allocated) scratch slot.
 static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_partials,
@@ -441,15 +448,25 @@ static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint3
     K.plog = partial_logs[k];
     K.pad[0] = K.pad[1] = K.pad[2] = 0;
   }
-  // subdomain: CircleDomain(Coset(half_odds(L-1).initial, s - 1)) (domain.rs:72-82 split)
-  const uint32_t half_init = sh::subgroup_gen(L + 1);
-  sh_point_tab half;
-  SH_TRY(sh_point_tab_make(ctx, half_init, s - 1, &half));
   void *d_acc, *d_parts;
   SH_TRY(sh_stage(ctx, kc.data(), kc.size() * sizeof(CombineK), &d_acc));
   std::vector<const void*> parts(4 * std::max<uint64_t>(n_acc, 1), nullptr);
   for (uint64_t i = 0; i < 4 * n_acc; i++) parts[i] = d_partials[i];
   SH_TRY(sh_stage(ctx, parts.data(), parts.size() * sizeof(void*), &d_parts));
+  return combine_and_extend(ctx, tw, (const uint32_t* const*)d_parts, (const CombineK*)d_acc, n_acc, L, b, d_out,
+                            q_scratch);
+}
+
+// The combine over the subdomain of log L - b (k_combine, per-accumulation constants in device
+// memory) and the quotient's extension: interpolate on the subdomain, evaluate on L.
+static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_parts,
+                              const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
+                              uint32_t* q_scratch) {
+  const uint32_t s = L - b;
+  // subdomain: CircleDomain(Coset(half_odds(L-1).initial, s - 1)) (domain.rs:72-82 split)
+  const uint32_t half_init = sh::subgroup_gen(L + 1);
+  sh_point_tab half;
+  SH_TRY(sh_point_tab_make(ctx, half_init, s - 1, &half));
   uint32_t* q = q_scratch;
   if (!q) {
     void* scr;
@@ -462,8 +479,8 @@ static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint3
   uint32_t* const* d_qtab = (uint32_t* const*)d_q;
   {
     SpanGuard span(ctx, "quotients_combine");
-    const uint32_t* const* pt = (const uint32_t* const*)d_parts;
-    const CombineK* kt = (const CombineK*)d_acc;
+    const uint32_t* const* pt = d_parts;
+    const CombineK* kt = d_kc;
     const uint32_t na = (uint32_t)n_acc;
     // accumulations per batch inversion: all of them up to 4 (every inverted norm is used); rows
     // per lane: 8 for one accumulation, 4 otherwise (the register budget of 3 waves per SIMD)
@@ -545,4 +562,203 @@ extern "C" int sh_accumulate(sh_ctx* ctx, uint32_t* const* d_col, const uint32_t
   k_accumulate<<<((1u << log_size) + 255) / 256, 256, 0, ctx->stream>>>((uint32_t* const*)d_a,
                                                                         (const uint32_t* const*)d_b, 1u << log_size);
   return sh_launch_check(ctx, "accumulate");
+}
+
+// ---------------------------------------------------------------- one device point
+// compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160) when every sampled
+// column has exactly one sample, all at one point p held in device memory with the sampled values
+// and the random coefficient (as a device channel leaves them): the line coefficients, the
+// per-size sums and the combine constants are computed on the device, so the whole driver runs
+// without the host reading p or the values.
+namespace {
+
+// Sorted position i (columns ascending by size, stable) takes column perm[i]: its sample gets the
+// random power of its place in the column order (build_samples_with_randomness_and_periodicity,
+// crates/stwo/src/core/pcs/quotients.rs:274-319) and complex_conjugate_line_coeffs (crates/stwo/
+// src/core/constraints.rs:119-134): a = conj(v) - v, c = conj(p.y) - p.y, b = v c - a p.y, each
+// times the power.  coef[i] = c; ba[2i] = b, ba[2i + 1] = a.
+__global__ void k_dpoint_terms(const uint32_t* __restrict__ point, const uint32_t* __restrict__ values,
+                               const uint32_t* __restrict__ powers, const uint32_t* __restrict__ perm, uint32_t n,
+                               uint32_t* __restrict__ coef, uint32_t* __restrict__ ba) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t col = perm[i];
+  const sh::qm31 py = sh::qload(point + 4), v = sh::qload(values + 4 * col), pw = sh::qload(powers + 4 * col);
+  const sh::qm31 av = sh::qsub(sh::qconj(v), v);
+  const sh::qm31 cv = sh::qsub(sh::qconj(py), py);
+  const sh::qm31 bv = sh::qsub(sh::qmul(v, cv), sh::qmul(av, py));
+  sh::qstore(sh::qmul(pw, cv), coef + 4 * i);
+  sh::qstore(sh::qmul(pw, bv), ba + 8 * i);
+  sh::qstore(sh::qmul(pw, av), ba + 8 * i + 4);
+}
+
+// One workgroup per size class k (sorted positions [off[k], off[k+1])): B[k] = sum of b,
+// FL[k] = sum of a.
+__global__ __launch_bounds__(256) void k_dpoint_sums(const uint32_t* __restrict__ ba, const uint32_t* __restrict__ off,
+                                                     uint32_t* __restrict__ B, uint32_t* __restrict__ FL) {
+  __shared__ uint32_t red[256][8];
+  const uint32_t k = blockIdx.x, t = threadIdx.x;
+  sh::qm31 sb = sh::qzero(), sa = sh::qzero();
+  for (uint32_t i = off[k] + t; i < off[k + 1]; i += 256) {
+    sb = sh::qadd(sb, sh::qload(ba + 8 * i));
+    sa = sh::qadd(sa, sh::qload(ba + 8 * i + 4));
+  }
+  sh::qstore(sb, red[t]);
+  sh::qstore(sa, red[t] + 4);
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh::qstore(sh::qadd(sh::qload(red[t]), sh::qload(red[t + w])), red[t]);
+      sh::qstore(sh::qadd(sh::qload(red[t] + 4), sh::qload(red[t + w] + 4)), red[t] + 4);
+    }
+    __syncthreads();
+  }
+  if (t < 4) {
+    B[4 * k + t] = red[0][t];
+    FL[4 * k + t] = red[0][4 + t];
+  }
+}
+
+// The combine constants of the one accumulation (the point, the first linear terms of every
+// size summed), as the host path builds them.
+__global__ void k_dpoint_combine_const(const uint32_t* __restrict__ point, const uint32_t* __restrict__ FL,
+                                       uint32_t n_fl, uint32_t plog, CombineK* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const sh::cm31 prx = {point[0], point[1]}, pix = {point[2], point[3]}, pry = {point[4], point[5]},
+                 piy = {point[6], point[7]};
+  const sh::cm31 c = sh::csub(sh::cmul(prx, piy), sh::cmul(pry, pix));
+  const sh::cm31 npiy = sh::cneg(piy);
+  sh::qm31 fl = sh::qzero();
+  for (uint32_t k = 0; k < n_fl; k++) fl = sh::qadd(fl, sh::qload(FL + 4 * k));
+  CombineK K;
+  K.c[0] = c.a;
+  K.c[1] = c.b;
+  K.npiy[0] = npiy.a;
+  K.npiy[1] = npiy.b;
+  K.pix[0] = pix.a;
+  K.pix[1] = pix.b;
+  uint32_t f[4];
+  sh::qstore(fl, f);
+  for (int w = 0; w < 4; w++) K.nfl[w] = sh::mneg(f[w]);
+  K.flim[0] = f[1];
+  K.flim[1] = f[3];
+  K.plog = plog;
+  K.pad[0] = K.pad[1] = K.pad[2] = 0;
+  *out = K;
+}
+
+}  // namespace
+
+extern "C" int sh_secure_powers_dev(sh_ctx* ctx, const uint32_t* d_felt, uint64_t n, int reversed, uint32_t* d_out);
+
+extern "C" int sh_fri_quotients_dpoint(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_cols,
+                                       const uint32_t* col_log_sizes, uint64_t n_cols, const uint32_t* d_point,
+                                       const uint32_t* d_values, const uint32_t* d_random_coeff,
+                                       uint32_t lifting_log_size, uint32_t log_blowup, uint32_t* const d_out[4]) {
+  if (!ctx || !tw || !d_out || !d_point || !d_random_coeff || (n_cols && (!d_cols || !col_log_sizes || !d_values)))
+    return SH_ERR_INVALID;
+  const uint32_t L = lifting_log_size, b = log_blowup;
+  SH_REQUIRE(ctx, n_cols > 0, "no sampled columns");
+  SH_REQUIRE(ctx, L >= b + 1 && L <= 30, "bad lifting log size %u / blowup %u", L, b);
+  SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(L + 1), L - 1, tw->root_initial_index, tw->root_log_size),
+             "domain of log size %u is not in the twiddle tower", L);
+  for (uint64_t i = 0; i < n_cols; i++)
+    SH_REQUIRE(ctx, col_log_sizes[i] >= b + 1 && col_log_sizes[i] <= L, "column log size %u not in [%u, %u]",
+               col_log_sizes[i], b + 1, L);
+  // columns by size, ascending (quotient_ops.rs:108-122), stable
+  std::vector<uint32_t> perm(n_cols);
+  for (uint64_t i = 0; i < n_cols; i++) perm[i] = (uint32_t)i;
+  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) { return col_log_sizes[x] < col_log_sizes[y]; });
+  std::vector<uint32_t> off{0}, logs;
+  std::vector<const void*> sorted(n_cols);
+  for (uint64_t i = 0; i < n_cols; i++) {
+    sorted[i] = d_cols[perm[i]];
+    if (i > 0 && col_log_sizes[perm[i]] != col_log_sizes[perm[i - 1]]) off.push_back((uint32_t)i);
+    if (i == 0 || col_log_sizes[perm[i]] != col_log_sizes[perm[i - 1]]) logs.push_back(col_log_sizes[perm[i]]);
+  }
+  off.push_back((uint32_t)n_cols);
+  const uint32_t n_cls = (uint32_t)logs.size();
+  // device work areas (pooled): powers, coefficients, (b, a) terms, B and FL per size, the combine
+  // constants, then one partial numerator per size (and a lifted sum when there are several)
+  const uint64_t w_pw = 4 * n_cols, w_coef = 4 * n_cols, w_ba = 8 * n_cols, w_B = 4 * n_cls, w_FL = 4 * n_cls;
+  const uint64_t w_kc = (sizeof(CombineK) + 3) / 4;
+  const uint64_t small_words = w_pw + w_coef + w_ba + w_B + w_FL + w_kc + 64;
+  uint32_t* work = nullptr;
+  SH_TRY(sh_dev_alloc(ctx, 4 * small_words, (void**)&work));
+  uint32_t* d_pw = work;
+  uint32_t* d_coef = d_pw + w_pw;
+  uint32_t* d_ba = d_coef + w_coef;
+  uint32_t* d_B = d_ba + w_ba;
+  uint32_t* d_FL = d_B + w_B;
+  CombineK* d_kc = (CombineK*)(((uintptr_t)(d_FL + w_FL) + 63) & ~(uintptr_t)63);
+  std::vector<uint32_t*> partial(n_cls, nullptr);
+  uint32_t* lifted = nullptr;
+  auto release = [&]() {
+    for (auto p : partial)
+      if (p) sh_dev_free(ctx, p);
+    if (lifted) sh_dev_free(ctx, lifted);
+    sh_dev_free(ctx, work);
+  };
+  int rc = SH_OK;
+  auto run = [&]() -> int {
+    void *d_perm, *d_off;
+    SH_TRY(sh_stage(ctx, perm.data(), 4 * n_cols, &d_perm));
+    SH_TRY(sh_stage(ctx, off.data(), 4 * off.size(), &d_off));
+    SH_TRY(sh_secure_powers_dev(ctx, d_random_coeff, n_cols, 0, d_pw));
+    {
+      SpanGuard span(ctx, "quotients_terms");
+      k_dpoint_terms<<<(unsigned)((n_cols + 255) / 256), 256, 0, ctx->stream>>>(
+          d_point, d_values, d_pw, (const uint32_t*)d_perm, (uint32_t)n_cols, d_coef, d_ba);
+      SH_TRY(sh_launch_check(ctx, "quotient_terms"));
+      k_dpoint_sums<<<n_cls, 256, 0, ctx->stream>>>(d_ba, (const uint32_t*)d_off, d_B, d_FL);
+      SH_TRY(sh_launch_check(ctx, "quotient_sums"));
+    }
+    // accumulate_numerators per size: out = S - B with S = sum_col c_col * col (one batch)
+    void* d_sorted;
+    SH_TRY(sh_stage(ctx, sorted.data(), sorted.size() * sizeof(void*), &d_sorted));
+    for (uint32_t k = 0; k < n_cls; k++) {
+      const uint32_t plog = logs[k] - b;
+      SH_TRY(sh_dev_alloc(ctx, 16ull << plog, (void**)&partial[k]));
+      const void* outp[4];
+      for (int c = 0; c < 4; c++) outp[c] = partial[k] + ((uint64_t)c << plog);
+      void* d_outs;
+      SH_TRY(sh_stage(ctx, outp, sizeof(outp), &d_outs));
+      OutSpec spec = {};
+      spec.n_out = 1;
+      spec.src[0] = 0;
+      spec.w[0] = sh::qone();
+      const uint32_t rows = 1u << plog;
+      SpanGuard span(ctx, "quotients_accumulate_numerators");
+      k_accum_num<1><<<(rows + 255) / 256, 256, 0, ctx->stream>>>(
+          (const uint32_t* const*)d_sorted + off[k], off[k + 1] - off[k], d_coef + 4ull * off[k], rows,
+          (uint32_t* const*)d_outs, spec, d_B + 4 * k);
+      SH_TRY(sh_launch_check(ctx, "accumulate_numerators"));
+    }
+    // one point: every size's partial lifted and summed (lift_and_accumulate, ascending sizes)
+    const uint32_t top = logs.back() - b;
+    const void* parts[4];
+    if (n_cls == 1) {
+      for (int c = 0; c < 4; c++) parts[c] = partial[0] + ((uint64_t)c << top);
+    } else {
+      SH_TRY(sh_dev_alloc(ctx, 16ull << top, (void**)&lifted));
+      std::vector<const uint32_t*> cols;
+      std::vector<uint32_t> plogs;
+      for (uint32_t k = 0; k < n_cls; k++) {
+        for (int c = 0; c < 4; c++) cols.push_back(partial[k] + ((uint64_t)c << (logs[k] - b)));
+        plogs.push_back(logs[k] - b);
+      }
+      uint32_t* o[4];
+      for (int c = 0; c < 4; c++) o[c] = lifted + ((uint64_t)c << top);
+      SH_TRY(sh_lift_and_accumulate(ctx, cols.data(), plogs.data(), n_cls, o));
+      for (int c = 0; c < 4; c++) parts[c] = o[c];
+    }
+    k_dpoint_combine_const<<<1, 64, 0, ctx->stream>>>(d_point, d_FL, n_cls, top, d_kc);
+    SH_TRY(sh_launch_check(ctx, "combine_const"));
+    void* d_parts;
+    SH_TRY(sh_stage(ctx, parts, sizeof(parts), &d_parts));
+    return combine_and_extend(ctx, tw, (const uint32_t* const*)d_parts, d_kc, 1, L, b, d_out, nullptr);
+  };
+  rc = run();
+  release();  // stream-ordered: later users of these buffers run after the kernels above
+  return rc;
 }

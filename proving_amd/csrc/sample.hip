@@ -35,8 +35,20 @@ struct EvalFactors {
 
 // b_lo[(i * np + pt)] = prod_{bit j of i} f[pt][j] for i < 2^sl, and
 // b_hi[(i * np + pt)] = prod_{bit j of i} f[pt][sl + j] for i < 2^nh (QM31 words).
+__device__ __forceinline__ void eval_table_entry(const EvalFactors& f, uint32_t np, uint32_t sl, uint32_t nh,
+                                                 uint32_t* __restrict__ b_lo, uint32_t* __restrict__ b_hi);
 __global__ __launch_bounds__(256) void k_eval_tables(EvalFactors f, uint32_t np, uint32_t sl, uint32_t nh,
                                                      uint32_t* __restrict__ b_lo, uint32_t* __restrict__ b_hi) {
+  eval_table_entry(f, np, sl, nh, b_lo, b_hi);
+}
+// The same with the factors in device memory (a point the device drew).
+__global__ __launch_bounds__(256) void k_eval_tables_dev(const EvalFactors* __restrict__ f, uint32_t np, uint32_t sl,
+                                                         uint32_t nh, uint32_t* __restrict__ b_lo,
+                                                         uint32_t* __restrict__ b_hi) {
+  eval_table_entry(*f, np, sl, nh, b_lo, b_hi);
+}
+__device__ __forceinline__ void eval_table_entry(const EvalFactors& f, uint32_t np, uint32_t sl, uint32_t nh,
+                                                 uint32_t* __restrict__ b_lo, uint32_t* __restrict__ b_hi) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n_lo = np << sl, n_hi = np << nh;
   if (t >= n_lo + n_hi) return;
@@ -281,6 +293,94 @@ extern "C" int sh_eval_at_points(sh_ctx* ctx, const uint32_t* const* d_coeffs, c
           for (int q = 0; q < 4; q++)
             out[4 * (idx[c] * n_points + p0 + pt) + q] = (uint32_t)(cls[4 * (c * np + pt) + q] % sh::P);
     }
+  }
+  return SH_OK;
+}
+
+namespace {
+// The factors of one point (slot 0) for polynomials of 2^n coefficients: the point doubled dbl
+// times (CirclePoint::repeated_double), then f_0 = y, f_1 = x, f_k = pi^(k-1)(x).
+__global__ void k_eval_factors(const uint32_t* __restrict__ point, uint32_t dbl, uint32_t n,
+                               EvalFactors* __restrict__ f) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sh::qm31 x = sh::qload(point), y = sh::qload(point + 4);
+  for (uint32_t k = 0; k < dbl; k++) {  // (x, y) -> (2x^2 - 1, 2xy)
+    const sh::qm31 xy = sh::qmul(x, y);
+    x = sh::qdouble_x(x);
+    y = sh::qadd(xy, xy);
+  }
+  for (uint32_t k = 0; k < n; k++) {
+    if (k == 0) {
+      sh::qstore(y, f->w[0][0]);
+    } else {
+      sh::qstore(x, f->w[0][k]);
+      x = sh::qdouble_x(x);
+    }
+  }
+}
+
+// values[idx[c]] = the reduced accumulator of class column c (one point).
+__global__ void k_eval_finalize(const unsigned long long* __restrict__ acc, uint32_t n, const uint32_t* __restrict__ idx,
+                                uint32_t* __restrict__ values) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * n) return;
+  values[4ull * idx[t >> 2] + (t & 3)] = lfinal(acc[t]);
+}
+}  // namespace
+
+// eval_at_point of every column at one QM31 point held in device memory (8 words the stream has
+// produced, e.g. drawn by a device channel), column i at the point doubled doublings[i] times
+// (compute_samples, crates/stwo/src/prover/pcs/mod.rs:176-222); d_values (device) receives 4
+// words per column.  Nothing returns to the host.
+extern "C" int sh_eval_at_dpoint(sh_ctx* ctx, const uint32_t* const* d_coeffs, const uint32_t* log_sizes,
+                                 const uint32_t* doublings, uint64_t n_cols, const uint32_t* d_point,
+                                 uint32_t* d_values) {
+  if (!ctx || (n_cols && (!d_coeffs || !log_sizes || !doublings || !d_point || !d_values))) return SH_ERR_INVALID;
+  if (n_cols == 0) return SH_OK;
+  std::vector<std::pair<uint32_t, uint32_t>> classes;  // (log size, doublings)
+  for (uint64_t i = 0; i < n_cols; i++) {
+    SH_REQUIRE(ctx, log_sizes[i] >= 1 && log_sizes[i] <= 30, "log size %u out of range", log_sizes[i]);
+    SH_REQUIRE(ctx, doublings[i] <= 31, "doubling count %u out of range", doublings[i]);
+    classes.push_back({log_sizes[i], doublings[i]});
+  }
+  std::sort(classes.begin(), classes.end());
+  classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
+  SpanGuard span(ctx, "eval_at_point");
+  for (auto& cl : classes) {
+    const uint32_t n = cl.first;
+    std::vector<const void*> ptrs;
+    std::vector<uint32_t> idx;
+    for (uint64_t i = 0; i < n_cols; i++)
+      if (log_sizes[i] == n && doublings[i] == cl.second) {
+        ptrs.push_back(d_coeffs[i]);
+        idx.push_back((uint32_t)i);
+      }
+    const uint32_t sl = std::min(n, kEvalLoLog), nh = n - sl;
+    void *d_ptrs, *d_idx, *d_tab, *d_cls;
+    SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_ptrs));
+    SH_TRY(sh_stage(ctx, idx.data(), idx.size() * 4, &d_idx));
+    const uint64_t lo_words = 4ull << sl, hi_words = 4ull << nh;
+    const uint64_t fac_bytes = (sizeof(EvalFactors) + 255) & ~255ull;
+    SH_TRY(sh_scratch_get(ctx, SCR_EVAL, fac_bytes + 4 * (lo_words + hi_words), &d_tab));
+    EvalFactors* d_fac = (EvalFactors*)d_tab;
+    uint32_t* d_lo = (uint32_t*)((uint8_t*)d_tab + fac_bytes);
+    uint32_t* d_hi = d_lo + lo_words;
+    k_eval_factors<<<1, 64, 0, ctx->stream>>>(d_point, cl.second, n, d_fac);
+    SH_TRY(sh_launch_check(ctx, "eval_factors"));
+    const uint32_t n_tab = (1u << sl) + (1u << nh);
+    k_eval_tables_dev<<<(n_tab + 255) / 256, 256, 0, ctx->stream>>>(d_fac, 1, sl, nh, d_lo, d_hi);
+    SH_TRY(sh_launch_check(ctx, "eval_tables"));
+    SH_TRY(sh_scratch_get(ctx, SCR_QUOT, 32ull * ptrs.size(), &d_cls));
+    SH_TRY(sh_memset(ctx, d_cls, 0, 32ull * ptrs.size()));
+    dim3 grid((unsigned)(((1ull << nh) + kEvalHiPerBlock - 1) / kEvalHiPerBlock), (unsigned)ptrs.size());
+    k_eval_points<1><<<grid, 256, 0, ctx->stream>>>((const uint32_t* const*)d_ptrs, n, (const uint32_t*)d_lo,
+                                                    (const uint32_t*)d_hi, (unsigned long long*)d_cls);
+    SH_TRY(sh_launch_check(ctx, "eval_at_points"));
+    const uint32_t nw = 4 * (uint32_t)ptrs.size();
+    k_eval_finalize<<<(nw + 255) / 256, 256, 0, ctx->stream>>>((const unsigned long long*)d_cls,
+                                                                (uint32_t)ptrs.size(), (const uint32_t*)d_idx,
+                                                                d_values);
+    SH_TRY(sh_launch_check(ctx, "eval_finalize"));
   }
   return SH_OK;
 }

@@ -17,12 +17,25 @@ import threading
 import numpy as np
 
 from . import field as F
-from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, FriParams, HashLayerSet, SecureColumnByCoords,
-                      _logs, _ptr_array, fri_prove, fri_quotients_arrays, tree_decommit)
+from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, DeviceBuffer, DeviceChannel, FriParams,
+                      HashLayerSet, SecureColumnByCoords, _logs, _ptr_array, fri_prove, fri_prove_dch,
+                      fri_quotients_arrays, fri_quotients_dpoint, tree_decommit)
 
 P = F.P
 OODS_POW_BITS = 16  # crates/stwo/src/core/proof_of_work.rs:8
 COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
+# Proofs keep their channel in device memory by default (DeviceChannel): from the trace root to
+# the FRI commit phase the host never waits for the device.  False: the host channel throughout.
+DEVICE_CHANNEL = True
+
+
+def _mix_root(channel, layers, root):
+    """MerkleChannel::mix_root: on the device from the tree's root layer when the channel is a
+    DeviceChannel (the root need not be known on the host), else on the host."""
+    if isinstance(channel, DeviceChannel):
+        channel.mix_words(layers[0].ptr, 8)
+    else:
+        channel.mix_root(root)
 
 
 class ConstraintsNotSatisfied(Exception):
@@ -175,8 +188,8 @@ class CommitmentSchemeProver:
                 empty = self.be._empty_tree = self.be.commit_lifted([], 0)
             layers, root = empty
         else:
-            layers, root = self.be.commit_lifted(evals, L)
-        channel.mix_root(root)
+            layers, root = self.be.commit_lifted(evals, L, want_root=not isinstance(channel, DeviceChannel))
+        _mix_root(channel, layers, root)
         self.trees.append(CommitmentTree(coeffs, evals, layers, root, L))
         return self.trees[-1]
 
@@ -192,8 +205,9 @@ class CommitmentSchemeProver:
             coeffs = ColumnSet(self.be.ctx, logs)
             ldes = ColumnSet(self.be.ctx, logs + b)
             layers = HashLayerSet(self.be.ctx, L)
-            _, _, _, root = self.be.commit_trace(evals, b, self.tw, coeffs=coeffs, ldes=ldes, layers=layers)
-            channel.mix_root(root)
+            _, _, _, root = self.be.commit_trace(evals, b, self.tw, coeffs=coeffs, ldes=ldes, layers=layers,
+                                                 want_root=not isinstance(channel, DeviceChannel))
+            _mix_root(channel, layers, root)
             self.trees.append(CommitmentTree(coeffs, ldes, layers, root, L))
             return self.trees[-1]
         coeffs = []
@@ -278,10 +292,67 @@ class CommitmentSchemeProver:
             # on a host thread while the FRI driver runs (the ctypes calls release the GIL);
             # its verdict is collected once the proof is assembled
             checker = _Background(check_samples, sampled_values)
-        params = FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
-                           self.fri["fold_step"])
-        fri_felts, queries = fri_prove(self.be, self.tw, channel, params, quotients, as_array=True)
+        fri_felts, queries = fri_prove(self.be, self.tw, channel, self.fri_params(), quotients, as_array=True)
         quotients.free()
+        return self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker)
+
+    def prove_values_dpoint(self, dch, channel, ws, oods_proof_of_work=None, check_samples=None):
+        """prove_values when every committed column is sampled once, at one point that the device
+        channel dch drew into ws (a ProofState): the samples are evaluated, mixed and combined
+        into the DEEP quotient on the device, and the FRI driver's commit-phase download brings
+        back the sampled values with the proof's other device-drawn words; the host channel
+        `channel` receives the final state.  check_samples as in prove_values (called with
+        (sampled values, the downloaded ProofState)).  Returns (felts, proof dict)."""
+        L = self.trees[-1].lifting_log_size
+        b = self.fri["log_blowup_factor"]
+        ctx = self.be.ctx
+        trees = [t for t in self.trees if len(t.evals)]
+        n_per_tree = [len(t.evals) for t in self.trees]
+        coeff_ptrs = np.concatenate([_ptr_array(t.coeffs) for t in trees])
+        coeff_logs = np.concatenate([np.asarray(_logs(t.coeffs), np.uint32) for t in trees])
+        eval_ptrs = np.concatenate([_ptr_array(t.evals) for t in trees])
+        eval_logs = np.concatenate([np.asarray(_logs(t.evals), np.uint32) for t in trees])
+        n_samples = len(eval_ptrs)
+        assert n_samples == ws.n_samples
+        # roots the host has not seen (mixed on the device) come back with the values
+        unknown = [i for i, t in enumerate(self.trees) if t.root is None]
+        assert len(unknown) <= ws.n_roots
+        for k, i in enumerate(unknown):
+            ctx.check(ctx.lib.sh_copy(ctx.h, ws.ptr(ws.ROOTS + 8 * k), self.trees[i].layers[0].ptr, 32), "sh_copy")
+        # compute_samples at the point doubled down to each column's domain
+        self.be.eval_at_dpoint(ColumnRefs(ctx, coeff_ptrs, coeff_logs), L - eval_logs, ws.ptr(ws.Z),
+                               ws.ptr(ws.VALS))
+        dch.mix_words(ws.ptr(ws.VALS), 4 * n_samples)
+        dch.draw_secure_felts(1, ws.ptr(ws.RC))
+        quotients = fri_quotients_dpoint(self.be, self.tw, ColumnRefs(ctx, eval_ptrs, eval_logs), ws.ptr(ws.Z),
+                                         ws.ptr(ws.VALS), ws.ptr(ws.RC), L, b)
+        fri_felts, queries, host = fri_prove_dch(self.be, self.tw, dch, channel, self.fri_params(), quotients,
+                                                 extra=(ws.ptr(0), ws.download_words))
+        quotients.free()
+        ws.read(host)
+        for k, i in enumerate(unknown):
+            self.trees[i].root = host[ws.ROOTS + 8 * k:ws.ROOTS + 8 * k + 8].tobytes()
+        values = ws.values
+        vl = values.tolist()
+        sampled_values, blocks, c = [], [], 0
+        for n in n_per_tree:
+            sampled_values.append([[vl[c + i]] for i in range(n)])
+            blocks.append(([1] * n, values[c:c + n]))
+            c += n
+        checker = _Background(check_samples, sampled_values, ws) if check_samples is not None else None
+        if oods_proof_of_work is None:
+            oods_proof_of_work = ws.nonce
+        return self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker)
+
+    def fri_params(self):
+        b = self.fri["log_blowup_factor"]
+        return FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
+                         self.fri["fold_step"])
+
+    def _decommit_and_serialize(self, fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker):
+        """The query phase of prove_values (crates/stwo/src/prover/pcs/mod.rs:286-315) after the FRI
+        proof: every tree decommitted at the queries, the check's verdict, the proof's felts."""
+        L = self.trees[-1].lifting_log_size
         pp_height = self.trees[0].lifting_log_size
         pp_positions = prepare_preprocessed_query_positions(queries, L, pp_height)
         queried_values, decommitments = [], []
@@ -297,6 +368,47 @@ class CommitmentSchemeProver:
                  "decommitments": decommitments, "queried_values": queried_values, "fri_felts": fri_felts,
                  "query_positions": queries}
         return serialize_commitment_scheme_proof(proof, as_array=True), proof
+
+
+class ProofState:
+    """Device words a proof's device channel fills, read back in one download with the FRI
+    commit phase: the composition coefficient, the OODS point, the DEEP random coefficient, the
+    OODS proof-of-work nonce, the roots mixed on the device, the sampled values; then, not read
+    back, the constraint powers."""
+    ALPHA, Z, RC, CTL, ROOTS = 0, 4, 12, 16, 20
+
+    def __init__(self, ctx, n_samples, n_powers, n_roots=2):
+        self.n_samples, self.n_roots = n_samples, n_roots
+        self.VALS = self.ROOTS + 8 * n_roots
+        self.POW = self.VALS + 4 * n_samples
+        self.download_words = self.POW
+        self.buf = DeviceBuffer(ctx, 4 * (self.POW + 4 * max(1, n_powers)))
+        self.host = None
+
+    def ptr(self, word):
+        return self.buf.ptr + 4 * word
+
+    def read(self, host):
+        self.host = host
+
+    @property
+    def alpha(self):
+        return [int(x) for x in self.host[self.ALPHA:self.ALPHA + 4]]
+
+    @property
+    def oods_point(self):
+        return [int(x) for x in self.host[self.Z:self.Z + 8]]
+
+    @property
+    def nonce(self):
+        return int(self.host[self.CTL]) | (int(self.host[self.CTL + 1]) << 32)
+
+    @property
+    def values(self):
+        return self.host[self.VALS:self.VALS + 4 * self.n_samples].reshape(-1, 4)
+
+    def free(self):
+        self.buf.free()
 
 
 def prepare_preprocessed_query_positions(query_positions, max_log_size, pp_max_log_size):
@@ -370,17 +482,26 @@ def serialize_commitment_scheme_proof(proof, as_array=False):
     return out if as_array else out.tolist()
 
 
-def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None):
+def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None,
+                         device_channel=None):
     """First phase of test_wide_fib_prove_with_blake's prover (crates/examples/src/wide_fibonacci/
     mod.rs:183-210): config with an empty, unlifted preprocessed tree; commit it and the trace.
     committed_trace may carry a CommitmentTree built elsewhere (batched across proofs); its root is
-    mixed exactly as a fresh commit would be.  Returns (scheme, channel)."""
+    mixed exactly as a fresh commit would be.  With the device channel (DEVICE_CHANNEL unless
+    device_channel says otherwise) a fresh trace's root is mixed on the device, and the host
+    channel is behind until the FRI commit phase returns.  Returns (scheme, channel)."""
     b = fri["log_blowup_factor"]
     channel = channel or Channel(backend.lib)
     scheme = CommitmentSchemeProver(backend, twiddles, fri, log_n + b, 0)
-    scheme.commit_evals([], channel)                       # preprocessed trace (empty)
+    scheme.device_channel = DEVICE_CHANNEL if device_channel is None else bool(device_channel)
+    scheme.dch = None
+    scheme.commit_evals([], channel)                       # preprocessed trace (empty; root cached)
     if committed_trace is None:
-        scheme.commit_evals(trace, channel)                # trace
+        if scheme.device_channel:
+            scheme.dch = DeviceChannel(backend, channel)
+            scheme.commit_evals(trace, scheme.dch)         # trace, root mixed on the device
+        else:
+            scheme.commit_evals(trace, channel)            # trace
     else:
         assert committed_trace.lifting_log_size == log_n + b
         channel.mix_root(committed_trace.root)
@@ -393,6 +514,8 @@ def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
     polynomial (evaluated on the committed evaluations, interpolated, split at the middle,
     committed as 8 coordinate polynomials), the OODS proof of work and point, prove_values.
     Runs on scheme.be (which may be another context than the one that committed the trace)."""
+    if getattr(scheme, "device_channel", False):
+        return _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n)
     backend, twiddles, fri = scheme.be, scheme.tw, scheme.fri
     ctx = backend.ctx
     b = fri["log_blowup_factor"]
@@ -427,13 +550,58 @@ def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
     return felts, proof
 
 
-def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None):
+def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
+    """finish_wide_fibonacci with the channel in device memory: the composition coefficient and
+    its powers, the composition root, the OODS proof of work and point, the sampled values and the
+    DEEP coefficient are drawn, mixed and used on the device; the host reads them back with the
+    FRI commit phase (prove_values_dpoint).  Same proof, byte for byte."""
+    backend, twiddles, fri = scheme.be, scheme.tw, scheme.fri
+    ctx = backend.ctx
+    b = fri["log_blowup_factor"]
+    trace_tree = scheme.trees[1]
+    # the trace committed elsewhere (batched) was mixed on the host: the device channel starts here
+    dch = scheme.dch if scheme.dch is not None and scheme.dch.be is backend else DeviceChannel(backend, channel)
+    m = n_cols - 2
+    ws = ProofState(ctx, n_cols + 8, m)
+    # compute_composition_polynomial (crates/stwo/src/prover/air/component_prover.rs:103-131)
+    dch.draw_secure_felts(1, ws.ptr(ws.ALPHA))
+    backend.secure_powers_dev(ws.ptr(ws.ALPHA), m, True, ws.ptr(ws.POW))  # ColumnAccumulator order
+    eval_log = log_n + 1
+    log_expansion = b - 1
+    assert log_expansion == 0, "only log_blowup_factor 1 (SubDomain {log_expansion: 0}) is driven here"
+    acc = SecureColumnByCoords(ctx, eval_log)
+    for c in acc.columns:
+        ctx.check(ctx.lib.sh_memset(ctx.h, c.ptr, 0, c.nbytes), "memset")
+    backend.wide_fibonacci_composition_dev(trace_tree.evals, log_n, eval_log, ws.ptr(ws.POW), acc, log_expansion)
+    backend.interpolate_columns(acc.columns, twiddles)
+    halves = [backend.split_at_mid(c) for c in acc.columns]
+    scheme.commit_polys([h[0] for h in halves] + [h[1] for h in halves], dch)
+    # OODS proof of work and point (crates/stwo/src/prover/mod.rs:74-81)
+    dch.grind(OODS_POW_BITS, ws.ptr(ws.CTL))
+    dch.draw_random_point(ws.ptr(ws.Z))
+
+    def check(sv, st):
+        alpha = st.alpha
+        return check_wide_fib_oods(sv, alpha, st.oods_point, log_n, b, backend.generate_secure_powers(alpha, m))
+
+    felts, proof = scheme.prove_values_dpoint(dch, channel, ws, check_samples=check)
+    proof["composition_random_coeff"] = ws.alpha
+    proof["oods_point"] = ws.oods_point
+    ws.free()
+    if dch is not scheme.dch:
+        dch.free()
+    return felts, proof
+
+
+def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, device_channel=None):
     """test_wide_fib_prove_with_blake's prover side (crates/examples/src/wide_fibonacci/mod.rs:
     183-220) through prove_ex: empty preprocessed tree, the trace tree, then
     finish_wide_fibonacci.  trace: device columns (BaseColumn, bit-reversed evaluations on
-    CanonicCoset(log_n)).  Returns (felts, proof dict, the channel)."""
+    CanonicCoset(log_n)).  device_channel: keep the channel in device memory (default
+    DEVICE_CHANNEL).  Returns (felts, proof dict, the channel in its final state)."""
     assert fri["log_blowup_factor"] >= 1, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
-    scheme, channel = start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel)
+    scheme, channel = start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel,
+                                           device_channel=device_channel)
     felts, proof = finish_wide_fibonacci(scheme, channel, len(trace), log_n)
     return felts, proof, channel
 

@@ -25,15 +25,36 @@ def params(pow_bits=4, last=0, n_queries=3, fold_step=1):
     (4, 4, params()), (12, 6, params(6, 1, 9)), (32, 10, params(8, 2, 20)), (64, 13, params(10, 0, 70)),
     # the fold step of the reference's Cairo proofs (packed FRI leaves)
     (16, 9, params(6, 0, 30, 3)), (8, 7, params(4, 1, 12, 2))])
-def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n, fri):
+@pytest.mark.parametrize("device_channel", [True, False], ids=["device_channel", "host_channel"])
+def test_wide_fib_proof_matches_oracle_and_verifies(ctx, backend, n_cols, log_n, fri, device_channel):
+    """The proof with the channel in device memory and with the host channel: both byte-identical
+    to the oracle's."""
     trace = wide_fib_trace(n_cols, log_n, 100 + n_cols)
     tw, itw = O.canonic_twiddles(log_n + 1)
     exp_felts, _ = PR.wide_fib_prove(trace, log_n, fri, tw, itw)
     tree = backend.precompute_canonic_twiddles(log_n + 1)
     cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
-    felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri)
+    felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, device_channel=device_channel)
     assert felts.tolist() == exp_felts
     assert PR.wide_fib_verify(felts.tolist(), n_cols, log_n, fri)
+
+
+def test_device_channel_final_state_and_proof_fields(ctx, backend):
+    """The device-channel proof leaves the caller's channel where the host-channel proof does, and
+    reports the same composition coefficient, OODS point, nonce and sampled values."""
+    n_cols, log_n, fri = 12, 8, params(6, 1, 9)
+    trace = wide_fib_trace(n_cols, log_n, 77)
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    out = {}
+    for dc in (True, False):
+        cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
+        felts, proof, ch = prove_wide_fibonacci(backend, tree, cols, log_n, fri, device_channel=dc)
+        out[dc] = (felts.tolist(), ch.digest, ch.s.n_draws, proof)
+    (fd, dd, nd, pd), (fh, dh, nh, ph) = out[True], out[False]
+    assert fd == fh and (dd, nd) == (dh, nh)
+    for k in ("composition_random_coeff", "oods_point", "oods_proof_of_work", "sampled_values", "commitments",
+              "query_positions"):
+        assert pd[k] == ph[k], k
 
 
 def test_wide_fib_bad_trace_rejected(ctx, backend):
@@ -44,8 +65,10 @@ def test_wide_fib_bad_trace_rejected(ctx, backend):
     trace[7][100] = (int(trace[7][100]) + 1) % P
     fri = params()
     tree = backend.precompute_canonic_twiddles(log_n + 1)
-    with pytest.raises(ConstraintsNotSatisfied):
-        prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri)
+    for device_channel in (True, False):
+        with pytest.raises(ConstraintsNotSatisfied):
+            prove_wide_fibonacci(backend, tree, [BaseColumn.from_cpu(ctx, c) for c in trace], log_n, fri,
+                                 device_channel=device_channel)
 
 
 def test_wide_fib_full_size_verifies(ctx, backend):
