@@ -305,6 +305,18 @@ int sh_wide_fibonacci_composition_dev(sh_ctx *ctx, const uint32_t *const *d_cols
                                       uint32_t trace_log, uint32_t eval_log, uint32_t log_expansion,
                                       const uint32_t *d_powers, uint32_t *const d_accum[4]);
 
+/* The sanity check closing prove_ex (crates/stwo/src/prover/mod.rs:118-132) for the
+ * wide-Fibonacci AIR, on device data: StarkProof::extract_composition_oods_eval
+ * (crates/stwo/src/core/proof.rs:27-53) of the 8 composition samples against the constraints
+ * evaluated on the n_cols trace samples at the OODS point d_point (8 words), the constraints
+ * weighted by d_powers_rev[j] = alpha^(n_cols - 3 - j) and divided by the vanishing polynomial
+ * of CanonicCoset(log_n).  d_values: the n_cols trace values then the 8 composition values.
+ * d_out (9 device words): 1 if they differ (ProvingError::ConstraintsNotSatisfied) else 0, then
+ * the two values. */
+int sh_wide_fibonacci_oods_check(sh_ctx *ctx, const uint32_t *d_values, uint64_t n_cols,
+                                 const uint32_t *d_powers_rev, const uint32_t *d_point, uint32_t log_n,
+                                 uint32_t *d_out);
+
 /* generate_trace of the wide-Fibonacci AIR (crates/examples/src/wide_fibonacci/mod.rs:27-48)
  * on the device from its two input columns d_c0, d_c1 (2^log_size rows): d_cols[0..n_cols)
  * receive c0, c1 and c_k = c_{k-2}^2 + c_{k-1}^2 (bit-reversed evaluations on

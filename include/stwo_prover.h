@@ -91,14 +91,37 @@ typedef struct {
 int sh_fri_prove(sh_ctx *ctx, const sh_twiddles *tw, sh_channel *ch, const sh_fri_params *params,
                  const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
                  uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);
-/* sh_fri_prove on a proof whose channel is the device channel d_ch: the driver starts from it
- * without the host reading it, and ch_out receives the channel's state at the end of the proof
- * (as sh_fri_prove leaves ch).  n_extra words at d_extra come back to h_extra with the commit
- * phase's own download (the proof's device-drawn values, read in the same round trip). */
+/* A committed tree opened at the proof's queries (CommitmentTreeProver::decommit,
+ * crates/stwo/src/prover/pcs/mod.rs:392-398, as sh_tree_decommit): inputs as sh_tree_decommit's;
+ * preprocessed: the positions are prepare_preprocessed_query_positions of the queries
+ * (crates/stwo/src/core/pcs/utils.rs:185-203), else the queries themselves.  Outputs: values
+ * (n_cols * n_positions words, room for n_cols * n_queries), witness (up to witness_cap
+ * hashes), n_positions, n_witness. */
+typedef struct {
+  const uint32_t *const *d_cols;
+  const uint32_t *log_sizes;
+  uint64_t n_cols;
+  const uint8_t *const *d_layers;
+  uint32_t L;
+  uint32_t preprocessed;
+  uint32_t *values;
+  uint8_t *witness;
+  uint64_t witness_cap;
+  uint64_t n_positions;
+  uint64_t n_witness;
+} sh_tree_opening;
+
+/* sh_fri_prove on a proof whose channel is the device channel d_ch (the rest of prove_values,
+ * crates/stwo/src/prover/pcs/mod.rs:265-315): the driver starts from it without the host reading
+ * it, and ch_out receives the channel's state at the end of the proof (as sh_fri_prove leaves
+ * ch).  n_extra words at d_extra come back to h_extra with the commit phase's own download (the
+ * proof's device-drawn values, read in the same round trip), and the n_trees trees are opened at
+ * the queries in the FRI decommitment's gather (one more round trip in all). */
 int sh_fri_prove_dch(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *d_ch, sh_channel *ch_out,
-                     const uint32_t *d_extra, uint64_t n_extra, uint32_t *h_extra, const sh_fri_params *params,
-                     const uint32_t *const column[4], uint32_t log_size, uint64_t *felts, uint64_t cap,
-                     uint64_t *n_felts, uint32_t *query_positions, uint64_t *n_query_positions);
+                     const uint32_t *d_extra, uint64_t n_extra, uint32_t *h_extra, sh_tree_opening *trees,
+                     uint32_t n_trees, const sh_fri_params *params, const uint32_t *const column[4],
+                     uint32_t log_size, uint64_t *felts, uint64_t cap, uint64_t *n_felts, uint32_t *query_positions,
+                     uint64_t *n_query_positions);
 
 /* ---------------------------------------------------------------- DEEP quotients */
 /* compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160): columns are the

@@ -45,6 +45,7 @@ SIGNATURES = {
     "sh_eval_at_dpoint": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp]),
     "sh_wide_fibonacci_composition": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
     "sh_wide_fibonacci_composition_dev": (c_int, [c_vp, c_vp, c_u64, c_u32, c_u32, c_u32, c_vp, c_vp]),
+    "sh_wide_fibonacci_oods_check": (c_int, [c_vp, c_vp, c_u64, c_vp, c_vp, c_u32, c_vp]),
     "sh_wide_fibonacci_trace": (c_int, [c_vp, c_vp, c_vp, c_u32, c_u64, c_vp]),
     "sh_merkle_build_leaves": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32, c_vp]),
     "sh_merkle_build_next_layer": (c_int, [c_vp, c_vp, c_u32, c_vp]),
@@ -99,8 +100,8 @@ SIGNATURES = {
     "sh_fri_prove": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),
     "sh_fri_quotients": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
     "sh_fri_quotients_dpoint": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
-    "sh_fri_prove_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64,
-                                 c_u64p, c_vp, c_u64p]),
+    "sh_fri_prove_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32,
+                                 c_vp, c_u64, c_u64p, c_vp, c_u64p]),
 }
 
 _lib = None

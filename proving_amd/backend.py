@@ -672,6 +672,12 @@ class _Ops:
                   "wide_fibonacci_composition_dev")
         return accum
 
+    def wide_fibonacci_oods_check(self, values_ptr, n_cols, powers_rev_ptr, point_ptr, log_n, out_ptr):
+        """prove_ex's closing sanity check on device data (9 words at out_ptr: mismatch flag, the
+        composition value, the constraint value)."""
+        self._chk(self.lib.sh_wide_fibonacci_oods_check(self.h, values_ptr, n_cols, powers_rev_ptr, point_ptr,
+                                                        log_n, out_ptr), "wide_fibonacci_oods_check")
+
     def eval_at_dpoint(self, coeffs, doublings, point_ptr, values_ptr):
         """compute_samples at one device point: column i at the point doubled doublings[i] times
         into values_ptr[4i..4i+4] (device)."""
@@ -858,23 +864,60 @@ def fri_prove(backend, twiddles, channel, params, column, cap=1 << 22, as_array=
     return out, qp[:nq.value].tolist()
 
 
-def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extra=None, cap=1 << 22):
+class TreeOpening(ctypes.Structure):
+    """sh_tree_opening: a committed tree opened at the proof's queries by sh_fri_prove_dch."""
+    _fields_ = [("d_cols", ctypes.c_void_p), ("log_sizes", ctypes.c_void_p), ("n_cols", ctypes.c_uint64),
+                ("d_layers", ctypes.c_void_p), ("L", ctypes.c_uint32), ("preprocessed", ctypes.c_uint32),
+                ("values", ctypes.c_void_p), ("witness", ctypes.c_void_p), ("witness_cap", ctypes.c_uint64),
+                ("n_positions", ctypes.c_uint64), ("n_witness", ctypes.c_uint64)]
+
+
+def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extra=None, trees=(), cap=1 << 22):
     """fri_prove on the device channel: channel_out (Channel) receives the final state.  extra:
-    (device pointer, n words) read back with the commit phase's download.  Returns (felts as a
-    uint64 array, sorted query positions, the extra words)."""
+    (device pointer, n words) read back with the commit phase's download.  trees: (cols, layers,
+    preprocessed) of committed trees to open at the queries in the same gather.  Returns (felts as
+    a uint64 array, sorted query positions, the extra words, [(values (n_cols, n_pos), witness
+    (n, 32) uint8) per tree])."""
     buf = getattr(backend, "_fri_felt_buf", None)
     if buf is None or len(buf) < cap:
         buf = backend._fri_felt_buf = np.empty(cap, np.uint64)
     nf = ctypes.c_uint64()
-    qp = np.empty(max(1, params.n_queries), np.uint32)
+    nqmax = max(1, params.n_queries)
+    qp = np.empty(nqmax, np.uint32)
     nq = ctypes.c_uint64()
     e_ptr, e_n = extra if extra is not None else (None, 0)
     h_extra = np.empty(max(1, e_n), np.uint32)
+    ops = (TreeOpening * max(1, len(trees)))()
+    keep = []
+    for o, (cols, layers, pre) in zip(ops, trees):
+        L = len(layers) - 1
+        n = len(cols)
+        values = np.empty((n, nqmax), np.uint32)
+        cap_w = max(1, nqmax * max(1, L))
+        wit = np.empty((cap_w, 32), np.uint8)
+        ptrs, logs = _ptrs(cols), _u32s(_logs(cols))
+        lp = _ptrs(layers) if L >= 0 else None
+        keep.append((values, wit, ptrs, logs, lp))
+        o.d_cols = ctypes.cast(ptrs, ctypes.c_void_p) if n else None
+        o.log_sizes = ctypes.cast(logs, ctypes.c_void_p) if n else None
+        o.n_cols = n
+        o.d_layers = ctypes.cast(lp, ctypes.c_void_p) if lp is not None else None
+        o.L = max(L, 0)
+        o.preprocessed = int(bool(pre))
+        o.values = values.ctypes.data
+        o.witness = wit.ctypes.data
+        o.witness_cap = cap_w
     backend._chk(backend.lib.sh_fri_prove_dch(backend.h, twiddles.h, dchannel.ptr, ctypes.byref(channel_out.s),
-                                              e_ptr, e_n, h_extra.ctypes.data, ctypes.byref(params),
-                                              _ptrs(column.columns), column.log_size, buf.ctypes.data, cap,
-                                              ctypes.byref(nf), qp.ctypes.data, ctypes.byref(nq)), "fri_prove_dch")
-    return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:e_n]
+                                              e_ptr, e_n, h_extra.ctypes.data, ops, len(trees),
+                                              ctypes.byref(params), _ptrs(column.columns), column.log_size,
+                                              buf.ctypes.data, cap, ctypes.byref(nf), qp.ctypes.data,
+                                              ctypes.byref(nq)), "fri_prove_dch")
+    openings = []
+    for o, (values, wit, *_) in zip(ops, keep):
+        k = o.n_positions
+        v = np.ascontiguousarray(values.reshape(-1)[:o.n_cols * k].reshape(o.n_cols, k))
+        openings.append((v, wit[:o.n_witness]))
+    return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:e_n], openings
 
 
 def fri_quotients_dpoint(backend, twiddles, columns, point_ptr, values_ptr, random_coeff_ptr, lifting_log_size,

@@ -38,6 +38,10 @@ struct sh_ctx {
   uint8_t* d_ring = nullptr;
   uint64_t ring_size = 0, ring_off = 0;
   std::unordered_map<std::string, uint64_t> table_cache;
+  // pinned host buffer for the protocol drivers' bulk tables and read-backs (grown on demand;
+  // every user waits for its copies before returning)
+  uint8_t* h_xfer = nullptr;
+  uint64_t h_xfer_size = 0;
   // staged tables whose copy is not known to be complete: ring offset -> (event after the copy,
   // stream that copied); a reuse from another stream waits on the event instead of the host
   struct StageEv {
@@ -113,6 +117,8 @@ int sh_fail(sh_ctx* ctx, int code, const char* fmt, ...);
 int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev);
 // The i-th fork/join event of the context (created on first use).
 hipEvent_t sh_sync_event(sh_ctx* ctx, size_t i);
+// The context's pinned transfer buffer, at least `bytes` long (see sh_ctx::h_xfer).
+int sh_host_xfer(sh_ctx* ctx, uint64_t bytes, uint8_t** out);
 // Ensures scratch buffer `slot` holds at least `bytes` and returns it.
 int sh_scratch_get(sh_ctx* ctx, int slot, uint64_t bytes, void** out);
 // The context's auxiliary stream (created on first use).

@@ -189,3 +189,82 @@ extern "C" int sh_wide_fibonacci_composition_dev(sh_ctx* ctx, const uint32_t* co
                                                  const uint32_t* d_powers, uint32_t* const d_accum[4]) {
   return composition(ctx, d_cols, n_cols, trace_log, eval_log, log_expansion, d_powers, true, d_accum);
 }
+
+// ---------------------------------------------------------------- OODS sanity check
+// The check closing prove_ex (crates/stwo/src/prover/mod.rs:118-132) for the wide-Fibonacci AIR,
+// on the device: the composition value at the OODS point z rebuilt from its 8 sampled
+// coordinates (StarkProof::extract_composition_oods_eval, crates/stwo/src/core/proof.rs:27-53:
+// left + x * right, x = z.x doubled log_n - 1 times, each half from_partial_evals) against the
+// constraints on the sampled trace values (Horner in alpha, i.e. constraint j weighted by
+// alpha^(m-1-j), times 1 / coset_vanishing(CanonicCoset(log_n), z),
+// crates/constraint-framework/src/component.rs:220-243).  values: n_cols trace values then the 8
+// composition values (4 words each); powers_rev[j] = alpha^(m-1-j).  out: [mismatch flag, lhs (4
+// words), rhs (4 words)].  One workgroup.
+namespace {
+
+__device__ sh::qm31 from_partial_evals(const uint32_t* e) {
+  const sh::qm31 bi = {{0, 1}, {0, 0}}, bu = {{0, 0}, {1, 0}}, biu = {{0, 0}, {0, 1}};
+  sh::qm31 r = sh::qload(e);
+  r = sh::qadd(r, sh::qmul(sh::qload(e + 4), bi));
+  r = sh::qadd(r, sh::qmul(sh::qload(e + 8), bu));
+  return sh::qadd(r, sh::qmul(sh::qload(e + 12), biu));
+}
+
+__device__ sh::qm31 double_x_times(sh::qm31 x, uint32_t n) {
+  for (uint32_t k = 0; k < n; k++) {
+    const sh::qm31 xx = sh::qmul(x, x);
+    x = sh::qsub(sh::qadd(xx, xx), sh::qone());
+  }
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_wide_fib_oods_check(const uint32_t* __restrict__ values, uint32_t n_cols,
+                                                             const uint32_t* __restrict__ powers_rev,
+                                                             const uint32_t* __restrict__ point, uint32_t shift_x,
+                                                             uint32_t shift_y, uint32_t log_n,
+                                                             uint32_t* __restrict__ out) {
+  __shared__ uint32_t red[256][4];
+  const uint32_t t = threadIdx.x, m = n_cols - 2;
+  sh::qm31 acc = sh::qzero();
+  for (uint32_t j = t; j < m; j += 256) {
+    const sh::qm31 a = sh::qload(values + 4 * j), b = sh::qload(values + 4 * (j + 1)),
+                   c = sh::qload(values + 4 * (j + 2));
+    const sh::qm31 cons = sh::qsub(c, sh::qadd(sh::qmul(a, a), sh::qmul(b, b)));
+    acc = sh::qadd(acc, sh::qmul(sh::qload(powers_rev + 4 * j), cons));
+  }
+  sh::qstore(acc, red[t]);
+  __syncthreads();
+  for (uint32_t w = 128; w > 0; w >>= 1) {
+    if (t < w) sh::qstore(sh::qadd(sh::qload(red[t]), sh::qload(red[t + w])), red[t]);
+    __syncthreads();
+  }
+  if (t != 0) return;
+  const sh::qm31 zx = sh::qload(point), zy = sh::qload(point + 4);
+  // coset_vanishing (crates/stwo/src/core/constraints.rs:12-35): x of z - initial + step / 2
+  const sh::qm31 px = sh::qsub(sh::qmulm(zx, shift_x), sh::qmulm(zy, shift_y));
+  const sh::qm31 rhs = sh::qmul(sh::qinv(double_x_times(px, log_n - 1)), sh::qload(red[0]));
+  const uint32_t* comp = values + 4 * n_cols;
+  const sh::qm31 lhs =
+      sh::qadd(from_partial_evals(comp), sh::qmul(double_x_times(zx, log_n - 1), from_partial_evals(comp + 16)));
+  out[0] = sh::qeq(lhs, rhs) ? 0u : 1u;
+  sh::qstore(lhs, out + 1);
+  sh::qstore(rhs, out + 5);
+}
+
+}  // namespace
+
+extern "C" int sh_wide_fibonacci_oods_check(sh_ctx* ctx, const uint32_t* d_values, uint64_t n_cols,
+                                            const uint32_t* d_powers_rev, const uint32_t* d_point, uint32_t log_n,
+                                            uint32_t* d_out) {
+  if (!ctx || !d_values || !d_point || !d_out || (n_cols > 2 && !d_powers_rev)) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, n_cols >= 2 && n_cols < (1ull << 24), "column count %llu out of range",
+             (unsigned long long)n_cols);
+  SH_REQUIRE(ctx, log_n >= 1 && log_n <= 30, "trace log size %u out of range", log_n);
+  // CanonicCoset(log_n).coset: initial index 2^(31 - (log_n + 1)), step 2^(31 - log_n)
+  const uint32_t init = 1u << (31 - (log_n + 1)), half_step = (1u << (31 - log_n)) >> 1;
+  const sh::mpt shift = sh::index_to_point_host((uint32_t)((half_step - init) & 0x7fffffffu));
+  SpanGuard span(ctx, "composition");
+  k_wide_fib_oods_check<<<1, 256, 0, ctx->stream>>>(d_values, (uint32_t)n_cols, d_powers_rev, d_point, shift.x,
+                                                     shift.y, log_n, d_out);
+  return sh_launch_check(ctx, "wide_fib_oods_check");
+}

@@ -4,6 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 
@@ -84,6 +85,7 @@ extern "C" int sh_ctx_destroy(sh_ctx* ctx) {
   for (auto& kv : ctx->pool)
     for (void* p : kv.second) hipFree(p);
   for (auto& kv : ctx->pool_live) hipFree(kv.first);
+  if (ctx->h_xfer) hipHostFree(ctx->h_xfer);
   if (ctx->d_ring) hipFree(ctx->d_ring);
   if (ctx->h_ring) hipHostFree(ctx->h_ring);
   if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
@@ -171,6 +173,19 @@ int sh_stage(sh_ctx* ctx, const void* host, uint64_t bytes, void** dev) {
   }
   ctx->ring_off += need;
   *dev = d;
+  return SH_OK;
+}
+
+int sh_host_xfer(sh_ctx* ctx, uint64_t bytes, uint8_t** out) {
+  if (ctx->h_xfer_size < bytes) {
+    if (ctx->h_xfer) SH_CHECK_HIP(ctx, hipHostFree(ctx->h_xfer));
+    ctx->h_xfer = nullptr;
+    ctx->h_xfer_size = 0;
+    const uint64_t sz = std::max<uint64_t>(bytes, 1ull << 20);
+    SH_CHECK_HIP(ctx, hipHostMalloc((void**)&ctx->h_xfer, sz, hipHostMallocDefault));
+    ctx->h_xfer_size = sz;
+  }
+  *out = ctx->h_xfer;
   return SH_OK;
 }
 

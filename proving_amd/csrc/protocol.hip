@@ -56,30 +56,84 @@ __global__ void k_gather_nodes(const uint4* const* nodes, uint32_t n, uint4* out
   if (i < 2 * n) out[i] = nodes[i >> 1][i & 1];
 }
 
-// Reads words (cols[i][rows[i]]) and 32-byte nodes (*nodes[j]) into host memory: one upload of
-// the address tables, one launch each, one download, one wait.
-int gather_words_nodes(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& rows,
-                       const std::vector<const uint8_t*>& nodes, std::vector<uint32_t>& words,
-                       std::vector<uint32_t>& node_words) {
+// Queried values at lifted rows: out[c * n_pos + k] = cols[c][lift(pos[k])] with
+// lift(p) = ((p >> (L - log_c + 1)) << 1) + (p & 1) (crates/stwo/src/prover/vcs_lifted/prover.rs:97-116).
+__global__ void k_gather_lifted(const uint32_t* const* cols, const uint32_t* logs, uint32_t n_cols, uint32_t L,
+                                const uint32_t* pos, uint32_t n_pos, uint32_t* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_cols * n_pos) return;
+  const uint32_t c = i / n_pos, k = i - c * n_pos;
+  const uint32_t p = pos[k];
+  out[i] = cols[c][((p >> (L - logs[c] + 1)) << 1) + (p & 1)];
+}
+
+// Up to 6 device regions copied into one buffer (dst + dst_off[r]), for a single download.
+struct PackArgs {
+  const uint32_t* src[6];
+  uint32_t len[6];
+  uint32_t dst_off[6];
+  uint32_t n;
+};
+__global__ void k_pack(PackArgs a, uint32_t* dst) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t r = 0; r < a.n; r++)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.len[r]; i += stride)
+      dst[a.dst_off[r] + i] = a.src[r][i];
+}
+
+// The queried values of one committed tree at its positions (k_gather_lifted).
+struct LiftedJob {
+  const uint32_t* const* cols;  // host array of n_cols device column addresses
+  const uint32_t* logs;
+  uint64_t n_cols;
+  uint32_t L;
+  std::vector<uint32_t> pos;
+  uint64_t n_values() const { return pos.empty() ? 0 : n_cols * pos.size(); }
+};
+
+// Reads words (cols[i][rows[i]]), 32-byte nodes (*nodes[j]) and the lifted values of every job
+// into host memory: one upload of the address tables (from the context's pinned buffer), one
+// launch per kind and job, one download, one wait.  values receives the jobs' values in order.
+int gather_all(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& rows,
+               const std::vector<const uint8_t*>& nodes, const std::vector<LiftedJob>& jobs,
+               std::vector<uint32_t>& words, std::vector<uint32_t>& node_words, std::vector<uint32_t>& values) {
   const uint64_t nw = cols.size(), nn = nodes.size();
+  uint64_t nv = 0;
+  for (auto& j : jobs) nv += j.n_values();
   words.assign(nw, 0);
   node_words.assign(8 * nn, 0);
-  if (nw + nn == 0) return SH_OK;
-  // device layout: [node addresses][column addresses][rows (padded to 16 B)][node words][words]
-  const uint64_t rows_bytes = (4 * nw + 15) & ~15ull;
-  const uint64_t tab_bytes = 8 * nn + 8 * nw + rows_bytes;
-  const uint64_t out_bytes = 32 * nn + 4 * nw;
-  std::vector<uint8_t> tab(tab_bytes, 0);
-  if (nn) memcpy(tab.data(), nodes.data(), 8 * nn);
+  values.assign(nv, 0);
+  if (nw + nn + nv == 0) return SH_OK;
+  auto pad16 = [](uint64_t x) { return (x + 15) & ~15ull; };
+  // tables: [node addresses][column addresses][rows] then per job [column addresses][logs][positions];
+  // outputs: [node words][words][job values]
+  const uint64_t o_cols = 8 * nn, o_rows = o_cols + 8 * nw;
+  uint64_t tab_bytes = pad16(o_rows + 4 * nw);
+  std::vector<uint64_t> o_job(jobs.size());
+  for (size_t k = 0; k < jobs.size(); k++) {
+    o_job[k] = tab_bytes;
+    if (jobs[k].n_values()) tab_bytes = pad16(tab_bytes + 12 * jobs[k].n_cols + 4 * jobs[k].pos.size());
+  }
+  const uint64_t out_bytes = 32 * nn + 4 * nw + 4 * nv;
+  uint8_t* h;
+  SH_TRY(sh_host_xfer(ctx, tab_bytes + out_bytes, &h));
+  if (nn) memcpy(h, nodes.data(), 8 * nn);
   if (nw) {
-    memcpy(tab.data() + 8 * nn, cols.data(), 8 * nw);
-    memcpy(tab.data() + 8 * nn + 8 * nw, rows.data(), 4 * nw);
+    memcpy(h + o_cols, cols.data(), 8 * nw);
+    memcpy(h + o_rows, rows.data(), 4 * nw);
+  }
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const LiftedJob& j = jobs[k];
+    if (!j.n_values()) continue;
+    memcpy(h + o_job[k], j.cols, 8 * j.n_cols);
+    memcpy(h + o_job[k] + 8 * j.n_cols, j.logs, 4 * j.n_cols);
+    memcpy(h + o_job[k] + 12 * j.n_cols, j.pos.data(), 4 * j.pos.size());
   }
   void* scr;
   SH_TRY(sh_scratch_get(ctx, SCR_MISC, tab_bytes + out_bytes, &scr));
   uint8_t* base = (uint8_t*)scr;
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(base, tab.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
   uint8_t* d_out = base + tab_bytes;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(base, h, tab_bytes, hipMemcpyHostToDevice, ctx->stream));
   if (nn) {
     k_gather_nodes<<<(unsigned)((2 * nn + 255) / 256), 256, 0, ctx->stream>>>((const uint4* const*)base,
                                                                                (uint32_t)nn, (uint4*)d_out);
@@ -87,16 +141,36 @@ int gather_words_nodes(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, co
   }
   if (nw) {
     k_gather<<<(unsigned)((nw + 255) / 256), 256, 0, ctx->stream>>>(
-        (const uint32_t* const*)(base + 8 * nn), (const uint32_t*)(base + 8 * nn + 8 * nw), (uint32_t)nw,
+        (const uint32_t* const*)(base + o_cols), (const uint32_t*)(base + o_rows), (uint32_t)nw,
         (uint32_t*)(d_out + 32 * nn));
     SH_TRY(sh_launch_check(ctx, "gather"));
   }
-  std::vector<uint32_t> host(out_bytes / 4);
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(host.data(), d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
+  uint64_t vo = 0;
+  for (size_t k = 0; k < jobs.size(); k++) {
+    const LiftedJob& j = jobs[k];
+    const uint64_t n = j.n_values();
+    if (!n) continue;
+    const uint8_t* t = base + o_job[k];
+    k_gather_lifted<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(
+        (const uint32_t* const*)t, (const uint32_t*)(t + 8 * j.n_cols), (uint32_t)j.n_cols, j.L,
+        (const uint32_t*)(t + 12 * j.n_cols), (uint32_t)j.pos.size(), (uint32_t*)(d_out + 32 * nn + 4 * nw) + vo);
+    SH_TRY(sh_launch_check(ctx, "gather_lifted"));
+    vo += n;
+  }
+  uint8_t* ho = h + tab_bytes;
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(ho, d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (nn) memcpy(node_words.data(), host.data(), 32 * nn);
-  if (nw) memcpy(words.data(), host.data() + 8 * nn, 4 * nw);
+  if (nn) memcpy(node_words.data(), ho, 32 * nn);
+  if (nw) memcpy(words.data(), ho + 32 * nn, 4 * nw);
+  if (nv) memcpy(values.data(), ho + 32 * nn + 4 * nw, 4 * nv);
   return SH_OK;
+}
+
+int gather_words_nodes(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, const std::vector<uint32_t>& rows,
+                       const std::vector<const uint8_t*>& nodes, std::vector<uint32_t>& words,
+                       std::vector<uint32_t>& node_words) {
+  std::vector<uint32_t> none;
+  return gather_all(ctx, cols, rows, nodes, {}, words, node_words, none);
 }
 
 // Reads n words (d_cols[i][rows[i]]) into host memory with one launch.
@@ -536,73 +610,46 @@ extern "C" int sh_merkle_decommit(sh_ctx* ctx, const uint8_t* const* d_layers, u
   return SH_OK;
 }
 
-// Queried values at lifted rows: out[c * n_pos + k] = cols[c][lift(pos[k])] with
-// lift(p) = ((p >> (L - log_c + 1)) << 1) + (p & 1) (crates/stwo/src/prover/vcs_lifted/prover.rs:97-116).
-__global__ void k_gather_lifted(const uint32_t* const* cols, const uint32_t* logs, uint32_t n_cols, uint32_t L,
-                                const uint32_t* pos, uint32_t n_pos, uint32_t* out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_cols * n_pos) return;
-  const uint32_t c = i / n_pos, k = i - c * n_pos;
-  const uint32_t p = pos[k];
-  out[i] = cols[c][((p >> (L - logs[c] + 1)) << 1) + (p & 1)];
+// One tree's opening (MerkleProverLifted::decommit, crates/stwo/src/prover/vcs_lifted/prover.rs:
+// 97-162) added to a gather: its witness nodes (sorted, deduplicated positions) and its values at
+// the positions as given.  *n_nodes receives the witness length.
+static int plan_tree_opening(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes, uint64_t n_cols,
+                             const uint8_t* const* d_layers, uint32_t L, std::vector<uint32_t> pos,
+                             std::vector<const uint8_t*>& nodes, std::vector<LiftedJob>& jobs, uint64_t* n_nodes) {
+  *n_nodes = 0;
+  const uint64_t n_pos = pos.size();
+  if ((n_cols && n_pos && (!d_cols || !log_sizes)) || (L > 0 && n_pos && !d_layers)) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, L < 31, "tree height %u out of range", L);
+  SH_REQUIRE(ctx, n_cols * n_pos < (1ull << 31), "too many queried values");
+  for (uint64_t k = 0; k < n_pos; k++) SH_REQUIRE(ctx, pos[k] < (1u << L), "query position %u out of range", pos[k]);
+  for (uint64_t c = 0; c < n_cols && n_pos; c++)
+    SH_REQUIRE(ctx, log_sizes[c] >= 1 && log_sizes[c] <= L, "column log size %u not in [1, %u]", log_sizes[c], L);
+  if (L > 0 && n_pos) {
+    auto need = merkle_witness_nodes(L, pos);
+    for (auto& nd : need) nodes.push_back(d_layers[nd.first] + 32ull * nd.second);
+    *n_nodes = need.size();
+  }
+  if (n_cols && n_pos) jobs.push_back({d_cols, log_sizes, n_cols, L, std::move(pos)});
+  return SH_OK;
 }
 
 extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, const uint32_t* log_sizes,
                                 uint64_t n_cols, const uint8_t* const* d_layers, uint32_t L,
                                 const uint32_t* positions, uint64_t n_pos, uint32_t* values, uint8_t* witness,
                                 uint64_t cap, uint64_t* n_witness) {
-  if (!ctx || !n_witness || (n_pos && !positions) || (n_cols && n_pos && (!d_cols || !log_sizes || !values)) ||
-      (L > 0 && n_pos && !d_layers))
-    return SH_ERR_INVALID;
+  if (!ctx || !n_witness || (n_pos && !positions) || (n_cols && n_pos && !values)) return SH_ERR_INVALID;
   *n_witness = 0;
-  SH_REQUIRE(ctx, L < 31, "tree height %u out of range", L);
-  SH_REQUIRE(ctx, n_cols * n_pos < (1ull << 31), "too many queried values");
-  for (uint64_t k = 0; k < n_pos; k++)
-    SH_REQUIRE(ctx, positions[k] < (1u << L), "query position %u out of range", positions[k]);
-  for (uint64_t c = 0; c < n_cols && n_pos; c++)
-    SH_REQUIRE(ctx, log_sizes[c] >= 1 && log_sizes[c] <= L, "column log size %u not in [1, %u]", log_sizes[c], L);
-  std::vector<std::pair<uint32_t, uint32_t>> need;
-  if (L > 0 && n_pos) need = merkle_witness_nodes(L, std::vector<uint32_t>(positions, positions + n_pos));
-  SH_REQUIRE(ctx, need.size() <= cap, "witness of %zu hashes exceeds the buffer (%llu)", need.size(),
+  std::vector<const uint8_t*> nodes;
+  std::vector<LiftedJob> jobs;
+  uint64_t nn = 0;
+  SH_TRY(plan_tree_opening(ctx, d_cols, log_sizes, n_cols, d_layers, L,
+                           std::vector<uint32_t>(positions, positions + n_pos), nodes, jobs, &nn));
+  SH_REQUIRE(ctx, nn <= cap, "witness of %llu hashes exceeds the buffer (%llu)", (unsigned long long)nn,
              (unsigned long long)cap);
-  const uint64_t nn = need.size(), nv = n_pos ? n_cols * n_pos : 0;
-  if (nn + nv == 0) return SH_OK;
-  // device layout: [node addresses][column addresses][log sizes][positions][node words][values]
-  const uint64_t nc = nv ? n_cols : 0, np_ = nv ? n_pos : 0;
-  const uint64_t o_cols = 8 * nn, o_logs = o_cols + 8 * nc, o_pos = o_logs + 4 * nc;
-  const uint64_t tab_bytes = (o_pos + 4 * np_ + 15) & ~15ull;
-  const uint64_t out_bytes = 32 * nn + 4 * nv;
-  std::vector<uint8_t> tab(tab_bytes, 0);
-  for (uint64_t k = 0; k < nn; k++) {
-    const uint8_t* a = d_layers[need[k].first] + 32ull * need[k].second;
-    memcpy(tab.data() + 8 * k, &a, 8);
-  }
-  if (nv) {
-    memcpy(tab.data() + o_cols, d_cols, 8 * nc);
-    memcpy(tab.data() + o_logs, log_sizes, 4 * nc);
-    memcpy(tab.data() + o_pos, positions, 4 * np_);
-  }
-  void* scr;
-  SH_TRY(sh_scratch_get(ctx, SCR_MISC, tab_bytes + out_bytes, &scr));
-  uint8_t* base = (uint8_t*)scr;
-  uint8_t* d_out = base + tab_bytes;
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(base, tab.data(), tab_bytes, hipMemcpyHostToDevice, ctx->stream));
-  if (nn) {
-    k_gather_nodes<<<(unsigned)((2 * nn + 255) / 256), 256, 0, ctx->stream>>>((const uint4* const*)base,
-                                                                               (uint32_t)nn, (uint4*)d_out);
-    SH_TRY(sh_launch_check(ctx, "gather_nodes"));
-  }
-  if (nv) {
-    k_gather_lifted<<<(unsigned)((nv + 255) / 256), 256, 0, ctx->stream>>>(
-        (const uint32_t* const*)(base + o_cols), (const uint32_t*)(base + o_logs), (uint32_t)nc, L,
-        (const uint32_t*)(base + o_pos), (uint32_t)np_, (uint32_t*)(d_out + 32 * nn));
-    SH_TRY(sh_launch_check(ctx, "gather_lifted"));
-  }
-  std::vector<uint8_t> host(out_bytes);
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(host.data(), d_out, out_bytes, hipMemcpyDeviceToHost, ctx->stream));
-  SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (nn) memcpy(witness, host.data(), 32 * nn);
-  if (nv) memcpy(values, host.data() + 32 * nn, 4 * nv);
+  std::vector<uint32_t> none, nw, vals;
+  SH_TRY(gather_all(ctx, {}, {}, nodes, jobs, none, nw, vals));
+  if (nn) memcpy(witness, nw.data(), 32 * nn);
+  if (!vals.empty()) memcpy(values, vals.data(), 4 * vals.size());
   *n_witness = nn;
   return SH_OK;
 }
@@ -612,12 +659,16 @@ extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, cons
 // it): the driver starts from it, and ch receives its state (read back with the commit phase's
 // download) before the host replays the layers.  d_extra / h_extra: n_extra words of the caller's
 // device memory that the same download brings back.
+// trees: committed trees opened at the queries with the FRI decommitment, in the same gather.
 static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const uint32_t* d_ch_in,
-                          const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, const sh_fri_params* p,
-                          const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
-                          uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
-  if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions || (n_extra && (!d_extra || !h_extra)))
+                          const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, sh_tree_opening* trees,
+                          uint32_t n_trees, const sh_fri_params* p, const uint32_t* const column[4],
+                          uint32_t log_size, uint64_t* felts, uint64_t cap, uint64_t* n_felts,
+                          uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions || (n_extra && (!d_extra || !h_extra)) ||
+      (n_trees && !trees))
     return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, n_extra < (1ull << 28), "too many extra words (%llu)", (unsigned long long)n_extra);
   SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 4, "fold_step %u out of range [1, 4]", p->fold_step);
   SH_REQUIRE(ctx, p->log_blowup_factor >= 1 && p->log_blowup_factor <= 16 && p->log_last_layer_degree_bound <= 10,
              "invalid FRI parameters");
@@ -651,9 +702,11 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   // downloaded roots, and checks that the alphas agree.
   const uint32_t max_layers = log_size + 1;
   // [ch: 16 words][alphas: 16 words per layer][roots: 8 words per layer][initial ch: 16 words]
+  // [last layer: 4 x 2^last_log words][extra words]: the commit phase's one download
   const uint64_t aux_words = 16 + 24 * max_layers + 16;
+  const uint64_t dl_words = aux_words + (4ull << last_log) + n_extra;
   uint32_t* d_aux = nullptr;
-  TRY_CLEAN(sh_dev_alloc(ctx, 4 * aux_words, (void**)&d_aux));
+  TRY_CLEAN(sh_dev_alloc(ctx, 4 * dl_words, (void**)&d_aux));
   uint32_t* d_ch = d_aux;
   uint32_t* d_alphas = d_aux + 16;
   uint32_t* d_roots = d_alphas + 16 * max_layers;
@@ -781,14 +834,32 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     n_layers++;
   }
   // one download: the device channel, the alphas and roots of every layer, the last layer
-  std::vector<uint32_t> aux(aux_words);
-  std::vector<uint32_t> words(4ull << last_log);
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(aux.data(), d_aux, 4 * aux.size(), hipMemcpyDeviceToHost, ctx->stream));
-  for (int c = 0; c < 4; c++)
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(words.data() + ((uint64_t)c << last_log), cur.cols[c], 4ull << last_log,
-                                     hipMemcpyDeviceToHost, ctx->stream));
-  if (n_extra) SH_CHECK_HIP(ctx, hipMemcpyAsync(h_extra, d_extra, 4 * n_extra, hipMemcpyDeviceToHost, ctx->stream));
+  {
+    PackArgs pa = {};
+    for (int c = 0; c < 4; c++) {
+      pa.src[c] = cur.cols[c];
+      pa.len[c] = 1u << last_log;
+      pa.dst_off[c] = (uint32_t)(aux_words + ((uint64_t)c << last_log));
+    }
+    pa.n = 4;
+    if (n_extra) {
+      pa.src[4] = d_extra;
+      pa.len[4] = (uint32_t)n_extra;
+      pa.dst_off[4] = (uint32_t)(aux_words + (4ull << last_log));
+      pa.n = 5;
+    }
+    const uint64_t longest = std::max<uint64_t>(1ull << last_log, n_extra);
+    k_pack<<<(unsigned)std::min<uint64_t>(256, (longest + 255) / 256), 256, 0, ctx->stream>>>(pa, d_aux);
+    TRY_CLEAN(sh_launch_check(ctx, "pack"));
+  }
+  uint8_t* h_dl;
+  TRY_CLEAN(sh_host_xfer(ctx, 4 * dl_words, &h_dl));
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(h_dl, d_aux, 4 * dl_words, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> aux(aux_words), words(4ull << last_log);
+  memcpy(aux.data(), h_dl, 4 * aux_words);
+  memcpy(words.data(), h_dl + 4 * aux_words, 16ull << last_log);
+  if (n_extra) memcpy(h_extra, h_dl + 4 * (aux_words + (4ull << last_log)), 4 * n_extra);
   if (d_ch_in) {
     memcpy(ch->digest, &aux[16 + 24 * max_layers], 32);
     ch->n_draws = aux[16 + 24 * max_layers + 8];
@@ -882,8 +953,46 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     for (auto& x : lq) x >>= k;  // Queries::fold (crates/stwo/src/core/queries.rs:48-55)
     lq.erase(std::unique(lq.begin(), lq.end()), lq.end());
   }
-  std::vector<uint32_t> gw, gnw;
-  TRY_CLEAN(gather_words_nodes(ctx, gc, gr, gn, gw, gnw));
+  // the committed trees' openings (CommitmentTreeProver::decommit, crates/stwo/src/prover/pcs/
+  // mod.rs:286-302) join the same gather
+  std::vector<LiftedJob> jobs;
+  std::vector<uint64_t> tree_node0(n_trees), tree_nodes(n_trees), tree_npos(n_trees);
+  for (uint32_t t = 0; t < n_trees; t++) {
+    sh_tree_opening& T = trees[t];
+    std::vector<uint32_t> pos;
+    if (!T.preprocessed) {
+      pos = q;
+    } else if (T.L > 0) {
+      // prepare_preprocessed_query_positions (crates/stwo/src/core/pcs/utils.rs:185-203)
+      for (uint32_t x : q)
+        pos.push_back(log_size < T.L ? ((x >> 1) << (T.L - log_size + 1)) + (x & 1)
+                                     : ((x >> (log_size - T.L + 1)) << 1) + (x & 1));
+    }
+    tree_node0[t] = gn.size();
+    tree_npos[t] = pos.size();
+    const size_t jobs_before = jobs.size();
+    TRY_CLEAN(plan_tree_opening(ctx, T.d_cols, T.log_sizes, T.n_cols, T.d_layers, T.L, std::move(pos), gn, jobs,
+                                &tree_nodes[t]));
+    if (tree_nodes[t] > T.witness_cap || (jobs.size() > jobs_before && !T.values)) {
+      cleanup_all();
+      return sh_fail(ctx, SH_ERR_INVALID, "tree %u: witness of %llu hashes exceeds the buffer or no value buffer", t,
+                     (unsigned long long)tree_nodes[t]);
+    }
+  }
+  std::vector<uint32_t> gw, gnw, gv;
+  TRY_CLEAN(gather_all(ctx, gc, gr, gn, jobs, gw, gnw, gv));
+  {
+    uint64_t vo = 0;
+    for (uint32_t t = 0; t < n_trees; t++) {
+      sh_tree_opening& T = trees[t];
+      T.n_positions = tree_npos[t];
+      T.n_witness = tree_nodes[t];
+      if (tree_nodes[t]) memcpy(T.witness, gnw.data() + 8 * tree_node0[t], 32 * tree_nodes[t]);
+      const uint64_t nv = tree_npos[t] ? T.n_cols * tree_npos[t] : 0;
+      if (nv) memcpy(T.values, gv.data() + vo, 4 * nv);
+      vo += nv;
+    }
+  }
   // FriProof (crates/cairo-serialize/src/serialize.rs:79-90): the first layer, the inner layers
   // (length-prefixed), each FriLayerProof (serialize.rs:66-77) = fri_witness, decommitment,
   // commitment, written straight from the gathered words
@@ -920,17 +1029,18 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
 extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const sh_fri_params* p,
                             const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
                             uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
-  return fri_prove_impl(ctx, tw, ch, nullptr, nullptr, 0, nullptr, p, column, log_size, felts, cap, n_felts,
-                        query_positions, n_query_positions);
+  return fri_prove_impl(ctx, tw, ch, nullptr, nullptr, 0, nullptr, nullptr, 0, p, column, log_size, felts, cap,
+                        n_felts, query_positions, n_query_positions);
 }
 
 extern "C" int sh_fri_prove_dch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* d_ch, sh_channel* ch_out,
-                                const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, const sh_fri_params* p,
+                                const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra,
+                                sh_tree_opening* trees, uint32_t n_trees, const sh_fri_params* p,
                                 const uint32_t* const column[4], uint32_t log_size, uint64_t* felts, uint64_t cap,
                                 uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
   if (!d_ch) return SH_ERR_INVALID;
-  return fri_prove_impl(ctx, tw, ch_out, d_ch, d_extra, n_extra, h_extra, p, column, log_size, felts, cap, n_felts,
-                        query_positions, n_query_positions);
+  return fri_prove_impl(ctx, tw, ch_out, d_ch, d_extra, n_extra, h_extra, trees, n_trees, p, column, log_size, felts,
+                        cap, n_felts, query_positions, n_query_positions);
 }
 
 // ------------------------------------------------------------------ DEEP quotient driver

@@ -296,13 +296,17 @@ class CommitmentSchemeProver:
         quotients.free()
         return self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker)
 
-    def prove_values_dpoint(self, dch, channel, ws, oods_proof_of_work=None, check_samples=None):
+    def prove_values_dpoint(self, dch, channel, ws, oods_proof_of_work=None, check_samples=None,
+                            device_check=None):
         """prove_values when every committed column is sampled once, at one point that the device
         channel dch drew into ws (a ProofState): the samples are evaluated, mixed and combined
         into the DEEP quotient on the device, and the FRI driver's commit-phase download brings
         back the sampled values with the proof's other device-drawn words; the host channel
         `channel` receives the final state.  check_samples as in prove_values (called with
-        (sampled values, the downloaded ProofState)).  Returns (felts, proof dict)."""
+        (sampled values, the downloaded ProofState)); device_check, when given, is called once the
+        sampled values are on the device to enqueue a check that writes ws's CHECK words, and the
+        proof raises ConstraintsNotSatisfied when that check's flag is set.  Returns (felts,
+        proof dict)."""
         L = self.trees[-1].lifting_log_size
         b = self.fri["log_blowup_factor"]
         ctx = self.be.ctx
@@ -322,12 +326,15 @@ class CommitmentSchemeProver:
         # compute_samples at the point doubled down to each column's domain
         self.be.eval_at_dpoint(ColumnRefs(ctx, coeff_ptrs, coeff_logs), L - eval_logs, ws.ptr(ws.Z),
                                ws.ptr(ws.VALS))
+        if device_check is not None:
+            device_check()
         dch.mix_words(ws.ptr(ws.VALS), 4 * n_samples)
         dch.draw_secure_felts(1, ws.ptr(ws.RC))
         quotients = fri_quotients_dpoint(self.be, self.tw, ColumnRefs(ctx, eval_ptrs, eval_logs), ws.ptr(ws.Z),
                                          ws.ptr(ws.VALS), ws.ptr(ws.RC), L, b)
-        fri_felts, queries, host = fri_prove_dch(self.be, self.tw, dch, channel, self.fri_params(), quotients,
-                                                 extra=(ws.ptr(0), ws.download_words))
+        fri_felts, queries, host, openings = fri_prove_dch(
+            self.be, self.tw, dch, channel, self.fri_params(), quotients, extra=(ws.ptr(0), ws.download_words),
+            trees=[(t.evals, t.layers, i == 0) for i, t in enumerate(self.trees)])
         quotients.free()
         ws.read(host)
         for k, i in enumerate(unknown):
@@ -342,24 +349,30 @@ class CommitmentSchemeProver:
         checker = _Background(check_samples, sampled_values, ws) if check_samples is not None else None
         if oods_proof_of_work is None:
             oods_proof_of_work = ws.nonce
-        return self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker)
+        out = self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker,
+                                           openings)
+        if device_check is not None and ws.check_failed:
+            raise ConstraintsNotSatisfied("Constraints not satisfied.")
+        return out
 
     def fri_params(self):
         b = self.fri["log_blowup_factor"]
         return FriParams(self.fri["pow_bits"], b, self.fri["log_last_layer_degree_bound"], self.fri["n_queries"],
                          self.fri["fold_step"])
 
-    def _decommit_and_serialize(self, fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker):
+    def _decommit_and_serialize(self, fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker,
+                                openings=None):
         """The query phase of prove_values (crates/stwo/src/prover/pcs/mod.rs:286-315) after the FRI
-        proof: every tree decommitted at the queries, the check's verdict, the proof's felts."""
-        L = self.trees[-1].lifting_log_size
-        pp_height = self.trees[0].lifting_log_size
-        pp_positions = prepare_preprocessed_query_positions(queries, L, pp_height)
-        queried_values, decommitments = [], []
-        for i, tree in enumerate(self.trees):
-            qv, wit = tree.decommit(self.be, pp_positions if i == 0 else queries)
-            queried_values.append(qv)
-            decommitments.append(wit)
+        proof: every tree decommitted at the queries (unless the FRI driver opened them already:
+        openings, one (values, witness) per tree), the check's verdict, the proof's felts."""
+        if openings is None:
+            L = self.trees[-1].lifting_log_size
+            pp_height = self.trees[0].lifting_log_size
+            pp_positions = prepare_preprocessed_query_positions(queries, L, pp_height)
+            openings = [tree.decommit(self.be, pp_positions if i == 0 else queries)
+                        for i, tree in enumerate(self.trees)]
+        queried_values = [qv for qv, _ in openings]
+        decommitments = [wit for _, wit in openings]
         if checker is not None:
             checker.result()  # re-raises the check's exception (ConstraintsNotSatisfied)
         proof = {"fri_params": dict(self.fri), "commitments": [t.root for t in self.trees],
@@ -375,7 +388,7 @@ class ProofState:
     commit phase: the composition coefficient, the OODS point, the DEEP random coefficient, the
     OODS proof-of-work nonce, the roots mixed on the device, the sampled values; then, not read
     back, the constraint powers."""
-    ALPHA, Z, RC, CTL, ROOTS = 0, 4, 12, 16, 20
+    ALPHA, Z, RC, CTL, CHECK, ROOTS = 0, 4, 12, 16, 20, 32
 
     def __init__(self, ctx, n_samples, n_powers, n_roots=2):
         self.n_samples, self.n_roots = n_samples, n_roots
@@ -402,6 +415,10 @@ class ProofState:
     @property
     def nonce(self):
         return int(self.host[self.CTL]) | (int(self.host[self.CTL + 1]) << 32)
+
+    @property
+    def check_failed(self):
+        return bool(self.host[self.CHECK])
 
     @property
     def values(self):
@@ -580,11 +597,10 @@ def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
     dch.grind(OODS_POW_BITS, ws.ptr(ws.CTL))
     dch.draw_random_point(ws.ptr(ws.Z))
 
-    def check(sv, st):
-        alpha = st.alpha
-        return check_wide_fib_oods(sv, alpha, st.oods_point, log_n, b, backend.generate_secure_powers(alpha, m))
-
-    felts, proof = scheme.prove_values_dpoint(dch, channel, ws, check_samples=check)
+    # the closing sanity check runs on the device, on the sampled values where they are
+    felts, proof = scheme.prove_values_dpoint(
+        dch, channel, ws, device_check=lambda: backend.wide_fibonacci_oods_check(
+            ws.ptr(ws.VALS), n_cols, ws.ptr(ws.POW), ws.ptr(ws.Z), log_n, ws.ptr(ws.CHECK)))
     proof["composition_random_coeff"] = ws.alpha
     proof["oods_point"] = ws.oods_point
     ws.free()

@@ -11,7 +11,8 @@ import pytest
 import oracle as O
 from oracle import protocol as PR
 from proving_amd.backend import (BaseColumn, Channel, ColumnRefs, DeviceBuffer, DeviceChannel, FriParams,
-                                 SecureColumnByCoords, _ptr_array, fri_prove, fri_prove_dch, fri_quotients_dpoint)
+                                 SecureColumnByCoords, _ptr_array, fri_prove, fri_prove_dch, fri_quotients_dpoint,
+                                 tree_decommit)
 
 pytestmark = pytest.mark.gpu
 P = O.P
@@ -146,7 +147,8 @@ def test_fri_quotients_dpoint(ctx, backend, logs, L):
 
 def test_fri_prove_dch_matches_host_channel(ctx, backend):
     """FRI from a device channel: the same proof, queries and final channel as the host-channel
-    driver, and the extra words come back with the commit phase."""
+    driver, the extra words come back with the commit phase, and the trees opened in the same
+    gather equal sh_tree_decommit's openings (a preprocessed tree of another height included)."""
     log = 10
     fri = FriParams(6, 1, 1, 20, 1)
     rng = np.random.default_rng(8)
@@ -164,12 +166,30 @@ def test_fri_prove_dch_matches_host_channel(ctx, backend):
     dch = DeviceChannel(backend, host)
     felts_h, queries_h = fri_prove(backend, tree, host, fri, q, as_array=True)
     extra = dev_words(ctx, np.arange(37, dtype=np.uint32) * 3 + 1)
+    # trees: a lifted tree at the FRI column's height with mixed column sizes, a preprocessed one
+    # of a lower height, and an empty one
+    trees = []
+    for logs, pre in (([log, log - 2, log, log - 4], False), ([log - 3, log - 3], True), ([], True)):
+        cols = [BaseColumn.from_cpu(ctx, rng.integers(0, P, 1 << lg, dtype=np.uint32)) for lg in logs]
+        L = max(logs) if logs else 0
+        layers, _ = backend.commit_lifted(sorted(cols, key=lambda c: c.log_size) if cols else [], L)
+        trees.append((cols, layers, pre))
     out = Channel(backend.lib)
-    felts_d, queries_d, got_extra = fri_prove_dch(backend, tree, dch, out, fri, q, extra=(extra.ptr, 37))
+    felts_d, queries_d, got_extra, openings = fri_prove_dch(backend, tree, dch, out, fri, q, extra=(extra.ptr, 37),
+                                                            trees=trees)
     assert felts_d.tolist() == felts_h.tolist()
     assert queries_d == queries_h
     assert (bytes(out.s.digest), out.s.n_draws) == (bytes(host.s.digest), host.s.n_draws)
     np.testing.assert_array_equal(got_extra, np.arange(37, dtype=np.uint32) * 3 + 1)
+    for (cols, layers, pre), (vals, wit) in zip(trees, openings):
+        L = len(layers) - 1
+        pos = PR.pp_query_positions(queries_d, log, L) if pre else queries_d
+        if not cols:
+            assert vals.size == 0 and len(wit) == 0
+            continue
+        ev, ew = tree_decommit(backend, cols, layers, pos)
+        np.testing.assert_array_equal(vals, ev)
+        np.testing.assert_array_equal(wit, ew)
 
 
 def test_composition_dev_matches_host_powers(ctx, backend):
@@ -185,3 +205,37 @@ def test_composition_dev_matches_host_powers(ctx, backend):
     d_pw = dev_words(ctx, powers)
     backend.wide_fibonacci_composition_dev(cols, log_n, eval_log, d_pw.ptr, acc, 0)
     np.testing.assert_array_equal(acc.to_cpu(), exp)
+
+
+@pytest.mark.parametrize("n_cols,log_n", [(3, 4), (12, 8), (300, 20)])
+def test_wide_fib_oods_check_values(ctx, backend, n_cols, log_n):
+    """The device sanity check's two values equal the oracle's composition rebuilt from its
+    samples and the oracle's constraint evaluation at the point, on random samples (which
+    disagree, so the flag is set); made to agree, the flag clears."""
+    rng = np.random.default_rng(n_cols)
+    vals = rng.integers(0, P, (n_cols + 8, 4), dtype=np.uint32)
+    alpha = [int(x) for x in rng.integers(0, P, 4)]
+    z = random_qm31_point(n_cols)
+    m = n_cols - 2
+    pw, p = [], [1, 0, 0, 0]
+    for _ in range(m):
+        pw.append(p)
+        p = PR.qmul(p, alpha)
+    rhs = PR.wide_fib_composition_at_point([list(map(int, v)) for v in vals[:n_cols]], alpha, z, log_n)
+    comp = [list(map(int, v)) for v in vals[n_cols:]]
+    x = PR._repeated_double(z, log_n - 1)[:4]
+    lhs = PR.qadd(PR.from_partial_evals(comp[:4]), PR.qmul(x, PR.from_partial_evals(comp[4:])))
+    d_pw = dev_words(ctx, np.array(pw[::-1], np.uint32))
+    d_z, out = dev_words(ctx, z), DeviceBuffer(ctx, 64)
+    d_vals = dev_words(ctx, vals)
+    backend.wide_fibonacci_oods_check(d_vals.ptr, n_cols, d_pw.ptr, d_z.ptr, log_n, out.ptr)
+    got = out.download(np.uint32, 9)
+    assert got[1:5].tolist() == [int(v) for v in lhs] and got[5:9].tolist() == [int(v) for v in rhs]
+    assert got[0] == (1 if list(lhs) != list(rhs) else 0)
+    # shift the first composition coordinate so that lhs == rhs: the flag clears
+    fix = PR.qsub(rhs, lhs)
+    vals[n_cols] = np.array(PR.qadd([int(v) for v in vals[n_cols]], fix), np.uint32)
+    d_vals = dev_words(ctx, vals)
+    backend.wide_fibonacci_oods_check(d_vals.ptr, n_cols, d_pw.ptr, d_z.ptr, log_n, out.ptr)
+    got = out.download(np.uint32, 9)
+    assert got[0] == 0 and got[1:5].tolist() == got[5:9].tolist()
