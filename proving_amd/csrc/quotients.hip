@@ -135,8 +135,38 @@ __device__ __forceinline__ uint32_t mad_red(uint32_t a, uint32_t b, uint32_t c) 
   const uint64_t v = (uint64_t)a * b + c;
   return sh::reduce_once(((uint32_t)v & sh::P) + (uint32_t)(v >> 31));
 }
+// c + a * b + d * e reduced to canonical, for canonical a, b, c, d, e: the sum is below 2^63,
+// so one fold and two conditional subtractions bring it to [0, P).
+__device__ __forceinline__ uint32_t mad2_red(uint32_t a, uint32_t b, uint32_t d, uint32_t e, uint32_t c) {
+  const uint64_t v = (uint64_t)a * b + (uint64_t)d * e + c;
+  const uint64_t f = (v & sh::P) + (v >> 31);  // below 2^32 + 2^31
+  const uint32_t g = (uint32_t)(f & sh::P) + (uint32_t)(f >> 31);  // below 2^31 + 2
+  return sh::reduce_once(g);
+}
 // v += a * b with v kept below 2^64: at most four products of two canonical words between folds.
 __device__ __forceinline__ void mac64(uint64_t& v, uint32_t a, uint32_t b) { v += (uint64_t)a * b; }
+
+// Loads and stores through a base address held in scalar registers and a 32-bit byte offset
+// (global_load/global_store with an SGPR base: no 64-bit address arithmetic per access).
+typedef const __attribute__((address_space(1))) uint32_t* gcptr;
+typedef __attribute__((address_space(1))) uint32_t* gptr;
+__device__ __forceinline__ uint32_t gld(const uint32_t* base, uint32_t byte_off) {
+  return *(gcptr)((const __attribute__((address_space(1))) char*)base + byte_off);
+}
+__device__ __forceinline__ void gst(uint32_t* base, uint32_t byte_off, uint32_t v) {
+  *(gptr)((__attribute__((address_space(1))) char*)base + byte_off) = v;
+}
+
+// Accumulations per combine launch (their partial-numerator addresses travel by value).
+constexpr int kCombineMaxAcc = 8;
+constexpr int kCombineMaxRb = 8;
+struct CombineArgs {
+  const uint32_t* part[kCombineMaxAcc][4];  // partial numerator k, coordinate c
+  uint32_t* out[4];
+  sh::mpt rot[kCombineMaxRb];  // rotation of the half-coset point from a lane's row 0 to its row j
+  uint32_t n_acc;              // accumulations of this launch (kc[0 .. n_acc))
+  uint32_t accumulate;         // add to what out holds (a later launch of a long sum)
+};
 
 // compute_quotients_and_combine's row loop (crates/stwo/src/prover/backend/cpu/quotients.rs:
 // 51-101): q[row] = sum_k (partial_k[lift(row)] - first_linear_k * p.y) / den_k(p).  Each lane
@@ -144,43 +174,58 @@ __device__ __forceinline__ void mac64(uint64_t& v, uint32_t a, uint32_t b) { v +
 // denominators of its rows for NK accumulations at once: den^-1 = conj(den) / |den|^2 with the
 // M31 norms batch-inverted (Montgomery's trick; the SIMD backend batch-inverts its denominators
 // too, crates/stwo/src/prover/backend/simd/quotients.rs:255-283): one exponentiation per lane per
-// NK accumulations instead of one per row and accumulation.  Products are summed lazily in 64 bits.
+// RB x NK denominators.  A lane's rows differ only in bits that the bit reversal sends below the
+// bits of its first row, so each row's domain point is the first row's point rotated by a fixed
+// point (one point addition per row).  Products are summed lazily in 64 bits.
 template <int NK, int RB>
-__global__ __launch_bounds__(256) void k_combine(const uint32_t* const* __restrict__ parts,
-                                                 const CombineK* __restrict__ kc, uint32_t n_acc, uint32_t sub_log,
-                                                 sh_point_tab half, uint32_t* const* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_combine(const CombineArgs a, const CombineK* __restrict__ kc,
+                                                 uint32_t sub_log, sh_point_tab half) {
   const uint32_t rows = 1u << sub_log;
   const uint32_t r0 = blockIdx.x * (256u * RB) + threadIdx.x;
   uint32_t px[RB], py[RB];
+  {
+    // CircleDomain(half).at(bitrev(r0)): the second half of the domain is the conjugate
+    const uint32_t j = sub_log ? sh::bitrev(r0 & (rows - 1), sub_log) : 0;
+    const uint32_t hs = rows >> 1;
+    const bool conj = j >= hs;
+    const sh::mpt p0 = sh_tab_point(half, conj ? j - hs : j);
 #pragma unroll
-  for (int j = 0; j < RB; j++) {
-    const uint32_t r = r0 + 256u * j;
-    sh::mpt dp = {1, 0};
-    if (r < rows) dp = sh_circle_domain_point(half, sub_log - 1, sh::bitrev(r, sub_log));
-    px[j] = dp.x;
-    py[j] = dp.y;
+    for (int k = 0; k < RB; k++) {
+      const sh::mpt p = k == 0 ? p0 : sh::padd(p0, a.rot[k]);
+      px[k] = p.x;
+      py[k] = conj ? sh::mneg(p.y) : p.y;
+    }
   }
   uint64_t acc[RB][4];
 #pragma unroll
   for (int j = 0; j < RB; j++)
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[j][c] = 0;
-  for (uint32_t k0 = 0; k0 < n_acc; k0 += NK) {
+  if (a.accumulate) {
+#pragma unroll
+    for (int j = 0; j < RB; j++) {
+      const uint32_t r = r0 + 256u * j;
+      if (r < rows)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[j][c] = gld(a.out[c], 4 * r);
+    }
+  }
+  for (uint32_t k0 = 0; k0 < a.n_acc; k0 += NK) {
     // denominators and their norms; an absent accumulation (the last chunk may be short) takes
     // norm 1, and a zero norm is replaced by 1 with its inverse forced to 0
     uint32_t d0[NK][RB], d1[NK][RB], nrm[NK][RB], pre[NK * RB];
     uint32_t run = 1;
 #pragma unroll
     for (int kk = 0; kk < NK; kk++) {
-      const bool has = k0 + kk < n_acc;
+      const bool has = k0 + kk < a.n_acc;
       const CombineK K = kc[has ? k0 + kk : 0];
 #pragma unroll
       for (int j = 0; j < RB; j++) {
-        const uint32_t a = mad_red(py[j], K.pix[0], mad_red(px[j], K.npiy[0], K.c[0]));
-        const uint32_t b = mad_red(py[j], K.pix[1], mad_red(px[j], K.npiy[1], K.c[1]));
-        d0[kk][j] = a;
-        d1[kk][j] = b;
-        const uint32_t n = mad_red(b, b, sh::mmul(a, a));
+        const uint32_t x = mad2_red(py[j], K.pix[0], px[j], K.npiy[0], K.c[0]);
+        const uint32_t y = mad2_red(py[j], K.pix[1], px[j], K.npiy[1], K.c[1]);
+        d0[kk][j] = x;
+        d1[kk][j] = y;
+        const uint32_t n = mad2_red(x, x, y, y, 0);
         nrm[kk][j] = (!has || n == 0) ? 1u : n;
         run = sh::mmul(run, nrm[kk][j]);
         pre[kk * RB + j] = run;
@@ -189,25 +234,43 @@ __global__ __launch_bounds__(256) void k_combine(const uint32_t* const* __restri
     uint32_t inv = sh::minv(run);
 #pragma unroll
     for (int kk = NK - 1; kk >= 0; kk--) {
-      const bool has = k0 + kk < n_acc;
-      const CombineK K = kc[has ? k0 + kk : 0];
+      const uint32_t k = k0 + kk;
+      const bool has = k < a.n_acc;
+      const CombineK K = kc[has ? k : 0];
       const uint32_t ratio = sub_log - K.plog;
-      const uint32_t* const* pk = parts + 4 * (has ? k0 + kk : 0);
 #pragma unroll
       for (int j = RB - 1; j >= 0; j--) {
         const int e = kk * RB + j;
-        const uint32_t a = d0[kk][j], b = d1[kk][j];
+        const uint32_t x = d0[kk][j], y = d1[kk][j];
         uint32_t ninv = e > 0 ? sh::mmul(inv, pre[e - 1]) : inv;  // 1 / norm of element e
         inv = sh::mmul(inv, nrm[kk][j]);
-        if (a == 0 && b == 0) ninv = 0;
+        if (x == 0 && y == 0) ninv = 0;
         const uint32_t r = r0 + 256u * j;
         if (!has || r >= rows) continue;
-        // den^-1 = (a - b i) / norm = (i0, -i1); g = p.y * den^-1 = (g0, g1)
-        const uint32_t i0 = sh::mmul(a, ninv), i1 = sh::mmul(b, ninv);
+        // den^-1 = (x - y i) / norm = (i0, -i1); g = p.y * den^-1 = (g0, g1)
+        const uint32_t i0 = sh::mmul(x, ninv), i1 = sh::mmul(y, ninv);
         const uint32_t ni1 = sh::P - i1;  // -Im den^-1, in [1, P]
         const uint32_t g0 = sh::mmul(py[j], i0), g1 = sh::mmul(py[j], ni1);
-        const uint32_t li = ((r >> (ratio + 1)) << 1) + (r & 1);
-        const uint32_t pv0 = pk[0][li], pv1 = pk[1][li], pv2 = pk[2][li], pv3 = pk[3][li];
+        const uint32_t lo = 4 * (((r >> (ratio + 1)) << 1) + (r & 1));
+        uint32_t pv0, pv1, pv2, pv3;
+        // the accumulation index is uniform: its addresses stay in scalar registers
+        switch (k) {
+#define SH_PART(K_)                   \
+  case K_:                            \
+    pv0 = gld(a.part[K_][0], lo);     \
+    pv1 = gld(a.part[K_][1], lo);     \
+    pv2 = gld(a.part[K_][2], lo);     \
+    pv3 = gld(a.part[K_][3], lo);     \
+    break;
+          SH_PART(0) SH_PART(1) SH_PART(2) SH_PART(3) SH_PART(4) SH_PART(5) SH_PART(6)
+          default:
+            pv0 = gld(a.part[7][0], lo);
+            pv1 = gld(a.part[7][1], lo);
+            pv2 = gld(a.part[7][2], lo);
+            pv3 = gld(a.part[7][3], lo);
+            break;
+#undef SH_PART
+        }
         // + partial * den^-1: each CM31 half (u0 + u1 i)(i0 - i1 i) = (u0 i0 + u1 i1) + (u1 i0 - u0 i1) i
         mac64(acc[j][0], pv0, i0);
         mac64(acc[j][0], pv1, i1);
@@ -236,7 +299,7 @@ __global__ __launch_bounds__(256) void k_combine(const uint32_t* const* __restri
     const uint32_t r = r0 + 256u * j;
     if (r >= rows) continue;
 #pragma unroll
-    for (int c = 0; c < 4; c++) out[c][r] = lazy_final(acc[j][c]);
+    for (int c = 0; c < 4; c++) gst(a.out[c], 4 * r, lazy_final(acc[j][c]));
   }
 }
 
@@ -409,7 +472,7 @@ int sh_static_scratch_init(sh_ctx* ctx) {
   return SH_OK;
 }
 
-static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_parts,
+static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_parts,
                               const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
                               uint32_t* q_scratch);
 
@@ -448,18 +511,14 @@ static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint3
     K.plog = partial_logs[k];
     K.pad[0] = K.pad[1] = K.pad[2] = 0;
   }
-  void *d_acc, *d_parts;
+  void* d_acc;
   SH_TRY(sh_stage(ctx, kc.data(), kc.size() * sizeof(CombineK), &d_acc));
-  std::vector<const void*> parts(4 * std::max<uint64_t>(n_acc, 1), nullptr);
-  for (uint64_t i = 0; i < 4 * n_acc; i++) parts[i] = d_partials[i];
-  SH_TRY(sh_stage(ctx, parts.data(), parts.size() * sizeof(void*), &d_parts));
-  return combine_and_extend(ctx, tw, (const uint32_t* const*)d_parts, (const CombineK*)d_acc, n_acc, L, b, d_out,
-                            q_scratch);
+  return combine_and_extend(ctx, tw, d_partials, (const CombineK*)d_acc, n_acc, L, b, d_out, q_scratch);
 }
 
 // The combine over the subdomain of log L - b (k_combine, per-accumulation constants in device
 // memory) and the quotient's extension: interpolate on the subdomain, evaluate on L.
-static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_parts,
+static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_parts,
                               const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
                               uint32_t* q_scratch) {
   const uint32_t s = L - b;
@@ -479,21 +538,38 @@ static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t
   uint32_t* const* d_qtab = (uint32_t* const*)d_q;
   {
     SpanGuard span(ctx, "quotients_combine");
-    const uint32_t* const* pt = d_parts;
-    const CombineK* kt = d_kc;
-    const uint32_t na = (uint32_t)n_acc;
-    // accumulations per batch inversion: all of them up to 4 (every inverted norm is used); rows
-    // per lane: 8 for one accumulation, 4 otherwise (the register budget of 3 waves per SIMD)
-    auto grid = [&](int rb) { return (unsigned)(((1ull << s) + 256ull * rb - 1) / (256ull * rb)); };
-    if (n_acc <= 1)
-      k_combine<1, 8><<<grid(8), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
-    else if (n_acc == 2)
-      k_combine<2, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
-    else if (n_acc == 3)
-      k_combine<3, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
-    else
-      k_combine<4, 4><<<grid(4), 256, 0, ctx->stream>>>(pt, kt, na, s, half, d_qtab);
-    SH_TRY(sh_launch_check(ctx, "combine"));
+    // accumulations per batch inversion: all of a launch's up to 4 (every inverted norm is used);
+    // rows per lane: 8 for one or two accumulations, 4 otherwise (the register budget)
+    const int nk = n_acc <= 1 ? 1 : n_acc == 2 ? 2 : n_acc == 3 ? 3 : 4;
+    const int rb = nk <= 2 ? 8 : 4;
+    CombineArgs a = {};
+    for (int c = 0; c < 4; c++) a.out[c] = q + ((uint64_t)c << s);
+    // row j of a lane is row 0 plus 256 j: its bit-reversed index adds bitrev(256 j), below the
+    // first row's bits, i.e. a rotation by that many steps of the half coset
+    const uint32_t step = sh::subgroup_gen(s - 1);
+    for (int j = 0; j < rb; j++) {
+      const uint64_t off = 256ull * j;
+      const uint32_t d = off < (1ull << s) ? sh::bitrev((uint32_t)off, s) : 0;
+      a.rot[j] = sh::index_to_point_host(sh::idx_reduce((uint64_t)d * step));
+    }
+    const unsigned grid = (unsigned)(((1ull << s) + 256ull * rb - 1) / (256ull * rb));
+    for (uint64_t k0 = 0; k0 < std::max<uint64_t>(n_acc, 1); k0 += kCombineMaxAcc) {
+      const uint64_t na = std::min<uint64_t>(kCombineMaxAcc, n_acc - std::min(n_acc, k0));
+      a.n_acc = (uint32_t)na;
+      a.accumulate = k0 > 0;
+      for (uint64_t k = 0; k < kCombineMaxAcc; k++)
+        for (int c = 0; c < 4; c++) a.part[k][c] = k < na ? h_parts[4 * (k0 + k) + c] : nullptr;
+      const CombineK* kt = d_kc + k0;
+      if (nk == 1)
+        k_combine<1, 8><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+      else if (nk == 2)
+        k_combine<2, 8><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+      else if (nk == 3)
+        k_combine<3, 4><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+      else
+        k_combine<4, 4><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+      SH_TRY(sh_launch_check(ctx, "combine"));
+    }
   }
   // interpolate on the subdomain (its twiddles are prefixes of the domain-L layers), evaluate on L
   SH_TRY(sh_fft_interpolate_batch(ctx, tw, (const uint32_t* const*)d_qtab, d_qtab, 4, s, L, half_init));
@@ -754,9 +830,7 @@ extern "C" int sh_fri_quotients_dpoint(sh_ctx* ctx, const sh_twiddles* tw, const
     }
     k_dpoint_combine_const<<<1, 64, 0, ctx->stream>>>(d_point, d_FL, n_cls, top, d_kc);
     SH_TRY(sh_launch_check(ctx, "combine_const"));
-    void* d_parts;
-    SH_TRY(sh_stage(ctx, parts, sizeof(parts), &d_parts));
-    return combine_and_extend(ctx, tw, (const uint32_t* const*)d_parts, d_kc, 1, L, b, d_out, nullptr);
+    return combine_and_extend(ctx, tw, (const uint32_t* const*)parts, d_kc, 1, L, b, d_out, nullptr);
   };
   rc = run();
   release();  // stream-ordered: later users of these buffers run after the kernels above
