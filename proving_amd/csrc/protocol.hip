@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <set>
 #include <tuple>
 #include <vector>
 
@@ -17,6 +18,11 @@
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
                          uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready);
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers);
+int sh_accumulate_numerators_fl(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols, uint32_t col_log,
+                                uint32_t log_blowup, const uint32_t* batch_points, const uint32_t* batch_term_offsets,
+                                const uint32_t* term_col_idx, const uint32_t* term_values,
+                                const uint32_t* term_random_coeffs, uint64_t n_batches, uint32_t* const* d_outs,
+                                uint32_t* first_linear, const uint8_t* apply_fl);
 int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t line_initial_index,
                      uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4]);
 int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
@@ -1086,7 +1092,16 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
     uint32_t log;
     uint32_t* buf;
     sh::qm31 fl;
+    bool fl_applied;  // the numerator took its first-linear term already
   };
+  auto key = [](const sh::qpt& p) {
+    return std::array<uint32_t, 8>{p.x.a.a, p.x.a.b, p.x.b.a, p.x.b.b, p.y.a.a, p.y.a.b, p.y.b.a, p.y.b.b};
+  };
+  // a point sampled by columns of one size only, the lifting size, has its numerator at the
+  // combine's size: the numerator kernel subtracts the first-linear term and the combine skips it
+  std::map<std::array<uint32_t, 8>, std::set<uint32_t>> point_sizes;
+  for (uint64_t i = 0; i < n_cols; i++)
+    for (auto& smp : samples[i]) point_sizes[key(smp.point)].insert(col_log_sizes[i]);
   std::vector<Acc> accs;
   std::vector<uint32_t> sizes(col_log_sizes, col_log_sizes + n_cols);
   std::sort(sizes.begin(), sizes.end());
@@ -1143,12 +1158,16 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
         free_accs();
         return rc;
       }
-      accs.push_back({pts[k], plog, buf, sh::qzero()});
+      const auto& ps = point_sizes[key(pts[k])];
+      accs.push_back({pts[k], plog, buf, sh::qzero(), ps.size() == 1 && lg == L});
       for (int c = 0; c < 4; c++) outs[4 * k + c] = buf + ((uint64_t)c << plog);
     }
     std::vector<uint32_t> fl(4 * pts.size());
-    rc = sh_accumulate_numerators(ctx, gcols.data(), gcols.size(), lg, log_blowup, bp.data(), off.data(),
-                                  tidx.data(), tval.data(), trc.data(), pts.size(), outs.data(), fl.data());
+    std::vector<uint8_t> apply(pts.size());
+    for (size_t k = 0; k < pts.size(); k++) apply[k] = accs[accs.size() - pts.size() + k].fl_applied;
+    rc = sh_accumulate_numerators_fl(ctx, gcols.data(), gcols.size(), lg, log_blowup, bp.data(), off.data(),
+                                     tidx.data(), tval.data(), trc.data(), pts.size(), outs.data(), fl.data(),
+                                     apply.data());
     if (rc != SH_OK) {
       free_accs();
       return rc;
@@ -1156,9 +1175,6 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
     for (size_t k = 0; k < pts.size(); k++) accs[accs.size() - pts.size() + k].fl = sh::qload(&fl[4 * k]);
   }
   // group per point, sorted by (x, y) (quotient_ops.rs:127-152); stable
-  auto key = [](const sh::qpt& p) {
-    return std::array<uint32_t, 8>{p.x.a.a, p.x.a.b, p.x.b.a, p.x.b.b, p.y.a.a, p.y.a.b, p.y.b.a, p.y.b.b};
-  };
   std::vector<size_t> order(accs.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return key(accs[a].point) < key(accs[b].point); });
@@ -1173,7 +1189,7 @@ extern "C" int sh_fri_quotients(sh_ctx* ctx, const sh_twiddles* tw, const uint32
     std::vector<uint32_t> logs;
     for (size_t k = i; k < j; k++) {
       const Acc& a = accs[order[k]];
-      fl = sh::qadd(fl, a.fl);
+      if (!a.fl_applied) fl = sh::qadd(fl, a.fl);
       for (int c = 0; c < 4; c++) cols.push_back(a.buf + ((uint64_t)c << a.log));
       logs.push_back(a.log);
     }

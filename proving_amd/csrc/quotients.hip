@@ -40,6 +40,12 @@ struct OutSpec {
   uint32_t src[kMaxOut];      // which accumulated sum feeds output b
   sh::qm31 w[kMaxOut];        // output b = w * S_src - B
   sh::qm31 B[kMaxOut];
+  // outputs whose bit is set also subtract FL_b * p.y, p the row's point of the domain
+  // CircleDomain(half) in bit-reversed order (the first-linear term of the combine, taken here
+  // when the numerator is already at the combine's size, so the combine skips it)
+  uint32_t fl_mask;
+  sh::qm31 FL[kMaxOut];
+  sh_point_tab half;
 };
 
 __device__ __forceinline__ void lazy_fold(uint64_t& v) { v = (v & sh::P) + (v >> 31); }
@@ -55,7 +61,8 @@ template <int NB>
 __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __restrict__ cols, uint32_t n_cols,
                                                    const uint32_t* __restrict__ coef, uint32_t rows,
                                                    uint32_t* const* __restrict__ outs, OutSpec spec,
-                                                   const uint32_t* __restrict__ dev_B) {
+                                                   const uint32_t* __restrict__ dev_B,
+                                                   const uint32_t* __restrict__ dev_FL) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   uint64_t acc[NB][4];
@@ -97,14 +104,20 @@ __global__ __launch_bounds__(256) void k_accum_num(const uint32_t* const* __rest
 #pragma unroll
   for (int j = 0; j < NB; j++)
     S[j] = {{lazy_final(acc[j][0]), lazy_final(acc[j][1])}, {lazy_final(acc[j][2]), lazy_final(acc[j][3])}};
+  uint32_t py = 0;
+  if (spec.fl_mask) {
+    const uint32_t sub_log = 31 - __builtin_clz(rows);
+    py = sh_circle_domain_point(spec.half, sub_log - 1, sh::bitrev(r, sub_log)).y;
+  }
   for (uint32_t b = 0; b < spec.n_out; b++) {
     sh::qm31 s = S[0];
 #pragma unroll
     for (int j = 1; j < NB; j++)
       if (spec.src[b] == (uint32_t)j) s = S[j];
-    // dev_B: the constant terms in device memory (a driver that computed them on the device)
+    // dev_B / dev_FL: the constant terms in device memory (a driver that computed them there)
     const sh::qm31 Bb = dev_B ? sh::qload(dev_B + 4 * b) : spec.B[b];
     sh::qm31 o = sh::qsub(sh::qmul(spec.w[b], s), Bb);
+    if ((spec.fl_mask >> b) & 1) o = sh::qsub(o, sh::qmulm(dev_FL ? sh::qload(dev_FL + 4 * b) : spec.FL[b], py));
     outs[4 * b + 0][r] = o.a.a;
     outs[4 * b + 1][r] = o.a.b;
     outs[4 * b + 2][r] = o.b.a;
@@ -177,7 +190,7 @@ struct CombineArgs {
 // RB x NK denominators.  A lane's rows differ only in bits that the bit reversal sends below the
 // bits of its first row, so each row's domain point is the first row's point rotated by a fixed
 // point (one point addition per row).  Products are summed lazily in 64 bits.
-template <int NK, int RB>
+template <int NK, int RB, bool FL>
 __global__ __launch_bounds__(256) void k_combine(const CombineArgs a, const CombineK* __restrict__ kc,
                                                  uint32_t sub_log, sh_point_tab half) {
   const uint32_t rows = 1u << sub_log;
@@ -250,7 +263,8 @@ __global__ __launch_bounds__(256) void k_combine(const CombineArgs a, const Comb
         // den^-1 = (x - y i) / norm = (i0, -i1); g = p.y * den^-1 = (g0, g1)
         const uint32_t i0 = sh::mmul(x, ninv), i1 = sh::mmul(y, ninv);
         const uint32_t ni1 = sh::P - i1;  // -Im den^-1, in [1, P]
-        const uint32_t g0 = sh::mmul(py[j], i0), g1 = sh::mmul(py[j], ni1);
+        // (FL false: every first-linear term is zero, taken by the numerators already)
+        const uint32_t g0 = FL ? sh::mmul(py[j], i0) : 0, g1 = FL ? sh::mmul(py[j], ni1) : 0;
         const uint32_t lo = 4 * (((r >> (ratio + 1)) << 1) + (r & 1));
         uint32_t pv0, pv1, pv2, pv3;
         // the accumulation index is uniform: its addresses stay in scalar registers
@@ -280,17 +294,22 @@ __global__ __launch_bounds__(256) void k_combine(const CombineArgs a, const Comb
         mac64(acc[j][2], pv3, i1);
         mac64(acc[j][3], pv3, i0);
         mac64(acc[j][3], pv2, ni1);
-        // - first_linear * g: each CM31 half -(f0 + f1 i)(g0 + g1 i) = (-f0 g0 + f1 g1) + (-f0 g1 - f1 g0) i
-        mac64(acc[j][0], K.nfl[0], g0);
-        mac64(acc[j][0], K.flim[0], g1);
-        mac64(acc[j][1], K.nfl[0], g1);
-        mac64(acc[j][1], K.nfl[1], g0);
-        mac64(acc[j][2], K.nfl[2], g0);
-        mac64(acc[j][2], K.flim[1], g1);
-        mac64(acc[j][3], K.nfl[2], g1);
-        mac64(acc[j][3], K.nfl[3], g0);
+        if (FL) {
+          // - first_linear * g: each CM31 half -(f0 + f1 i)(g0 + g1 i) = (-f0 g0 + f1 g1) + (-f0 g1 - f1 g0) i
+          mac64(acc[j][0], K.nfl[0], g0);
+          mac64(acc[j][0], K.flim[0], g1);
+          mac64(acc[j][1], K.nfl[0], g1);
+          mac64(acc[j][1], K.nfl[1], g0);
+          mac64(acc[j][2], K.nfl[2], g0);
+          mac64(acc[j][2], K.flim[1], g1);
+          mac64(acc[j][3], K.nfl[2], g1);
+          mac64(acc[j][3], K.nfl[3], g0);
+        }
+        // four products since the last fold with FL, two without: fold every accumulation, or
+        // every second one
+        if (FL || (kk & 1) || kk == 0)
 #pragma unroll
-        for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
+          for (int c = 0; c < 4; c++) lazy_fold(acc[j][c]);
       }
     }
   }
@@ -342,11 +361,13 @@ bool qis_zero(sh::qm31 x) { return sh::qeq(x, sh::qzero()); }
 
 }  // namespace
 
-extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
-                                        uint32_t col_log, uint32_t log_blowup, const uint32_t* batch_points,
-                                        const uint32_t* batch_term_offsets, const uint32_t* term_col_idx,
-                                        const uint32_t* term_values, const uint32_t* term_random_coeffs,
-                                        uint64_t n_batches, uint32_t* const* d_outs, uint32_t* first_linear) {
+// apply_fl (optional, one flag per batch): that batch's output also subtracts its first-linear
+// term times the row's p.y (OutSpec::fl_mask); its first_linear still reports the term.
+int sh_accumulate_numerators_fl(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols, uint32_t col_log,
+                                uint32_t log_blowup, const uint32_t* batch_points, const uint32_t* batch_term_offsets,
+                                const uint32_t* term_col_idx, const uint32_t* term_values,
+                                const uint32_t* term_random_coeffs, uint64_t n_batches, uint32_t* const* d_outs,
+                                uint32_t* first_linear, const uint8_t* apply_fl) {
   if (!ctx || (n_batches && (!batch_points || !batch_term_offsets || !d_outs || !first_linear)))
     return SH_ERR_INVALID;
   if (n_batches == 0) return SH_OK;
@@ -393,6 +414,22 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
         }
   }
   const uint32_t rows = 1u << (col_log - log_blowup);
+  sh_point_tab half = {};
+  bool any_fl = false;
+  for (uint64_t bi = 0; bi < NB && apply_fl; bi++) any_fl |= apply_fl[bi] != 0;
+  if (any_fl) {
+    // the rows' domain: the subdomain of the combine, CircleDomain(half_odds(L - 1).initial,
+    // log - 1) with L = col_log (domain.rs:72-82 split)
+    SH_REQUIRE(ctx, col_log > log_blowup, "column too small for the blowup");
+    SH_TRY(sh_point_tab_make(ctx, sh::subgroup_gen(col_log + 1), col_log - log_blowup - 1, &half));
+  }
+  auto set_fl = [&](OutSpec& spec, uint32_t k, uint64_t bi) {
+    if (apply_fl && apply_fl[bi]) {
+      spec.fl_mask |= 1u << k;
+      spec.FL[k] = FL[bi];
+      spec.half = half;
+    }
+  };
   void* d_cols_tab;
   SH_TRY(sh_stage(ctx, d_cols, n_cols * sizeof(void*), &d_cols_tab));
   auto launch = [&](int nb, const std::vector<uint32_t>& coef_words, OutSpec spec,
@@ -406,10 +443,10 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
     uint32_t* const* ot = (uint32_t* const*)d_outs_tab;
     SpanGuard span(ctx, "quotients_accumulate_numerators");
     switch (nb) {
-      case 1: k_accum_num<1><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
-      case 2: k_accum_num<2><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
-      case 3: k_accum_num<3><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
-      default: k_accum_num<4><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr); break;
+      case 1: k_accum_num<1><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr, nullptr); break;
+      case 2: k_accum_num<2><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr, nullptr); break;
+      case 3: k_accum_num<3><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr, nullptr); break;
+      default: k_accum_num<4><<<grid, 256, 0, ctx->stream>>>(ct, (uint32_t)n_cols, (const uint32_t*)d_coef, rows, ot, spec, nullptr, nullptr); break;
     }
     return sh_launch_check(ctx, "accumulate_numerators");
   };
@@ -426,6 +463,7 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
         spec.src[k] = 0;
         spec.w[k] = w[b0 + k];
         spec.B[k] = B[b0 + k];
+        set_fl(spec, k, b0 + k);
         for (int c = 0; c < 4; c++) outp.push_back(d_outs[4 * (b0 + k) + c]);
       }
       SH_TRY(launch(1, coef, spec, outp));
@@ -442,12 +480,23 @@ extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_co
         spec.src[j] = j;
         spec.w[j] = sh::qone();
         spec.B[j] = B[b0 + j];
+        set_fl(spec, j, b0 + j);
         for (int c = 0; c < 4; c++) outp.push_back(d_outs[4 * (b0 + j) + c]);
       }
       SH_TRY(launch(nb, coef, spec, outp));
     }
   }
   return SH_OK;
+}
+
+extern "C" int sh_accumulate_numerators(sh_ctx* ctx, const uint32_t* const* d_cols, uint64_t n_cols,
+                                        uint32_t col_log, uint32_t log_blowup, const uint32_t* batch_points,
+                                        const uint32_t* batch_term_offsets, const uint32_t* term_col_idx,
+                                        const uint32_t* term_values, const uint32_t* term_random_coeffs,
+                                        uint64_t n_batches, uint32_t* const* d_outs, uint32_t* first_linear) {
+  return sh_accumulate_numerators_fl(ctx, d_cols, n_cols, col_log, log_blowup, batch_points, batch_term_offsets,
+                                     term_col_idx, term_values, term_random_coeffs, n_batches, d_outs, first_linear,
+                                     nullptr);
 }
 
 // The allocation-free variant's scratch: the subdomain quotient (4 coordinate arrays of up to
@@ -474,7 +523,7 @@ int sh_static_scratch_init(sh_ctx* ctx) {
 
 static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_parts,
                               const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
-                              uint32_t* q_scratch);
+                              uint32_t* q_scratch, bool with_fl);
 
 // q_scratch: 4 << (L - b) words for the subdomain quotient, or nullptr to use the context's
 // (lazily allocated) scratch slot.
@@ -511,16 +560,19 @@ static int quotients_and_combine(sh_ctx* ctx, const sh_twiddles* tw, const uint3
     K.plog = partial_logs[k];
     K.pad[0] = K.pad[1] = K.pad[2] = 0;
   }
+  bool with_fl = false;
+  for (uint64_t i = 0; i < 4 * n_acc; i++) with_fl |= first_linear[i] != 0;
   void* d_acc;
   SH_TRY(sh_stage(ctx, kc.data(), kc.size() * sizeof(CombineK), &d_acc));
-  return combine_and_extend(ctx, tw, d_partials, (const CombineK*)d_acc, n_acc, L, b, d_out, q_scratch);
+  return combine_and_extend(ctx, tw, d_partials, (const CombineK*)d_acc, n_acc, L, b, d_out, q_scratch, with_fl);
 }
 
 // The combine over the subdomain of log L - b (k_combine, per-accumulation constants in device
-// memory) and the quotient's extension: interpolate on the subdomain, evaluate on L.
+// memory) and the quotient's extension: interpolate on the subdomain, evaluate on L.  with_fl
+// false: every first-linear term is zero (the numerators took it), and the combine skips it.
 static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_parts,
                               const CombineK* d_kc, uint64_t n_acc, uint32_t L, uint32_t b, uint32_t* const* d_out,
-                              uint32_t* q_scratch) {
+                              uint32_t* q_scratch, bool with_fl) {
   const uint32_t s = L - b;
   // subdomain: CircleDomain(Coset(half_odds(L-1).initial, s - 1)) (domain.rs:72-82 split)
   const uint32_t half_init = sh::subgroup_gen(L + 1);
@@ -560,14 +612,18 @@ static int combine_and_extend(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t
       for (uint64_t k = 0; k < kCombineMaxAcc; k++)
         for (int c = 0; c < 4; c++) a.part[k][c] = k < na ? h_parts[4 * (k0 + k) + c] : nullptr;
       const CombineK* kt = d_kc + k0;
+#define SH_COMBINE(NK_, RB_)                                                            \
+  (with_fl ? k_combine<NK_, RB_, true><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half)  \
+           : k_combine<NK_, RB_, false><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half))
       if (nk == 1)
-        k_combine<1, 8><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+        SH_COMBINE(1, 8);
       else if (nk == 2)
-        k_combine<2, 8><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+        SH_COMBINE(2, 8);
       else if (nk == 3)
-        k_combine<3, 4><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+        SH_COMBINE(3, 4);
       else
-        k_combine<4, 4><<<grid, 256, 0, ctx->stream>>>(a, kt, s, half);
+        SH_COMBINE(4, 4);
+#undef SH_COMBINE
       SH_TRY(sh_launch_check(ctx, "combine"));
     }
   }
@@ -790,6 +846,7 @@ extern "C" int sh_fri_quotients_dpoint(sh_ctx* ctx, const sh_twiddles* tw, const
       SH_TRY(sh_launch_check(ctx, "quotient_sums"));
     }
     // accumulate_numerators per size: out = S - B with S = sum_col c_col * col (one batch)
+    const bool fl_in_numerator = n_cls == 1 && logs[0] == L;
     void* d_sorted;
     SH_TRY(sh_stage(ctx, sorted.data(), sorted.size() * sizeof(void*), &d_sorted));
     for (uint32_t k = 0; k < n_cls; k++) {
@@ -803,11 +860,16 @@ extern "C" int sh_fri_quotients_dpoint(sh_ctx* ctx, const sh_twiddles* tw, const
       spec.n_out = 1;
       spec.src[0] = 0;
       spec.w[0] = sh::qone();
+      if (fl_in_numerator) {
+        // the only size, at the combine's size: the numerator takes the first-linear term
+        spec.fl_mask = 1;
+        SH_TRY(sh_point_tab_make(ctx, sh::subgroup_gen(L + 1), L - b - 1, &spec.half));
+      }
       const uint32_t rows = 1u << plog;
       SpanGuard span(ctx, "quotients_accumulate_numerators");
       k_accum_num<1><<<(rows + 255) / 256, 256, 0, ctx->stream>>>(
           (const uint32_t* const*)d_sorted + off[k], off[k + 1] - off[k], d_coef + 4ull * off[k], rows,
-          (uint32_t* const*)d_outs, spec, d_B + 4 * k);
+          (uint32_t* const*)d_outs, spec, d_B + 4 * k, d_FL + 4 * k);
       SH_TRY(sh_launch_check(ctx, "accumulate_numerators"));
     }
     // one point: every size's partial lifted and summed (lift_and_accumulate, ascending sizes)
@@ -828,9 +890,10 @@ extern "C" int sh_fri_quotients_dpoint(sh_ctx* ctx, const sh_twiddles* tw, const
       SH_TRY(sh_lift_and_accumulate(ctx, cols.data(), plogs.data(), n_cls, o));
       for (int c = 0; c < 4; c++) parts[c] = o[c];
     }
-    k_dpoint_combine_const<<<1, 64, 0, ctx->stream>>>(d_point, d_FL, n_cls, top, d_kc);
+    k_dpoint_combine_const<<<1, 64, 0, ctx->stream>>>(d_point, d_FL, fl_in_numerator ? 0 : n_cls, top, d_kc);
     SH_TRY(sh_launch_check(ctx, "combine_const"));
-    return combine_and_extend(ctx, tw, (const uint32_t* const*)parts, d_kc, 1, L, b, d_out, nullptr);
+    return combine_and_extend(ctx, tw, (const uint32_t* const*)parts, d_kc, 1, L, b, d_out, nullptr,
+                              !fl_in_numerator);
   };
   rc = run();
   release();  // stream-ordered: later users of these buffers run after the kernels above
