@@ -3,8 +3,8 @@
 Requests arrive over time (a Poisson stream in the benchmark) and are admitted as worker slots
 and the HBM budget allow; the proofs of different requests overlap on the device, and each is
 the same bytes whether it is made alone or in the mix.  The default schedule (ProofServer's
-"lanes") keeps one lane in strict arrival order and lets small requests bypass it on a second
-lane while a large proof occupies the first; the "batched" schedule commits the traces of every
+"lanes") runs whole proofs on worker contexts, each idle worker taking the oldest waiting request,
+so two proofs share the device whenever two are waiting; the "batched" schedule commits the traces of every
 admitted batch together (one interpolation and evaluation launch chain per column size across
 requests) and finishes the proofs on worker contexts.  Twiddle trees stay resident.
 
@@ -54,14 +54,14 @@ class ProofServer:
     """The request loop.  Two scheduling modes, both admitting by the HBM budget and keeping the
     twiddle tree resident:
 
-    * "lanes" (default): each worker context proves whole requests.  Worker 0 is the main lane
-      and takes requests strictly in arrival order; the other workers are bypass lanes that take
-      only small requests (at most small_cells trace cells), oldest first, and only while the main
-      lane is busy.  A small proof then runs on the device beside a large one instead of queueing
-      ahead of or behind it: the small ones' latency drops, and the large ones wait only behind
-      what arrived before them on the main lane.  One stream per worker (a context creates its
-      helper stream only for multi-size commits), so the process stays within the device's
-      hardware queues.
+    * "lanes" (default): each worker context proves whole requests, and an idle worker takes the
+      oldest waiting request of any size (first come, first served over n_workers lanes).  Two
+      proofs then share the device whenever two requests wait, and one proof's host phases are
+      covered by the other's kernels.  (Measured against a main lane in arrival order plus a lane for small
+      requests only, and against big proofs running alone: at 80% and 105% of capacity this had
+      the lowest 99th percentile and the same throughput.)  One stream per worker (a context
+      creates its helper stream only for multi-size commits), so the process stays within the
+      device's hardware queues.
     * "batched": the scheduler commits the traces of each admitted batch together on its own
       context (one interpolation and evaluation launch chain per column size across requests)
       and the workers finish the proofs, shortest job first with aging.
@@ -69,12 +69,11 @@ class ProofServer:
     whole_proofs=True selects "lanes", False "batched" (the names the tests use)."""
 
     def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05,
-                 whole_proofs=True, small_cells=1 << 23):
+                 whole_proofs=True):
         self.fri = dict(fri)
         self.mode = "lanes" if whole_proofs else "batched"
         self.whole_proofs = whole_proofs
         self.twiddle_log = twiddle_log
-        self.small_cells = small_cells
         self.budget = hbm_budget
         self.n_workers = n_workers
         self.max_in_flight = max_in_flight or (n_workers if whole_proofs else 2 * n_workers)
@@ -163,31 +162,23 @@ class ProofServer:
         return r.n_cols << r.log_n
 
     def _dispatch_lanes(self, waiting, now):
-        """Hands requests to idle lanes: the main lane takes the oldest request of any size, a
-        bypass lane the oldest small one while the main lane is busy.  Returns the dispatched."""
+        """Hands the oldest waiting requests to idle workers, as the HBM budget allows.  Returns
+        the dispatched."""
         b = self.fri["log_blowup_factor"]
         out = []
         with self._lock:
             for k in range(self.n_workers):
                 if self._busy[k] or not waiting:
                     continue
-                if k == 0:
-                    cand = 0
-                elif self._busy[0]:
-                    cand = next((i for i, r in enumerate(waiting) if self._cells(r) <= self.small_cells), None)
-                    if cand is None:
-                        continue
-                else:
-                    continue  # the main lane is idle: it takes the next request itself
-                r = waiting[cand]
+                r = waiting[0]
                 cost = device_bytes(r.n_cols, r.log_n, b)
                 if self._reserved + cost > self.budget and self._in_flight > 0:
-                    continue
+                    break  # the oldest waits for memory; nothing overtakes it
                 self._reserved += cost
                 self._in_flight += 1
                 self._busy[k] = True
                 r.admitted = now
-                waiting.pop(cand)
+                waiting.pop(0)
                 out.append((k, r, cost))
         for k, r, cost in out:
             self._queues[k].put((r, None, None, cost))
