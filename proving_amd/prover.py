@@ -13,6 +13,7 @@ small proof vectors. There is no CPU fallback: without libstwo_hip.so nothing he
 """
 import ctypes
 import threading
+import weakref
 
 import numpy as np
 
@@ -27,6 +28,9 @@ COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
 # Proofs keep their channel in device memory by default (DeviceChannel): from the trace root to
 # the FRI commit phase the host never waits for the device.  False: the host channel throughout.
 DEVICE_CHANNEL = True
+# With the device channel, a trace committed again from the same buffers replays a captured
+# hipGraph of the batched commit (CommitmentSchemeProver._commit_captured).
+GRAPH_COMMITS = True
 
 
 def _mix_root(channel, layers, root):
@@ -201,6 +205,10 @@ class CommitmentSchemeProver:
         L = self.lifting_log_size(len(self.trees))
         logs = np.asarray(_logs(evals), np.uint32)
         if len(evals) and L == int(logs.max()) + b:
+            if isinstance(channel, DeviceChannel) and getattr(self, "graph_commit", False):
+                tree = self._commit_captured(evals, logs, b, L, channel)
+                if tree is not None:
+                    return tree
             # every coefficient and every LDE column of the tree in one pooled allocation each
             coeffs = ColumnSet(self.be.ctx, logs)
             ldes = ColumnSet(self.be.ctx, logs + b)
@@ -218,6 +226,45 @@ class CommitmentSchemeProver:
         if coeffs:
             self.be.interpolate_columns(coeffs, self.tw)
         return self.commit_polys(coeffs, channel)
+
+    def _commit_captured(self, evals, logs, b, L, channel):
+        """The batched commit as a replayed hipGraph (sh_commit_trace_graph_create) when the same
+        evaluation buffers are committed again (a prover run over and over on resident traces):
+        the graph owns the tree's coefficient, LDE and layer buffers and is kept on the backend
+        (one per backend).  The second commit of a set of buffers captures the graph (whose warm-up
+        run is that commit); later ones replay it.  Returns None to have the caller commit eagerly:
+        on the first commit of a set of buffers, and when the cached graph's buffers still belong
+        to a live tree of another proof."""
+        key = (tuple(int(p) for p in _ptr_array(evals)), tuple(int(x) for x in logs), b, id(self.tw))
+        cached = getattr(self.be, "_commit_graph", None)
+        in_use = cached is not None and cached["tree"] is not None and cached["tree"]() is not None
+        if cached is not None and cached["key"] == key:
+            if in_use:
+                return None
+            cached["graph"].launch()
+        else:
+            seen = self.be.__dict__.setdefault("_commit_seen", [])
+            if key not in seen or in_use:
+                seen.append(key)
+                del seen[:-4]
+                return None
+            if cached is not None:
+                cached["graph"].free()
+            coeffs = ColumnSet(self.be.ctx, logs)
+            ldes = ColumnSet(self.be.ctx, logs + b)
+            layers = HashLayerSet(self.be.ctx, L)
+            graph = self.be.commit_trace_graph(evals, b, self.tw, coeffs, ldes, layers)
+            # the graph reads the evaluations at whatever buffers have these addresses when it is
+            # replayed (a replay happens only for a commit of buffers at the same addresses); it
+            # keeps the twiddle tree its kernels address, and its own output buffers
+            graph.buffers = None
+            cached = self.be._commit_graph = {"key": key, "graph": graph, "coeffs": coeffs, "ldes": ldes,
+                                              "layers": layers, "tw": self.tw, "tree": None}
+        _mix_root(channel, cached["layers"], None)
+        tree = CommitmentTree(cached["coeffs"], cached["ldes"], cached["layers"], None, L)
+        cached["tree"] = weakref.ref(tree)
+        self.trees.append(tree)
+        return tree
 
     def commit_polys(self, coeffs, channel):
         """TreeBuilder::extend_polys + commit: evaluate with the blowup, then commit."""
@@ -500,17 +547,20 @@ def serialize_commitment_scheme_proof(proof, as_array=False):
 
 
 def start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, committed_trace=None,
-                         device_channel=None):
+                         device_channel=None, graph_commit=None):
     """First phase of test_wide_fib_prove_with_blake's prover (crates/examples/src/wide_fibonacci/
     mod.rs:183-210): config with an empty, unlifted preprocessed tree; commit it and the trace.
     committed_trace may carry a CommitmentTree built elsewhere (batched across proofs); its root is
     mixed exactly as a fresh commit would be.  With the device channel (DEVICE_CHANNEL unless
     device_channel says otherwise) a fresh trace's root is mixed on the device, and the host
-    channel is behind until the FRI commit phase returns.  Returns (scheme, channel)."""
+    channel is behind until the FRI commit phase returns; a trace committed again from the same
+    buffers then replays a captured graph of its commit (GRAPH_COMMITS unless graph_commit says
+    otherwise).  Returns (scheme, channel)."""
     b = fri["log_blowup_factor"]
     channel = channel or Channel(backend.lib)
     scheme = CommitmentSchemeProver(backend, twiddles, fri, log_n + b, 0)
     scheme.device_channel = DEVICE_CHANNEL if device_channel is None else bool(device_channel)
+    scheme.graph_commit = GRAPH_COMMITS if graph_commit is None else bool(graph_commit)
     scheme.dch = None
     scheme.commit_evals([], channel)                       # preprocessed trace (empty; root cached)
     if committed_trace is None:
@@ -609,15 +659,17 @@ def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
     return felts, proof
 
 
-def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, device_channel=None):
+def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, device_channel=None,
+                         graph_commit=None):
     """test_wide_fib_prove_with_blake's prover side (crates/examples/src/wide_fibonacci/mod.rs:
     183-220) through prove_ex: empty preprocessed tree, the trace tree, then
     finish_wide_fibonacci.  trace: device columns (BaseColumn, bit-reversed evaluations on
     CanonicCoset(log_n)).  device_channel: keep the channel in device memory (default
-    DEVICE_CHANNEL).  Returns (felts, proof dict, the channel in its final state)."""
+    DEVICE_CHANNEL); graph_commit: replay a captured commit of a trace proved again from the same
+    buffers (default GRAPH_COMMITS).  Returns (felts, proof dict, the channel in its final state)."""
     assert fri["log_blowup_factor"] >= 1, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
     scheme, channel = start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel,
-                                           device_channel=device_channel)
+                                           device_channel=device_channel, graph_commit=graph_commit)
     felts, proof = finish_wide_fibonacci(scheme, channel, len(trace), log_n)
     return felts, proof, channel
 

@@ -119,7 +119,8 @@ class ProofServer:
                 if scheme is None:              # the whole proof on this worker's context
                     c0, c1 = BaseColumn.from_cpu(ctx, req.c0), BaseColumn.from_cpu(ctx, req.c1)
                     trace = be.wide_fibonacci_trace(c0, c1, req.n_cols)
-                    req.felts, _, _ = prove_wide_fibonacci(be, tw, trace, req.log_n, self.fri)
+                    # every request's trace is new: no commit graph to replay
+                    req.felts, _, _ = prove_wide_fibonacci(be, tw, trace, req.log_n, self.fri, graph_commit=False)
                 else:
                     scheme.be = be              # continue the proof on this worker's context
                     req.felts, _ = finish_wide_fibonacci(scheme, channel, req.n_cols, req.log_n)
@@ -259,7 +260,7 @@ def prove_sequential(backend, twiddles, requests, fri, realtime=True):
         r.admitted = time.perf_counter() - t0
         c0, c1 = BaseColumn.from_cpu(backend.ctx, r.c0), BaseColumn.from_cpu(backend.ctx, r.c1)
         trace = backend.wide_fibonacci_trace(c0, c1, r.n_cols)
-        r.felts, _, _ = prove_wide_fibonacci(backend, twiddles, trace, r.log_n, fri)
+        r.felts, _, _ = prove_wide_fibonacci(backend, twiddles, trace, r.log_n, fri, graph_commit=False)
         backend.ctx.sync()
         r.done = time.perf_counter() - t0
     return requests

@@ -9,7 +9,7 @@ import pytest
 import oracle as O
 from oracle import protocol as PR
 from proving_amd.backend import BaseColumn
-from proving_amd.prover import ConstraintsNotSatisfied, prove_wide_fibonacci
+from proving_amd.prover import ConstraintsNotSatisfied, finish_wide_fibonacci, prove_wide_fibonacci, start_wide_fibonacci
 from proving_amd.workloads import WIDE_FIB, wide_fib_trace
 
 pytestmark = pytest.mark.gpu
@@ -81,3 +81,31 @@ def test_wide_fib_full_size_verifies(ctx, backend):
     del trace
     felts, proof, _ = prove_wide_fibonacci(backend, tree, cols, w["log_size"], w["fri"])
     assert PR.wide_fib_verify(felts.tolist(), w["n_cols"], w["log_size"], w["fri"])
+
+
+def test_repeated_proofs_replay_the_captured_commit(ctx, backend):
+    """Proving a trace again from the same buffers: the first proof commits eagerly, the second
+    captures the commit as a graph, later ones replay it, and a proof started while another's
+    tree from that graph is alive commits eagerly; every proof is byte-equal to the oracle's,
+    also after the trace buffers are overwritten with another trace."""
+    n_cols, log_n, fri = 12, 9, params(6, 1, 9)
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    traces = [wide_fib_trace(n_cols, log_n, s) for s in (31, 32)]
+    exp = [PR.wide_fib_prove(t, log_n, fri, tw, itw)[0] for t in traces]
+    cols = [BaseColumn.from_cpu(ctx, c) for c in traces[0]]
+    for k in range(4):
+        felts, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+        assert felts.tolist() == exp[0], f"proof {k}"
+    assert backend._commit_graph is not None
+    # a proof in progress holds the graph's tree: another proof of the same buffers commits eagerly
+    scheme, ch = start_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    felts2, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    felts1, _ = finish_wide_fibonacci(scheme, ch, n_cols, log_n)
+    assert felts1.tolist() == exp[0] and felts2.tolist() == exp[0]
+    del scheme
+    # new contents in the same buffers: the replayed graph reads them
+    for c, h in zip(cols, traces[1]):
+        c.upload(h)
+    felts, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    assert felts.tolist() == exp[1]
