@@ -335,16 +335,21 @@ __global__ void k_fri_channel(const uint32_t* root, uint32_t* ch, uint32_t k, ui
 }
 
 // ---- the small FRI layers in one launch (fold_step 1)
-// At 2^7 values and below a layer's leaves, tree, channel step and fold are each too small to fill the
-// device, so each launch costs its latency.  One workgroup runs every remaining layer: the leaves
+// At 2^9 values and below a layer's leaves, tree, channel step and fold are each too small to fill
+// the device, so each launch costs its latency (a layer of 2^8..2^13 values as separate launches
+// takes about 60 us).  One workgroup of 1024 lanes runs every remaining layer: the leaves
 // (Blake2s of the 4 coordinates, 16 bytes) and the tree with four lanes per hash through LDS, the
 // channel step (mix the root, draw alpha: FriInnerLayerProver::new + draw_secure_felt,
 // crates/stwo/src/prover/fri.rs:175-194) and the fold (fold_line, crates/stwo/src/prover/fri.rs:
 // 41-45) into the next layer's evaluation.  Every layer's evaluation and tree stay in device memory
 // for the decommitment, exactly as the per-layer launches leave them.
-constexpr uint32_t kFriTailMaxLog = 7;
+// (Measured on the FRI + DEEP phase of 200 x 2^21: layers up to 2^7 with 512 lanes 2.47 ms, up
+// to 2^9 with 1024 lanes 2.37 ms, up to 2^10 with 1024 lanes 2.46 ms: one CU is the bound there.)
+constexpr uint32_t kFriTailMaxLog = 9;
 constexpr uint32_t kFriTailMaxLayers = 12;
-constexpr uint32_t kFriTailThreads = 512;
+constexpr uint32_t kFriTailThreads = 1024;
+// the tree's next level (4 << log words at most) fits in the leaf-message area
+static_assert((kFriTailThreads / 4) * 16 >= (4u << kFriTailMaxLog), "FRI tail work area too small");
 
 struct FriTailLayer {
   const uint32_t* eval;  // 4 coordinate columns of 2^log words, contiguous
