@@ -10,6 +10,7 @@
 //
 // One lane per output element; a lane folds 2^fold_step inputs in registers
 // (fold_step <= 4), reading each coordinate array once.  HBM-bound.
+#include "blake2s.h"
 #include "common.h"
 
 namespace {
@@ -43,11 +44,30 @@ struct Cols4 {
   uint32_t* d[4];
 };
 
+// The leaf of a folded value in the next layer's tree (one value per leaf: Blake2s of its 4
+// coordinates, 16 bytes; FriInnerLayerProver::new, crates/stwo/src/prover/fri.rs:357-368), so
+// that a fold also writes the next layer's leaf layer and the next commit starts above it.
+__device__ __forceinline__ void leaf_of(sh::qm31 v, uint32_t* __restrict__ leaves, uint32_t i) {
+  uint32_t m[16], h[8];
+  m[0] = v.a.a;
+  m[1] = v.a.b;
+  m[2] = v.b.a;
+  m[3] = v.b.b;
+#pragma unroll
+  for (int k = 4; k < 16; k++) m[k] = 0;
+  sh::b2_init(h);
+  sh::b2_compress(h, m, 16, true);
+  uint4* o = (uint4*)(leaves + 8ull * i);
+  o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+  o[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
 // Alphas come by value, or from device memory (d_al != nullptr: 4 words per fold, written by an
 // earlier kernel on the stream, e.g. the device-side channel of the FRI driver).
 template <int F>
 __global__ __launch_bounds__(256) void k_fold_line(Cols4 io, const uint32_t* __restrict__ itw, uint32_t K,
-                                                   uint32_t n, Alphas al, const uint32_t* __restrict__ d_al) {
+                                                   uint32_t n, Alphas al, const uint32_t* __restrict__ d_al,
+                                                   uint32_t* __restrict__ leaves) {
   const uint32_t n_out = 1u << (n - F);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
@@ -73,13 +93,14 @@ __global__ __launch_bounds__(256) void k_fold_line(Cols4 io, const uint32_t* __r
     }
   }
   st4(io.d, i, v[0]);
+  if (leaves) leaf_of(v[0], leaves, i);
 }
 
 // y^-1 from the circle layer of the inverse tree; domains of log size <= 2 take
 // host-computed inverses (c0, c1) for outputs 0 and 1.
 __global__ __launch_bounds__(256) void k_fold_circle(Cols4 io, const uint32_t* __restrict__ itw, uint32_t K,
                                                      uint32_t m, sh::qm31 alpha, const uint32_t* __restrict__ d_al,
-                                                     uint32_t c0, uint32_t c1) {
+                                                     uint32_t c0, uint32_t c1, uint32_t* __restrict__ leaves) {
   const uint32_t n_out = 1u << (m - 1);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_out) return;
@@ -93,7 +114,9 @@ __global__ __launch_bounds__(256) void k_fold_circle(Cols4 io, const uint32_t* _
   sh::qm31 a = ld4(io.s, 2 * i), b = ld4(io.s, 2 * i + 1);
   sh::qm31 f0 = sh::qadd(a, b);
   sh::qm31 f1 = sh::qmulm(sh::qsub(a, b), yinv);
-  st4(io.d, i, sh::qadd(f0, sh::qmul(alpha, f1)));
+  const sh::qm31 r = sh::qadd(f0, sh::qmul(alpha, f1));
+  st4(io.d, i, r);
+  if (leaves) leaf_of(r, leaves, i);
 }
 
 // decompose: per-block sums of the first and second halves.
@@ -137,7 +160,8 @@ __global__ __launch_bounds__(256) void k_decompose_apply(Cols4 io, uint32_t n, c
 
 static int fold_line_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4],
                             uint32_t line_initial_index, uint32_t log_size, const Alphas& al, const uint32_t* d_al,
-                            uint32_t fold_step, uint32_t* const dst[4]) {
+                            uint32_t fold_step, uint32_t* const dst[4], uint32_t* leaves = nullptr) {
+  SH_REQUIRE(ctx, !leaves || fold_step == 1, "leaf hashes come with single folds only");
   SH_REQUIRE(ctx, fold_step >= 1 && fold_step <= 4, "fold_step %u not in [1, 4]", fold_step);
   SH_REQUIRE(ctx, log_size >= fold_step && log_size <= 30, "Evaluation too small (log %u, fold_step %u)",
              log_size, fold_step);
@@ -152,16 +176,17 @@ static int fold_line_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* 
   unsigned grid = (n_out + 255) / 256;
   SpanGuard span(ctx, "fri_fold_line");
   switch (fold_step) {
-    case 1: k_fold_line<1><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
-    case 2: k_fold_line<2><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
-    case 3: k_fold_line<3><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
-    default: k_fold_line<4><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al); break;
+    case 1: k_fold_line<1><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al, leaves); break;
+    case 2: k_fold_line<2><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al, leaves); break;
+    case 3: k_fold_line<3><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al, leaves); break;
+    default: k_fold_line<4><<<grid, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, al, d_al, leaves); break;
   }
   return sh_launch_check(ctx, "fold_line");
 }
 
 static int fold_circle_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
-                              sh::qm31 alpha, const uint32_t* d_al, uint32_t* const dst[4]) {
+                              sh::qm31 alpha, const uint32_t* d_al, uint32_t* const dst[4],
+                              uint32_t* leaves = nullptr) {
   SH_REQUIRE(ctx, log_size >= 1 && log_size <= 30, "circle domain log size %u out of range", log_size);
   SH_REQUIRE(ctx, sh::is_doubling_of(sh::subgroup_gen(log_size + 1), log_size - 1, tw->root_initial_index,
                                      tw->root_log_size),
@@ -182,7 +207,7 @@ static int fold_circle_launch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t
   uint32_t n_out = 1u << (log_size - 1);
   SpanGuard span(ctx, "fri_fold_circle");
   k_fold_circle<<<(n_out + 255) / 256, 256, 0, ctx->stream>>>(io, tw->itw, tw->root_log_size, log_size, alpha, d_al,
-                                                             c0, c1);
+                                                             c0, c1, leaves);
   return sh_launch_check(ctx, "fold_circle_into_line");
 }
 
@@ -204,13 +229,16 @@ extern "C" int sh_fold_circle_into_line(sh_ctx* ctx, const sh_twiddles* tw, cons
 
 // The same folds with the alphas read on the device (4 words per fold at d_alphas), for drivers
 // that draw them with a device-side channel and so never wait for the host between layers.
+// leaves (optional, fold_step 1): also the leaf hash of every output value (32 bytes each), the
+// leaf layer of the next FRI layer's tree.
 int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t line_initial_index,
-                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4]) {
-  return fold_line_launch(ctx, tw, src, line_initial_index, log_size, Alphas{}, d_alphas, fold_step, dst);
+                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4],
+                     uint32_t* leaves) {
+  return fold_line_launch(ctx, tw, src, line_initial_index, log_size, Alphas{}, d_alphas, fold_step, dst, leaves);
 }
 int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
-                       const uint32_t* d_alpha, uint32_t* const dst[4]) {
-  return fold_circle_launch(ctx, tw, src, log_size, sh::qzero(), d_alpha, dst);
+                       const uint32_t* d_alpha, uint32_t* const dst[4], uint32_t* leaves) {
+  return fold_circle_launch(ctx, tw, src, log_size, sh::qzero(), d_alpha, dst, leaves);
 }
 
 extern "C" int sh_fri_decompose(sh_ctx* ctx, const uint32_t* const src[4], uint32_t log_size,

@@ -24,9 +24,10 @@ int sh_accumulate_numerators_fl(sh_ctx* ctx, const uint32_t* const* d_cols, uint
                                 const uint32_t* term_random_coeffs, uint64_t n_batches, uint32_t* const* d_outs,
                                 uint32_t* first_linear, const uint8_t* apply_fl);
 int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t line_initial_index,
-                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4]);
+                     uint32_t log_size, const uint32_t* d_alphas, uint32_t fold_step, uint32_t* const dst[4],
+                     uint32_t* leaves = nullptr);
 int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
-                       const uint32_t* d_alpha, uint32_t* const dst[4]);
+                       const uint32_t* d_alpha, uint32_t* const dst[4], uint32_t* leaves = nullptr);
 
 namespace {
 
@@ -249,8 +250,8 @@ struct DeviceTree {
   }
 };
 
-int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t,
-                   bool fetch_root = true) {
+// The layer buffers of a tree of height log_size (pooled, all layers in one allocation).
+int alloc_tree(sh_ctx* ctx, uint32_t log_size, DeviceTree& t) {
   t.L = log_size;
   uint64_t total = 32ull * ((2ull << log_size) - 1);
   t.owner = ctx;
@@ -261,9 +262,19 @@ int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32
     t.layers[k] = t.buf + off;
     off += 32ull << k;
   }
-  std::vector<uint32_t> logs(cols.size(), log_size);
-  SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), cols.size(), log_size, (uint32_t*)t.layers[log_size],
-                              nullptr));
+  return SH_OK;
+}
+
+// leaves_ready: t is allocated already and its leaf layer written (by the fold that made the
+// columns); only the layers above it are built.
+int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t,
+                   bool fetch_root = true, bool leaves_ready = false) {
+  if (!leaves_ready) {
+    SH_TRY(alloc_tree(ctx, log_size, t));
+    std::vector<uint32_t> logs(cols.size(), log_size);
+    SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), cols.size(), log_size,
+                                (uint32_t*)t.layers[log_size], nullptr));
+  }
   SH_TRY(sh_build_layers_impl(ctx, log_size, (uint32_t* const*)t.layers.data()));
   if (!fetch_root) return SH_OK;
   SH_CHECK_HIP(ctx, hipMemcpyAsync(t.root.data(), t.layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
@@ -500,6 +511,7 @@ struct FriLayer {
   DeviceTree tree;
   uint32_t fold_step = 1;  // folds from this layer to the next
   bool pack = false;       // leaves of 4 consecutive values
+  bool leaves_done = false;  // tree allocated and its leaves written by the fold that made this layer
   void free_all() {
     if (buf) sh_dev_free(owner, buf);
     buf = nullptr;
@@ -776,8 +788,25 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   // line folds with alpha^2, alpha^4, ... (one multi-step launch)
   uint32_t line_log = log_size - 1;
   uint32_t line_init = sh::subgroup_gen(line_log + 2);  // half_odds(line_log)
+  // with fold_step 1 a fold also hashes the leaves of the layer it makes, when that layer is
+  // committed by its own launches (the tail workgroup hashes its layers itself)
+  auto leaves_by_fold = [&](uint32_t lg) {
+    return fs == 1 && lg > last_log && !(lg <= kFriTailMaxLog && lg - last_log <= kFriTailMaxLayers);
+  };
+  auto prepare_leaves = [&](FriLayer& l, uint32_t lg, uint32_t** leaves) -> int {
+    *leaves = nullptr;
+    if (!leaves_by_fold(lg)) return SH_OK;
+    SH_TRY(alloc_tree(ctx, lg, l.tree));
+    l.leaves_done = true;
+    *leaves = (uint32_t*)l.tree.layers[lg];
+    return SH_OK;
+  };
   TRY_CLEAN(alloc_eval(cur, log_size - 1));
-  TRY_CLEAN(sh_fold_circle_dev(ctx, tw, column, log_size, d_alphas, out_cols(cur).data()));
+  {
+    uint32_t* leaves;
+    TRY_CLEAN(prepare_leaves(cur, log_size - 1, &leaves));
+    TRY_CLEAN(sh_fold_circle_dev(ctx, tw, column, log_size, d_alphas, out_cols(cur).data(), leaves));
+  }
   if (fs > 1) {
     TRY_CLEAN(alloc_eval(tmp, line_log - (fs - 1)));
     TRY_CLEAN(sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 4, fs - 1, out_cols(tmp).data()));
@@ -829,10 +858,17 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     const uint32_t k = line_log > last_log + fs ? fs : line_log - last_log;
     cur.fold_step = k;
     cur.pack = line_log >= 2 && k > 1;
-    TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
+    if (cur.leaves_done)
+      TRY_CLEAN(commit_columns(ctx, {cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3]}, line_log, cur.tree, false,
+                               true));
+    else
+      TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
     TRY_CLEAN(channel_step(cur.tree, n_layers, k));
     TRY_CLEAN(alloc_eval(tmp, line_log - k));
-    rc = sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 16 * n_layers, k, out_cols(tmp).data());
+    uint32_t* next_leaves = nullptr;
+    if (k == 1) TRY_CLEAN(prepare_leaves(tmp, line_log - 1, &next_leaves));
+    rc = sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 16 * n_layers, k, out_cols(tmp).data(),
+                          next_leaves);
     inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
     cur = FriLayer();
     std::swap(cur, tmp);
