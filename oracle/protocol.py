@@ -312,10 +312,10 @@ def fold_coset(evals, init_idx, log, alpha):
     return ev[0]
 
 
-def fold_circle(evals, init_idx, fold_step, alpha):
+def fold_circle(evals, init_idx, fold_step, alpha, fold_coset_fn=None):
     """SparseEvaluation::fold_circle for one subset (crates/stwo/src/core/fri.rs:725-756): the
     circle-to-line fold on CircleDomain(Coset(init_idx, fold_step - 1)), then fold_step - 1 line
-    folds with alpha^2, alpha^4, ..."""
+    folds with alpha^2, alpha^4, ... (by fold_coset_fn when given)."""
     half = fold_step - 1
     buf = []
     for i in range(1 << half):
@@ -324,13 +324,18 @@ def fold_circle(evals, init_idx, fold_step, alpha):
         yi = O.m31_inv(y)
         f0, f1 = qadd(evals[2 * i], evals[2 * i + 1]), qmulm(qsub(evals[2 * i], evals[2 * i + 1]), yi)
         buf.append(qadd(qmul(alpha, f1), f0))
-    return buf[0] if fold_step == 1 else fold_coset(buf, init_idx, half, qmul(alpha, alpha))
+    return buf[0] if fold_step == 1 else (fold_coset_fn or fold_coset)(buf, init_idx, half, qmul(alpha, alpha))
 
 
-def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn):
+def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn, fold_circle_fn=None,
+               fold_coset_fn=None):
     """FriVerifier::commit / verify_proof_of_work / sample_query_positions / decommit
     (crates/stwo/src/core/fri.rs:156-378, 480-700).  first_layer_query_evals_fn(queries) returns the
-    column's QM31 values at the sorted queries.  Returns the queries, raises on failure."""
+    column's QM31 values at the sorted queries.  fold_circle_fn / fold_coset_fn (same arguments as
+    fold_circle / fold_coset) replace the folds of a subset, for checking another implementation
+    of them.  Returns the queries, raises on failure."""
+    fold_circle_fn = fold_circle_fn or fold_circle
+    fold_coset_fn = fold_coset_fn or fold_coset
     b, fs = params["log_blowup_factor"], params["fold_step"]
     ch.mix_hash(proof["first_layer"]["commitment"])
     col_log = log_degree_bound + b
@@ -399,12 +404,12 @@ def fri_verify(proof, params, ch, log_degree_bound, first_layer_query_evals_fn):
 
     subsets = rebuild(queries, first_layer_query_evals_fn(queries), proof["first_layer"], col_log, fs, pack0)
     half_init = O.canonic_half_initial(col_log)
-    evals = [fold_circle(sub, _circle_domain_index(half_init, col_log - 1, bitrev(start, col_log)), fs, alpha0)
+    evals = [fold_circle_fn(sub, _circle_domain_index(half_init, col_log - 1, bitrev(start, col_log)), fs, alpha0)
              for start, sub in subsets]
     qs = _fold_queries(queries, fs)
     for li, llog, alpha, lp, step, pack in inner:
         subsets = rebuild(qs, evals, lp, llog, step, pack)
-        evals = [fold_coset(sub, (li + (1 << (31 - llog)) * bitrev(start, llog)) % (1 << 31), step, alpha)
+        evals = [fold_coset_fn(sub, (li + (1 << (31 - llog)) * bitrev(start, llog)) % (1 << 31), step, alpha)
                  for start, sub in subsets]
         qs = _fold_queries(qs, step)
     # last layer (fri.rs:320-334)
@@ -731,9 +736,13 @@ def _verify_tree(root, height, col_logs, positions, queried, witness):
     return merkle_verify(root, height, positions, rows, witness)
 
 
-def pcs_verify_values(tree_roots, tree_heights, tree_col_logs, sampled_points, proof, ch, fri):
+def pcs_verify_values(tree_roots, tree_heights, tree_col_logs, sampled_points, proof, ch, fri, answers_fn=None,
+                      **fold_fns):
     """CommitmentSchemeVerifier::verify_values (crates/stwo/src/core/pcs/verifier.rs:57-130).
-    tree_col_logs: extended (blown-up) log sizes per tree. Raises AssertionError on failure."""
+    tree_col_logs: extended (blown-up) log sizes per tree. Raises AssertionError on failure.
+    answers_fn (same arguments as fri_answers) and fold_fns (fri_verify's fold_circle_fn,
+    fold_coset_fn) replace the DEEP quotients and the folds, for checking other implementations."""
+    answers_fn = answers_fn or fri_answers
     ch.mix_felts(np.array([v for t in proof["sampled_values"] for col in t for v in col],
                           np.uint32).reshape(-1))
     rc = [int(v) for v in ch.draw_secure_felt()]
@@ -753,9 +762,9 @@ def pcs_verify_values(tree_roots, tree_heights, tree_col_logs, sampled_points, p
             assert _verify_tree(root, h, logs, pos, proof["queried_values"][i], proof["decommitments"][i]), \
                 f"tree {i} decommitment invalid"
         qv = [col for t in proof["queried_values"] for col in t]
-        return fri_answers(flat_logs, flat_samples, rc, queries, qv, L)
+        return answers_fn(flat_logs, flat_samples, rc, queries, qv, L)
 
-    fri_verify(proof["fri_proof"], fri, ch, L - b, first_layer_evals)
+    fri_verify(proof["fri_proof"], fri, ch, L - b, first_layer_evals, **fold_fns)
     return True
 
 

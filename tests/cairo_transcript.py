@@ -176,10 +176,11 @@ def public_data_parts(pd):
     return public, output, program
 
 
-def replay(proof, channel, verify_pow):
+def replay(proof, channel, verify_pow, until=None):
     """Runs the verifier's channel operations over the proof; returns the proof-of-work checks and
     the channel just before the queries are drawn.  `channel` is a fresh Blake2s channel
-    (mix_u32s, mix_u64, mix_felts, mix_root, digest); `verify_pow(digest, bits, nonce)`."""
+    (mix_u32s, mix_u64, mix_felts, mix_root, digest); `verify_pow(digest, bits, nonce)`.
+    until="oods": stop right after the OODS nonce is mixed, where verify_ex draws the OODS point."""
     ch = channel
     cfg = proof["config"]
     checks = {}
@@ -210,6 +211,8 @@ def replay(proof, channel, verify_pow):
     ch.mix_root(proof["commitments"][3])
     checks["oods_pow"] = verify_pow(ch.digest, OODS_POW_BITS, proof["oods_pow"])
     ch.mix_u64(proof["oods_pow"])
+    if until == "oods":
+        return checks, ch
     # OODS point drawn; verify_values mixes every sampled value, draws the FRI coefficient
     ch.mix_felts([v for tree in proof["sampled_values"] for col in tree for v in col])
     fri = proof["fri"]
@@ -280,3 +283,81 @@ def fri_opening_shape(queries, col_log, params):
         shape.append((n_values, merkle_witness_count(mpos, log - 2 if pack else log)))
         qs = cosets
     return shape
+
+
+# N_TRACE_COLUMNS of every component (crates/cairo-air/src/components/<name>.rs, the constant of
+# that name, lines 5-11): the width of its block of the trace tree.  memory_address_to_id is
+# MEMORY_ADDRESS_TO_ID_SPLIT * 2 (memory_address_to_id.rs:31-32), verify_bitwise_xor_12
+# 1 << (2 * EXPAND_BITS) (verify_bitwise_xor_12.rs:4-8), memory_id_to_big BIG_N_COLUMNS =
+# FELT252_N_WORDS + 1 per big log size (memory_id_to_big.rs:21-23, 217-223).
+TRACE_COLUMNS = {
+    "add_opcode": 103, "add_opcode_small": 39, "add_ap_opcode": 17, "assert_eq_opcode": 12,
+    "assert_eq_opcode_imm": 9, "assert_eq_opcode_double_deref": 19, "blake_compress_opcode": 174,
+    "call_opcode_abs": 25, "call_opcode_rel_imm": 24, "jnz_opcode_non_taken": 9, "jnz_opcode_taken": 47,
+    "jump_opcode_abs": 14, "jump_opcode_double_deref": 21, "jump_opcode_rel": 16, "jump_opcode_rel_imm": 13,
+    "mul_opcode": 130, "mul_opcode_small": 37, "qm_31_add_mul_opcode": 73, "ret_opcode": 16,
+    "verify_instruction": 17, "blake_round": 212, "blake_g": 53, "blake_round_sigma": 1, "triple_xor_32": 21,
+    "verify_bitwise_xor_12": 16, "add_mod_builtin": 267, "bitwise_builtin": 89, "mul_mod_builtin": 426,
+    "pedersen_builtin": 3, "pedersen_builtin_narrow_windows": 3, "poseidon_builtin": 6,
+    "range_check96_builtin": 12, "range_check_builtin": 17, "ec_op_builtin": 273, "partial_ec_mul_generic": 624,
+    "pedersen_aggregator_window_bits_18": 206, "partial_ec_mul_window_bits_18": 297,
+    "pedersen_points_table_window_bits_18": 1, "pedersen_aggregator_window_bits_9": 234,
+    "partial_ec_mul_window_bits_9": 311, "pedersen_points_table_window_bits_9": 1, "poseidon_aggregator": 342,
+    "poseidon_3_partial_rounds_chain": 169, "poseidon_full_round_chain": 126, "cube_252": 141,
+    "poseidon_round_keys": 1, "range_check_252_width_27": 20, "memory_address_to_id": 32,
+    "memory_id_to_big": FELT252_N_WORDS + 1, "memory_id_to_small": 9, "range_check_6": 1, "range_check_8": 1,
+    "range_check_11": 1, "range_check_12": 1, "range_check_18": 2, "range_check_20": 8, "range_check_4_3": 1,
+    "range_check_4_4": 1, "range_check_9_9": 8, "range_check_7_2_5": 1, "range_check_3_6_6_3": 1,
+    "range_check_4_4_4_4": 1, "range_check_3_3_3_3_3": 1, "verify_bitwise_xor_4": 1, "verify_bitwise_xor_7": 1,
+    "verify_bitwise_xor_8": 2, "verify_bitwise_xor_9": 1,
+}
+
+
+def component_instances(claims):
+    """(name, log size) of every component the claim enables, in CairoClaim::log_sizes order
+    (crates/cairo-air/src/claims.rs:383-465, the field order); memory_id_to_big once per big log
+    size (components/memory_id_to_big.rs:217-240)."""
+    out = []
+    for name, kind, payload in claims:
+        if payload is None:
+            continue
+        if kind == "log":
+            out.append((name, payload))
+        elif kind == "big":
+            out += [(name, lg) for lg in payload]
+        else:
+            out.append((name, kind))
+    return out
+
+
+def column_log_sizes(proof):
+    """Log sizes (before blowup) of the trace tree's and the interaction tree's columns, in column
+    order (CairoClaim::log_sizes, claims.rs:383-465).  Trace tree: N_TRACE_COLUMNS per component.
+    Interaction tree: a component's interaction columns end with the 4 coordinate columns of its
+    last cumulative sum, which logup samples at offsets [-1, 0] (crates/constraint-framework/src/
+    lib.rs:208-209) and every other interaction column at [0] (:201-202), so the blocks are read off
+    the sample counts: each ends at its 4th consecutive column with two samples."""
+    inst = component_instances(proof["claims"])
+    trace = [lg for name, lg in inst for _ in range(TRACE_COLUMNS[name])]
+    blocks, cur, twos = [], 0, 0
+    for col in proof["sampled_values"][2]:
+        cur += 1
+        twos = twos + 1 if len(col) == 2 else 0
+        if twos == 4:
+            blocks.append(cur)
+            cur = twos = 0
+    assert cur == 0 and len(blocks) == len(inst), "interaction blocks do not match the components"
+    interaction = [lg for (_, lg), k in zip(inst, blocks) for _ in range(k)]
+    assert len(trace) == len(proof["sampled_values"][1])
+    return trace, interaction
+
+
+def columns_in_order(proof, tree, n_queries, col_logs):
+    """Undoes sort_and_transpose_queried_values (crates/cairo-air/src/utils.rs:221-266): the
+    per-column queried values, column order, from the rows (columns stably sorted by log size)."""
+    rows = tree_rows(proof, tree, n_queries)
+    order = sorted(range(len(col_logs)), key=lambda i: col_logs[i])
+    cols = [None] * len(col_logs)
+    for j, i in enumerate(order):
+        cols[i] = [int(r[j]) for r in rows]
+    return cols

@@ -14,9 +14,18 @@ Checked with this build's code:
     preprocessed one at its own height) against the proof's roots; on the GPU the leaf hashes are
     recomputed by the library's leaf kernel.
   * the FRI layer schedule and opening sizes (fold_step 3, packed leaves) equal those this build's
-    FRI driver produces for the same queries.
-Not checked: the values inside the FRI layer openings, whose first-layer values are the DEEP
-quotients of the Cairo AIR's masks (the AIR is not part of this build).
+    FRI driver produces for the same queries;
+  * the whole FRI decommitment verifies against the proof's roots: the first layer's values at the
+    queries are the DEEP quotients of the queried values (fri_answers with the periodicity samples
+    of two-sample columns and the random-coefficient order across the four trees), then every
+    circle and line fold of fold_step 3 and the last-layer polynomial.  The oracle does this on the
+    CPU; on the GPU the quotient numerators come from the library's numerator kernel and every line
+    fold from its fold kernel.  The sample points come from the proof's own layout: a column
+    sampled once is sampled at the OODS point, one sampled twice is the last cumulative sum of a
+    logup (mask offsets [-1, 0], crates/constraint-framework/src/lib.rs:208-209), and a wrong
+    offset, or a wrong periodicity, fails the first layer's Merkle root.  The column log sizes
+    that order the queried values come from the claim (tests/cairo_transcript.py TRACE_COLUMNS);
+    the Cairo AIR's constraints are not restated.
 """
 import ctypes
 import os
@@ -173,3 +182,125 @@ def test_gpu_leaf_hashes_open_reference_trees(proof, backend, ctx):
         height = pp_height if t == 0 else L
         pos = CT.preprocessed_positions(q, L, pp_height) if t == 0 else q
         assert PR.merkle_verify_leaves(proof["commitments"][t], height, pos, got, proof["decommitments"][t])
+
+
+# ---------------------------------------------------------------- the FRI decommitment
+def _shift(z, g_index, k):
+    """z + k * P(g_index) for k in {-1, 0, 1}: CirclePoint::mul_signed + add
+    (crates/stwo/src/core/circle.rs:101-103, 106-118), P an M31 point, z a QM31 point [x4, y4]."""
+    if k == 0:
+        return list(z)
+    g = O.index_to_point(g_index)
+    gx, gy = [int(g[0]), 0, 0, 0], [int(g[1]) if k > 0 else (CT.P - int(g[1])) % CT.P, 0, 0, 0]
+    x, y = z[:4], z[4:]
+    return PR.qsub(PR.qmul(x, gx), PR.qmul(y, gy)) + PR.qadd(PR.qmul(x, gy), PR.qmul(y, gx))
+
+
+def _pcs_inputs(proof, two_offsets=(-1, 0), interaction_log_shift=0):
+    """Arguments of the oracle's pcs_verify_values for the reference proof: tree roots and heights,
+    extended column log sizes, sample points (mask_points, crates/constraint-framework/src/
+    component.rs:202-213: z + offset * CanonicCoset(L - b).step(); the composition columns at z,
+    crates/stwo/src/core/verifier.rs:98-100), the proof in the oracle's layout, and the channel
+    just before verify_values mixes the sampled values."""
+    L, q, pp_height = _open(proof)
+    b = proof["config"]["log_blowup_factor"]
+    _, ch = CT.replay(proof, _OracleChannel(), O.verify_pow_nonce, until="oods")
+    z = [int(v) for v in ch.random_point()]   # CirclePoint::get_random_point, verifier.rs:95
+    trace, inter = CT.column_log_sizes(proof)
+    npp = len(proof["sampled_values"][0])
+    logs = [[pp_height] * npp, [lg + b for lg in trace], [lg + b + interaction_log_shift for lg in inter], [L] * 8]
+    step = 1 << (31 - (L - b))
+    points = [[[_shift(z, step, k) for k in ((0,) if len(c) == 1 else two_offsets if len(c) == 2 else ())]
+               for c in tree] for tree in proof["sampled_values"]]
+    queried = [CT.columns_in_order(proof, t, len(q), logs[t]) for t in range(4)]
+    fri = proof["fri"]
+
+    def layer(lp):
+        return {"commitment": lp["commitment"], "fri_witness": [list(w) for w in lp["fri_witness"]],
+                "hash_witness": lp["decommitment"]}
+
+    oproof = {"sampled_values": [[[list(v) for v in c] for c in t] for t in proof["sampled_values"]],
+              "queried_values": queried, "decommitments": proof["decommitments"],
+              "fri_proof": {"first_layer": layer(fri["first_layer"]),
+                            "inner_layers": [layer(lp) for lp in fri["inner_layers"]],
+                            "last_layer_poly": [list(v) for v in fri["last_layer_poly"]],
+                            "proof_of_work": fri["proof_of_work"]}}
+    return (proof["commitments"], [pp_height, L, L, L], logs, points, oproof, ch, dict(proof["config"]))
+
+
+def test_fri_decommitment_verifies(proof):
+    """CommitmentSchemeVerifier::verify_values (crates/stwo/src/core/pcs/verifier.rs:55-130) on the
+    reference proof with the oracle's fri_answers and folds: the four trees open, and the FRI
+    first layer, its six inner layers and the last layer accept the values computed here."""
+    assert PR.pcs_verify_values(*_pcs_inputs(proof))
+
+
+@pytest.mark.parametrize("case", ["next_row_mask", "no_prev_row", "periodicity_log"])
+def test_fri_decommitment_rejects_wrong_samples(proof, case):
+    """Each of these changes only the DEEP quotients, and the first FRI layer's root refuses them:
+    the cumulative-sum columns sampled at offsets [0, 1] or [0, 0] instead of [-1, 0], or the
+    periodicity sample taken with the interaction columns' log sizes one too large."""
+    kw = {"next_row_mask": {"two_offsets": (0, 1)}, "no_prev_row": {"two_offsets": (0, 0)},
+          "periodicity_log": {"interaction_log_shift": 1}}[case]
+    with pytest.raises(AssertionError, match="CommitmentInvalid"):
+        PR.pcs_verify_values(*_pcs_inputs(proof, **kw))
+
+
+@pytest.mark.gpu
+def test_gpu_numerators_and_folds_verify_reference_fri(proof, backend, ctx):
+    """The same decommitment with the library doing the arithmetic: the DEEP quotient numerators
+    of every sample batch at the queried rows from sh_accumulate_numerators (the rows as columns,
+    blowup 0), and every line fold of every FRI subset from sh_fold_line on a twiddle tree rooted
+    at that subset's coset.  The row sums over the batches (numerator minus first-linear term times
+    p.y, times the CM31 denominator inverse: accumulate_row_quotients,
+    crates/stwo/src/core/pcs/quotients.rs:155-188) are taken on the host and must equal the
+    oracle's fri_answers; the reference's roots then accept the result."""
+    from proving_amd.backend import BaseColumn, SecureColumnByCoords
+    calls = {"answers": 0, "folds": 0}
+
+    def answers(flat_logs, flat_samples, rc, queries, qv, L):
+        batches = list(PR._batches(PR._build_samples(flat_logs, flat_samples, rc, L)).items())
+        k = max(1, (len(queries) - 1).bit_length())
+        cols = [BaseColumn.from_cpu(ctx, np.pad(np.array(c, np.uint32), (0, (1 << k) - len(queries)))) for c in qv]
+        parts = backend.accumulate_numerators(cols, [(list(pt), terms) for pt, terms in batches], 0)
+        nums = [(pt, part.to_cpu(), [int(x) for x in fl]) for pt, part, fl in parts]
+        for c in cols:
+            c.free()
+        for _, part, _ in parts:
+            part.free()
+        half = O.canonic_half_initial(L)
+        got = []
+        for r, pos in enumerate(queries):
+            px, py = (int(c) for c in PR.circle_domain_at(half, L - 1, PR.bitrev(pos, L)))
+            acc = [0, 0, 0, 0]
+            for pt, num, fl in nums:
+                pt = [int(x) for x in pt]
+                n = PR.qsub([int(num[c][r]) for c in range(4)], PR.qmulm(fl, py))
+                t0 = PR._cmul([(pt[0] - px) % CT.P, pt[1]], [pt[6], pt[7]])
+                t1 = PR._cmul([(pt[4] - py) % CT.P, pt[5]], [pt[2], pt[3]])
+                den = PR._cinv([(t0[0] - t1[0]) % CT.P, (t0[1] - t1[1]) % CT.P])
+                acc = PR.qadd(acc, PR._cmul(n[:2], den) + PR._cmul(n[2:], den))
+            got.append(acc)
+        assert got == PR.fri_answers(flat_logs, flat_samples, rc, queries, qv, L)
+        calls["answers"] += 1
+        return got
+
+    def fold_coset(evals, init_idx, log, alpha):
+        alphas = [list(alpha)]
+        for _ in range(log - 1):
+            alphas.append(PR.qmul(alphas[-1], alphas[-1]))
+        tw = backend.precompute_twiddles(init_idx, log)
+        src = SecureColumnByCoords.from_cpu(ctx, np.array(evals, np.uint32).T.copy())
+        out = backend.fold_line(src, init_idx, alphas, tw)
+        v = [int(x) for x in out.to_cpu()[:, 0]]
+        for c in (src, out, tw):
+            c.free()
+        calls["folds"] += 1
+        return v
+
+    def fold_circle(evals, init_idx, fold_step, alpha):
+        return PR.fold_circle(evals, init_idx, fold_step, alpha, fold_coset_fn=fold_coset)
+
+    assert PR.pcs_verify_values(*_pcs_inputs(proof), answers_fn=answers, fold_circle_fn=fold_circle,
+                                fold_coset_fn=fold_coset)
+    assert calls["answers"] == 1 and calls["folds"] > 6 * 32
