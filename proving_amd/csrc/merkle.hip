@@ -27,6 +27,8 @@ constexpr uint32_t kMsgStride = 17;  // odd stride: conflict-free per-lane messa
 
 struct LeafGroupArgs {
   const uint32_t* const* cols;  // device table of this group's columns
+  const uint32_t* base;         // with stride != 0: column k is base + k * stride (no table reads)
+  uint64_t stride;              // words between consecutive columns, 0 when they are not evenly spaced
   uint32_t cnt;                 // columns in the group
   uint32_t log_n;               // rows = 2^log_n
   const uint32_t* state_in;     // SoA state of the previous group (nullptr for the first)
@@ -42,9 +44,18 @@ __device__ __forceinline__ void load_msg(const uint32_t* lm, uint32_t m[16]) {
   for (int j = 0; j < 16; j++) m[j] = lm[j];
 }
 
+// Column k of a leaf group: with STRIDED the columns lie at a constant stride in one allocation
+// (the address is uniform arithmetic, so the load takes a scalar base and no table read waits).
+template <bool STRIDED>
+__device__ __forceinline__ const uint32_t* group_col(const LeafGroupArgs& a, uint32_t k) {
+  if constexpr (STRIDED) return a.base + (uint64_t)k * a.stride;
+  return a.cols[k];
+}
+
 // One lane per row.  Whole 16-column blocks are loaded straight into registers (16 independent
 // loads in flight) while the previous block is compressed; only a partial block at either end of
 // the group goes through the lane's LDS slot.
+template <bool STRIDED>
 __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
   __shared__ uint32_t msg_lds[kThreads * kMsgStride];
   const uint32_t rows = 1u << a.log_n;
@@ -68,7 +79,7 @@ __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
   uint32_t m[16];
   bool full = false;  // m holds a full block that is compressed once more data is known to follow
   if (pend) {         // complete the block carried in from the previous group
-    while (pend < 16 && k < cnt) lm[pend++] = a.cols[k++][r];
+    while (pend < 16 && k < cnt) lm[pend++] = group_col<STRIDED>(a, k++)[r];
     if (pend == 16) {
       load_msg(lm, m);
       full = true;
@@ -78,7 +89,7 @@ __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
   while (k + 16 <= cnt) {  // (pend is 0 whenever this runs)
     uint32_t n[16];
 #pragma unroll
-    for (int j = 0; j < 16; j++) n[j] = a.cols[k + j][r];
+    for (int j = 0; j < 16; j++) n[j] = group_col<STRIDED>(a, k + j)[r];
     if (full) {
       t += 64;
       sh::b2_compress(h, m, t, false);
@@ -94,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void k_leaves_group(LeafGroupArgs a) {
       sh::b2_compress(h, m, t, false);
       full = false;
     }
-    while (k < cnt) lm[pend++] = a.cols[k++][r];
+    while (k < cnt) lm[pend++] = group_col<STRIDED>(a, k++)[r];
   }
   if (a.state_out) {
     if (full) {  // more data follows in a later group: compress here, at the smaller size
@@ -384,6 +395,16 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
     LeafGroupArgs a;
     a.cols = d_cols + groups[g].first;
     a.cnt = (uint32_t)(groups[g].second - groups[g].first);
+    // evenly spaced columns (one allocation, equal sizes) are addressed as base + k * stride
+    a.base = h_cols[groups[g].first];
+    a.stride = 0;
+    if (a.cnt > 1 && h_cols[groups[g].first + 1] > a.base) {
+      const uint64_t st = (uint64_t)(h_cols[groups[g].first + 1] - a.base);
+      bool even = st >= (1ull << log_sizes[groups[g].first]);
+      for (uint64_t i = groups[g].first + 2; i < groups[g].second && even; i++)
+        even = h_cols[i] - h_cols[i - 1] == (ptrdiff_t)st;
+      if (even) a.stride = st;
+    }
     a.log_n = log_sizes[groups[g].first];
     a.state_in = state_in;
     a.in_log = in_log;
@@ -421,7 +442,11 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
     }
     if (group_ready) SH_CHECK_HIP(ctx, hipStreamWaitEvent(ctx->stream, group_ready[g], 0));
     SpanGuard span(ctx, "merkle_leaves_group");
-    k_leaves_group<<<(unsigned)((rows + kThreads - 1) / kThreads), kThreads, 0, ctx->stream>>>(a);
+    const unsigned grid = (unsigned)((rows + kThreads - 1) / kThreads);
+    if (a.stride)
+      k_leaves_group<true><<<grid, kThreads, 0, ctx->stream>>>(a);
+    else
+      k_leaves_group<false><<<grid, kThreads, 0, ctx->stream>>>(a);
     SH_TRY(sh_launch_check(ctx, "leaves_group"));
     state_in = a.state_out;
     in_log = a.log_n;
