@@ -268,8 +268,10 @@ def bench_prove(args):
     _tune(args, be, [n] * N, w["fri"]["log_blowup_factor"], tree)
     res = {}
 
-    def step():
-        felts, proof, _ = prove_wide_fibonacci(be, tree, trace, n, w["fri"])
+    def step(graph=True):
+        # graph: from the second proof of these buffers on, a replay of the captured graph of the
+        # proof's device work; else every launch issued eagerly
+        felts, proof, _ = prove_wide_fibonacci(be, tree, trace, n, w["fri"], graph_commit=graph)
         res["felts"] = felts
 
     step()
@@ -280,8 +282,11 @@ def bench_prove(args):
     sec = _timed(ctx, step, steps)  # timed without per-launch event spans
     assert np.array_equal(res["felts"], first), "proofs differ between runs"
     ms = sec * 1e3 / steps
-    ctx.profile(True)  # a separate pass with spans, for the kernel table
-    _timed(ctx, step, steps)
+    step(False)
+    eager_ms = _timed(ctx, lambda: step(False), steps) * 1e3 / steps
+    assert np.array_equal(res["felts"], first), "eager and graph proofs differ"
+    ctx.profile(True)  # a separate eager pass with spans, for the kernel table
+    _timed(ctx, lambda: step(False), steps)
     prof = ctx.profile_report()
     ctx.profile(False)
     out = {"metric": "wide-Fibonacci proofs/sec (256 x 2^20, 70 queries, 26-bit grind)",
@@ -289,6 +294,7 @@ def bench_prove(args):
            "ms_per_step": round(ms, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "u32 (M31/QM31)", "data": "synthetic",
            "config": {"workload": w["name"], "columns": N, "log_size": n, "fri": w["fri"]},
+           "eager_ms_per_step": round(eager_ms, 2),
            "kernels": _kernel_table(prof, steps), "proof_felts": len(first)}
     if not args.no_cpu_baseline:
         import oracle as O

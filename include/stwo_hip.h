@@ -66,7 +66,8 @@ int sh_free(sh_ctx *ctx, void *d);
  * crates/stwo/src/prover/mempool.rs:14-58): freed blocks are reused by later
  * allocations of the same rounded size, stream-ordered, without a device sync.
  * A buffer still read by work enqueued on this context may be freed; one that
- * another context or the host is using asynchronously must not be.
+ * another context or the host is using asynchronously must not be.  sh_free of an address the
+ * context did not hand out (or handed out and took back already) releases nothing.
  * sh_pool_trim releases every cached block (synchronising the context first). */
 int sh_pool_trim(sh_ctx *ctx);
 int sh_upload(sh_ctx *ctx, void *d_dst, const void *h_src, uint64_t bytes);
@@ -340,6 +341,21 @@ int sh_commit_trace_mixed_graph_create(sh_ctx *ctx, const sh_twiddles *tw, const
                                        uint32_t *const *d_coeffs, const uint32_t *log_sizes, uint64_t n,
                                        uint32_t log_blowup, uint32_t *const *d_ldes, uint8_t *const *d_layers,
                                        sh_graph **out);
+/* Capture of any sequence of entry points that only enqueue device work (no counterpart in the
+ * reference; the prover captures a proof's device work with it).  sh_graph_create makes an empty
+ * graph that owns a staging ring, scratch buffers and an arena of device blocks.  Between
+ * sh_graph_warmup_begin and sh_graph_warmup_end the calls run eagerly with those resources (every
+ * pooled allocation comes from the arena, and a block freed at any later time goes back to it);
+ * between sh_graph_capture_begin and sh_graph_capture_end the same calls, with the same arguments
+ * and the warm-up's allocations freed again, are recorded instead of run.  A call that waits for
+ * the device or stages a table the warm-up did not stage fails the capture.  sh_graph_launch then
+ * replays the recorded work; sh_graph_destroy frees the arena's blocks, so buffers taken from it
+ * must not be used afterwards.  One graph at a time per context is open; all on the main stream. */
+int sh_graph_create(sh_ctx *ctx, sh_graph **out);
+int sh_graph_warmup_begin(sh_ctx *ctx, sh_graph *g);
+int sh_graph_warmup_end(sh_ctx *ctx, sh_graph *g);
+int sh_graph_capture_begin(sh_ctx *ctx, sh_graph *g);
+int sh_graph_capture_end(sh_ctx *ctx, sh_graph *g);
 int sh_graph_launch(sh_ctx *ctx, sh_graph *g);
 int sh_graph_destroy(sh_ctx *ctx, sh_graph *g);
 

@@ -122,6 +122,23 @@ int sh_fri_prove_dch(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *d_ch, s
                      uint32_t n_trees, const sh_fri_params *params, const uint32_t *const column[4],
                      uint32_t log_size, uint64_t *felts, uint64_t cap, uint64_t *n_felts, uint32_t *query_positions,
                      uint64_t *n_query_positions);
+/* sh_fri_prove_dch in two halves, so that the first can be recorded into a graph
+ * (sh_graph_capture_begin) with the rest of a proof's device work.  sh_fri_commit_dch enqueues
+ * FriProver::commit (crates/stwo/src/prover/fri.rs:113-226) on the device channel d_ch without
+ * waiting for the device; *out receives the run (every layer's evaluation and tree).
+ * sh_fri_decommit_dch waits for that work, then does what sh_fri_prove_dch does after it (the host
+ * replay of the channel, checked against the device's alphas; the last layer; the proof of work;
+ * the queries; one gather for the FRI witnesses and the trees' openings; the felts).  It leaves
+ * the run's buffers in place: after a graph replay of the commit half it may be called again on
+ * the same run.  sh_fri_run_free releases the run. */
+typedef struct sh_fri_run sh_fri_run;
+int sh_fri_commit_dch(sh_ctx *ctx, const sh_twiddles *tw, const uint32_t *d_ch, const uint32_t *d_extra,
+                      uint64_t n_extra, const sh_fri_params *params, const uint32_t *const column[4],
+                      uint32_t log_size, sh_fri_run **out);
+int sh_fri_decommit_dch(sh_ctx *ctx, sh_fri_run *run, sh_channel *ch_out, uint32_t *h_extra, sh_tree_opening *trees,
+                        uint32_t n_trees, uint64_t *felts, uint64_t cap, uint64_t *n_felts, uint32_t *query_positions,
+                        uint64_t *n_query_positions);
+int sh_fri_run_free(sh_ctx *ctx, sh_fri_run *run);
 
 /* ---------------------------------------------------------------- DEEP quotients */
 /* compute_fri_quotients (crates/stwo/src/prover/pcs/quotient_ops.rs:86-160): columns are the

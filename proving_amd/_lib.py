@@ -77,6 +77,11 @@ SIGNATURES = {
                                              ctypes.POINTER(c_vp)]),
     "sh_commit_trace_mixed_graph_create": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp,
                                              ctypes.POINTER(c_vp)]),
+    "sh_graph_create": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "sh_graph_warmup_begin": (c_int, [c_vp, c_vp]),
+    "sh_graph_warmup_end": (c_int, [c_vp, c_vp]),
+    "sh_graph_capture_begin": (c_int, [c_vp, c_vp]),
+    "sh_graph_capture_end": (c_int, [c_vp, c_vp]),
     "sh_graph_launch": (c_int, [c_vp, c_vp]),
     "sh_graph_destroy": (c_int, [c_vp, c_vp]),
     # stwo_prover.h
@@ -102,6 +107,9 @@ SIGNATURES = {
     "sh_fri_quotients_dpoint": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_vp, c_u32, c_u32, c_vp]),
     "sh_fri_prove_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_u32, c_vp, c_vp, c_u32,
                                  c_vp, c_u64, c_u64p, c_vp, c_u64p]),
+    "sh_fri_commit_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_u64, c_vp, c_vp, c_u32, ctypes.POINTER(c_vp)]),
+    "sh_fri_decommit_dch": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp, c_u64, c_u64p, c_vp, c_u64p]),
+    "sh_fri_run_free": (c_int, [c_vp, c_vp]),
 }
 
 _lib = None

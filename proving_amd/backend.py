@@ -8,6 +8,7 @@ Argument meaning and error behaviour follow the reference: where it would
 panic (``assert!``) the C-ABI returns SH_ERR_INVALID and this module raises
 ``HipError``.
 """
+import contextlib
 import ctypes
 import json
 import os
@@ -327,6 +328,9 @@ class HashLayerSet:
     def __iter__(self):
         return iter(self._views)
 
+    def free(self):
+        self.buf.free()
+
 
 class TwiddleTree:
     """TwiddleTree<HipBackend> (crates/stwo/src/prover/poly/twiddles.rs:10-15)."""
@@ -371,6 +375,53 @@ class CommitGraph:
         if self.h is not None and not self.ctx._closed:
             self.ctx.lib.sh_graph_destroy(self.ctx.h, self.h)
         self.h = None
+
+
+class Graph:
+    """A captured sequence of entry points (sh_graph_create and friends): the calls made inside
+    warmup() run eagerly on the graph's own staging ring, scratch and device arena; the same calls
+    made inside capture() (with the warm-up's buffers freed again first) are recorded, and
+    launch() replays them.  Buffers allocated inside either belong to the graph's arena: free()
+    releases them, so objects holding them must be dropped first."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.sh_graph_create(ctx.h, ctypes.byref(h)), "graph_create")
+        self.h = h.value
+        self.captured = False
+
+    def _phase(self, begin, end, what):
+        @contextlib.contextmanager
+        def scope():
+            self.ctx.check(begin(self.ctx.h, self.h), what + "_begin")
+            try:
+                yield self
+            except BaseException:
+                end(self.ctx.h, self.h)  # close the phase; the first error is the one reported
+                raise
+            self.ctx.check(end(self.ctx.h, self.h), what + "_end")
+        return scope()
+
+    def warmup(self):
+        return self._phase(self.ctx.lib.sh_graph_warmup_begin, self.ctx.lib.sh_graph_warmup_end, "graph_warmup")
+
+    def capture(self):
+        lib = self.ctx.lib
+
+        def end(ctx_h, h):
+            rc = lib.sh_graph_capture_end(ctx_h, h)
+            self.captured = rc == 0
+            return rc
+        return self._phase(lib.sh_graph_capture_begin, end, "graph_capture")
+
+    def launch(self):
+        self.ctx.check(self.ctx.lib.sh_graph_launch(self.ctx.h, self.h), "graph_launch")
+
+    def free(self):
+        if self.h:
+            self.ctx.check(self.ctx.lib.sh_graph_destroy(self.ctx.h, self.h), "graph_destroy")
+            self.h = None
 
 
 def canonic_half_initial(log_size):
@@ -753,11 +804,16 @@ class DeviceChannel:
     stream-ordered launches on the backend's context: a proof mixes its roots, nonces and sampled
     values and draws its randomness without the host waiting (dchannel.hip)."""
 
-    def __init__(self, backend, channel=None):
+    def __init__(self, backend, channel=None, load=True):
         self.be = backend
         self.buf = DeviceBuffer(backend.ctx, 64)
+        if load:
+            self.load(channel)
+
+    def load(self, channel=None):
+        """Sets the device state to the host channel's (a fresh channel when None)."""
         st = channel.s if channel is not None else ChannelState()
-        backend._chk(backend.lib.sh_dchannel_load(backend.h, self.buf.ptr, st.digest, st.n_draws), "dchannel_load")
+        self.be._chk(self.be.lib.sh_dchannel_load(self.be.h, self.buf.ptr, st.digest, st.n_draws), "dchannel_load")
 
     @property
     def ptr(self):
@@ -872,28 +928,16 @@ class TreeOpening(ctypes.Structure):
                 ("n_positions", ctypes.c_uint64), ("n_witness", ctypes.c_uint64)]
 
 
-def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extra=None, trees=(), cap=1 << 22):
-    """fri_prove on the device channel: channel_out (Channel) receives the final state.  extra:
-    (device pointer, n words) read back with the commit phase's download.  trees: (cols, layers,
-    preprocessed) of committed trees to open at the queries in the same gather.  Returns (felts as
-    a uint64 array, sorted query positions, the extra words, [(values (n_cols, n_pos), witness
-    (n, 32) uint8) per tree])."""
-    buf = getattr(backend, "_fri_felt_buf", None)
-    if buf is None or len(buf) < cap:
-        buf = backend._fri_felt_buf = np.empty(cap, np.uint64)
-    nf = ctypes.c_uint64()
-    nqmax = max(1, params.n_queries)
-    qp = np.empty(nqmax, np.uint32)
-    nq = ctypes.c_uint64()
-    e_ptr, e_n = extra if extra is not None else (None, 0)
-    h_extra = np.empty(max(1, e_n), np.uint32)
+def _tree_openings(trees, n_queries):
+    """sh_tree_opening records (and the host arrays they point at) for trees given as
+    (cols, layers, preprocessed)."""
     ops = (TreeOpening * max(1, len(trees)))()
     keep = []
     for o, (cols, layers, pre) in zip(ops, trees):
         L = len(layers) - 1
         n = len(cols)
-        values = np.empty((n, nqmax), np.uint32)
-        cap_w = max(1, nqmax * max(1, L))
+        values = np.empty((n, n_queries), np.uint32)
+        cap_w = max(1, n_queries * max(1, L))
         wit = np.empty((cap_w, 32), np.uint8)
         ptrs, logs = _ptrs(cols), _u32s(_logs(cols))
         lp = _ptrs(layers) if L >= 0 else None
@@ -907,17 +951,91 @@ def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extr
         o.values = values.ctypes.data
         o.witness = wit.ctypes.data
         o.witness_cap = cap_w
-    backend._chk(backend.lib.sh_fri_prove_dch(backend.h, twiddles.h, dchannel.ptr, ctypes.byref(channel_out.s),
-                                              e_ptr, e_n, h_extra.ctypes.data, ops, len(trees),
-                                              ctypes.byref(params), _ptrs(column.columns), column.log_size,
-                                              buf.ctypes.data, cap, ctypes.byref(nf), qp.ctypes.data,
-                                              ctypes.byref(nq)), "fri_prove_dch")
+    return ops, keep
+
+
+def _openings(ops, keep):
     openings = []
     for o, (values, wit, *_) in zip(ops, keep):
         k = o.n_positions
         v = np.ascontiguousarray(values.reshape(-1)[:o.n_cols * k].reshape(o.n_cols, k))
         openings.append((v, wit[:o.n_witness]))
-    return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:e_n], openings
+    return openings
+
+
+def _felt_buf(backend, cap):
+    buf = getattr(backend, "_fri_felt_buf", None)
+    if buf is None or len(buf) < cap:
+        # kept on the backend: a fresh buffer would be paged in again on every proof
+        buf = backend._fri_felt_buf = np.empty(cap, np.uint64)
+    return buf
+
+
+def fri_prove_dch(backend, twiddles, dchannel, channel_out, params, column, extra=None, trees=(), cap=1 << 22):
+    """fri_prove on the device channel: channel_out (Channel) receives the final state.  extra:
+    (device pointer, n words) read back with the commit phase's download.  trees: (cols, layers,
+    preprocessed) of committed trees to open at the queries in the same gather.  Returns (felts as
+    a uint64 array, sorted query positions, the extra words, [(values (n_cols, n_pos), witness
+    (n, 32) uint8) per tree])."""
+    buf = _felt_buf(backend, cap)
+    nf = ctypes.c_uint64()
+    nqmax = max(1, params.n_queries)
+    qp = np.empty(nqmax, np.uint32)
+    nq = ctypes.c_uint64()
+    e_ptr, e_n = extra if extra is not None else (None, 0)
+    h_extra = np.empty(max(1, e_n), np.uint32)
+    ops, keep = _tree_openings(trees, nqmax)
+    backend._chk(backend.lib.sh_fri_prove_dch(backend.h, twiddles.h, dchannel.ptr, ctypes.byref(channel_out.s),
+                                              e_ptr, e_n, h_extra.ctypes.data, ops, len(trees),
+                                              ctypes.byref(params), _ptrs(column.columns), column.log_size,
+                                              buf.ctypes.data, cap, ctypes.byref(nf), qp.ctypes.data,
+                                              ctypes.byref(nq)), "fri_prove_dch")
+    return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:e_n], _openings(ops, keep)
+
+
+class FriRun:
+    """One FRI proof between its commit phase, enqueued by fri_commit_dch (sh_fri_commit_dch;
+    recordable into a Graph), and its decommitment."""
+
+    def __init__(self, backend, h, params, n_extra):
+        self.be, self.h, self.params, self.n_extra = backend, h, params, n_extra
+
+    def decommit(self, channel_out, trees=(), cap=1 << 22):
+        """Waits for the commit phase and finishes the proof (sh_fri_decommit_dch): returns what
+        fri_prove_dch returns.  Callable again after a graph replays the commit phase."""
+        be = self.be
+        buf = _felt_buf(be, cap)
+        nf = ctypes.c_uint64()
+        nqmax = max(1, self.params.n_queries)
+        qp = np.empty(nqmax, np.uint32)
+        nq = ctypes.c_uint64()
+        h_extra = np.empty(max(1, self.n_extra), np.uint32)
+        ops, keep = _tree_openings(trees, nqmax)
+        be._chk(be.lib.sh_fri_decommit_dch(be.h, self.h, ctypes.byref(channel_out.s), h_extra.ctypes.data, ops,
+                                           len(trees), buf.ctypes.data, cap, ctypes.byref(nf), qp.ctypes.data,
+                                           ctypes.byref(nq)), "fri_decommit_dch")
+        return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:self.n_extra], _openings(ops, keep)
+
+    def free(self):
+        if self.h:
+            self.be._chk(self.be.lib.sh_fri_run_free(self.be.h, self.h), "fri_run_free")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def fri_commit_dch(backend, twiddles, dchannel, params, column, extra=None):
+    """The commit half of fri_prove_dch (sh_fri_commit_dch): enqueued only; returns a FriRun."""
+    e_ptr, e_n = extra if extra is not None else (None, 0)
+    h = ctypes.c_void_p()
+    backend._chk(backend.lib.sh_fri_commit_dch(backend.h, twiddles.h, dchannel.ptr, e_ptr, e_n,
+                                               ctypes.byref(params), _ptrs(column.columns), column.log_size,
+                                               ctypes.byref(h)), "fri_commit_dch")
+    return FriRun(backend, h.value, params, e_n)
 
 
 def fri_quotients_dpoint(backend, twiddles, columns, point_ptr, values_ptr, random_coeff_ptr, lifting_log_size,

@@ -23,6 +23,20 @@ struct sh_scratch {
   uint64_t bytes = 0;
 };
 
+// The device blocks of one captured graph (graph.hip): while sh_ctx::arena points at it, pooled
+// allocations are served from it, and a block freed at any later time goes back to its own
+// arena, never to the context's pool, so nothing outside the graph reuses memory its replays
+// write.  The warm-up run records the block each allocation received; the capture that follows
+// (the warm-up's blocks freed again) hands out the same blocks in the same order, so every
+// address the captured work bakes in, and every staged table of addresses, is the warm-up's.
+struct sh_arena {
+  std::unordered_map<uint64_t, std::vector<void*>> free;          // size class -> free blocks
+  std::unordered_map<void*, std::pair<uint64_t, bool>> blocks;    // block -> (size class, live)
+  std::vector<void*> seq;  // the blocks the warm-up's allocations received, in order
+  size_t cursor = 0;       // the capture's next allocation receives seq[cursor]
+  int mode = 0;            // 1: warm-up (records seq), 2: capture (replays seq)
+};
+
 struct sh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;     // the stream every entry point orders its work on
@@ -66,6 +80,8 @@ struct sh_ctx {
   std::unordered_map<uint64_t, std::vector<void*>> pool;
   std::unordered_map<void*, uint64_t> pool_live;
   uint64_t pool_bytes = 0;
+  sh_arena* arena = nullptr;                        // set during a graph's warm-up and capture
+  std::unordered_map<void*, sh_arena*> arena_of;    // every live graph's blocks
   // FFT pass shape per transform log size (see fft.hip)
   struct FftTune {
     uint8_t block_log, stride_layers;
@@ -126,6 +142,8 @@ int sh_aux_stream(sh_ctx* ctx, hipStream_t* out);
 // Pooled device allocation for buffers that outlive one launch (see sh_ctx::pool).
 int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out);
 void sh_dev_free(sh_ctx* ctx, void* p);
+// Frees every block of an arena (the caller has synchronised the context's streams).
+void sh_arena_release(sh_ctx* ctx, sh_arena* a);
 // Profiling spans around launches.
 void sh_prof_begin(sh_ctx* ctx, const char* cls, sh_prof_rec* rec);
 void sh_prof_end(sh_ctx* ctx, sh_prof_rec* rec);

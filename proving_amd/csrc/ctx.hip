@@ -319,6 +319,38 @@ int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
   std::lock_guard<std::mutex> lock(ctx->pool_mu);
   *out = nullptr;
   const uint64_t cls = ((bytes ? bytes : 4) + 255) & ~255ull;
+  if (ctx->arena) {
+    sh_arena* a = ctx->arena;
+    auto& fl = a->free[cls];
+    if (a->mode == 2) {  // capture: the block the warm-up's allocation at this point received
+      void* p = a->cursor < a->seq.size() ? a->seq[a->cursor] : nullptr;
+      auto b = p ? a->blocks.find(p) : a->blocks.end();
+      if (b == a->blocks.end() || b->second.first != cls || b->second.second)
+        return sh_fail(ctx, SH_ERR_INVALID,
+                       "graph capture: allocation %zu (%llu bytes) differs from the warm-up's", a->cursor,
+                       (unsigned long long)cls);
+      fl.erase(std::find(fl.begin(), fl.end(), p));
+      b->second.second = true;
+      a->cursor++;
+      *out = p;
+      return SH_OK;
+    }
+    if (!fl.empty()) {
+      *out = fl.back();
+      fl.pop_back();
+      a->blocks[*out].second = true;
+    } else {
+      if (hipMalloc(out, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        *out = nullptr;
+        return sh_fail(ctx, SH_ERR_OOM, "hipMalloc of %llu bytes for a graph failed", (unsigned long long)cls);
+      }
+      a->blocks[*out] = {cls, true};
+      ctx->arena_of[*out] = a;
+    }
+    if (a->mode == 1) a->seq.push_back(*out);
+    return SH_OK;
+  }
   auto it = ctx->pool.find(cls);
   if (it != ctx->pool.end() && !it->second.empty()) {
     *out = it->second.back();
@@ -339,14 +371,20 @@ int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
 void sh_dev_free(sh_ctx* ctx, void* p) {
   if (!p) return;
   std::lock_guard<std::mutex> lock(ctx->pool_mu);
-  auto it = ctx->pool_live.find(p);
-  if (it == ctx->pool_live.end()) {  // not ours: free it the plain way
-    hipStreamSynchronize(ctx->main_stream);
-    for (auto a : ctx->aux)
-      if (a) hipStreamSynchronize(a);
-    hipFree(p);
+  auto ao = ctx->arena_of.find(p);
+  if (ao != ctx->arena_of.end()) {  // a graph's block goes back to that graph
+    auto& b = ao->second->blocks[p];
+    if (b.second) {
+      b.second = false;
+      ao->second->free[b.first].push_back(p);
+    }
     return;
   }
+  auto it = ctx->pool_live.find(p);
+  // Not a live block of the pool or of a live graph: freed already, or a block of a graph that
+  // was destroyed (every such block was released with it).  Nothing is released: the address may
+  // by now belong to a newer allocation.
+  if (it == ctx->pool_live.end()) return;
   const uint64_t cls = it->second;
   ctx->pool_live.erase(it);
   if (ctx->pool_bytes + cls > kPoolCapBytes) {
@@ -358,6 +396,17 @@ void sh_dev_free(sh_ctx* ctx, void* p) {
   }
   ctx->pool[cls].push_back(p);
   ctx->pool_bytes += cls;
+}
+
+void sh_arena_release(sh_ctx* ctx, sh_arena* a) {
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
+  for (auto& kv : a->blocks) {
+    hipFree(kv.first);
+    ctx->arena_of.erase(kv.first);
+  }
+  a->blocks.clear();
+  a->free.clear();
+  if (ctx->arena == a) ctx->arena = nullptr;
 }
 
 extern "C" int sh_alloc(sh_ctx* ctx, uint64_t bytes, void** d_out) {

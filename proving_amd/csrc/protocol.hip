@@ -386,8 +386,8 @@ __global__ __launch_bounds__(kFriTailThreads) void k_fri_tail(FriTailArgs a) {
   __shared__ uint32_t alpha_w[4];
   const uint32_t tid = threadIdx.x, q = tid & 3, quad = tid >> 2;
   constexpr uint32_t kQuads = kFriTailThreads / 4;
-  uint32_t sidx[40];
-  quad_sigma(q, sidx);
+  uint32_t sidx[5];
+  quad_sigma_packed(q, sidx);  // packed: the 1024-lane workgroup leaves 128 VGPRs per lane
   uint32_t h_lo, h_hi, d_leaf, d_node, d_draw;
   quad_init(q, 16, h_lo, h_hi, d_leaf);  // a leaf: the 4 coordinates, 16 bytes
   quad_init(q, 64, h_lo, h_hi, d_node);  // a node, or digest || root
@@ -678,19 +678,44 @@ extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, cons
 }
 
 // ------------------------------------------------------------------ FRI prover
+// The state of one FRI proof between its commit phase (enqueued only) and its decommitment: every
+// layer's evaluation and tree, and the device words of the commit phase's one download.
+struct sh_fri_run {
+  sh_ctx* owner = nullptr;
+  sh_fri_params p{};
+  uint32_t log_size = 0, fs = 0, last_log = 0, max_layers = 0, n_layers = 0;
+  uint32_t last_line_init = 0;  // the last layer's line domain
+  bool from_dch = false;        // the proof's channel is a device channel (copied to CH)
+  sh_channel ch_in{};           // the host channel the commit started from (host-channel proofs)
+  uint64_t n_extra = 0;
+  uint32_t* d_aux = nullptr;
+  // word offsets in d_aux: the channel, the alphas (16 words per layer), the roots (8 per layer),
+  // the initial channel; then the last layer's evaluation and the extra words (the download)
+  uint64_t ALPHAS = 0, ROOTS = 0, INIT = 0, EVAL = 0, EXTRA = 0, dl_words = 0;
+  FriLayer first, cur;
+  std::vector<FriLayer> inner;
+  void free_all() {
+    first.tree.free_all();
+    for (auto& l : inner) l.free_all();
+    inner.clear();
+    cur.free_all();
+    if (d_aux) sh_dev_free(owner, d_aux);
+    d_aux = nullptr;
+  }
+};
+
+// FriProver::commit (crates/stwo/src/prover/fri.rs:113-226), enqueued without waiting for the
+// device: each layer's root is mixed into a device copy of the channel and the folding alphas are
+// drawn there (k_fri_channel), so the folds read them from device memory.  The host replays the
+// same channel operations afterwards, from the downloaded roots, and checks that the alphas agree.
 // d_ch_in: when set, the proof's channel is this device channel (16 words, as dchannel.hip keeps
-// it): the driver starts from it, and ch receives its state (read back with the commit phase's
-// download) before the host replays the layers.  d_extra / h_extra: n_extra words of the caller's
-// device memory that the same download brings back.
-// trees: committed trees opened at the queries with the FRI decommitment, in the same gather.
-static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const uint32_t* d_ch_in,
-                          const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, sh_tree_opening* trees,
-                          uint32_t n_trees, const sh_fri_params* p, const uint32_t* const column[4],
-                          uint32_t log_size, uint64_t* felts, uint64_t cap, uint64_t* n_felts,
-                          uint32_t* query_positions, uint64_t* n_query_positions) {
-  if (!ctx || !tw || !ch || !p || !column || !n_felts || !n_query_positions || (n_extra && (!d_extra || !h_extra)) ||
-      (n_trees && !trees))
-    return SH_ERR_INVALID;
+// it); else ch is staged.  d_extra: n_extra words of the caller's device memory that the commit
+// phase's download also brings back.
+static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel* ch, const uint32_t* d_ch_in,
+                           const uint32_t* d_extra, uint64_t n_extra, const sh_fri_params* p,
+                           const uint32_t* const column[4], uint32_t log_size, sh_fri_run** out) {
+  if (!ctx || !tw || !p || !column || !out || (!ch && !d_ch_in) || (n_extra && !d_extra)) return SH_ERR_INVALID;
+  *out = nullptr;
   SH_REQUIRE(ctx, n_extra < (1ull << 28), "too many extra words (%llu)", (unsigned long long)n_extra);
   SH_REQUIRE(ctx, p->fold_step >= 1 && p->fold_step <= 4, "fold_step %u out of range [1, 4]", p->fold_step);
   SH_REQUIRE(ctx, p->log_blowup_factor >= 1 && p->log_blowup_factor <= 16 && p->log_last_layer_degree_bound <= 10,
@@ -699,66 +724,60 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   const uint32_t last_log = p->log_last_layer_degree_bound + p->log_blowup_factor;
   SH_REQUIRE(ctx, log_size >= 2 && log_size <= 30 && log_size >= fs + last_log,
              "The circle-to-line fold results in a smaller line domain than the last layer.");
+  sh_fri_run* run = new sh_fri_run();
+  run->owner = ctx;
+  run->p = *p;
+  run->log_size = log_size;
+  run->fs = fs;
+  run->last_log = last_log;
+  run->from_dch = d_ch_in != nullptr;
+  if (ch) run->ch_in = *ch;
+  run->n_extra = n_extra;
+  const uint32_t max_layers = run->max_layers = log_size + 1;
+  run->ALPHAS = 16;
+  run->ROOTS = run->ALPHAS + 16ull * max_layers;
+  run->INIT = run->ROOTS + 8ull * max_layers;
+  run->EVAL = run->INIT + 16;
+  run->EXTRA = run->EVAL + (4ull << last_log);
+  run->dl_words = run->EXTRA + n_extra;
   int rc = SH_OK;
-  FriLayer first;
-  std::vector<FriLayer> inner;
-  FriLayer cur;  // the evaluation being folded (not committed yet)
-  FriLayer tmp;  // ping-pong buffer of a multi-step fold
-  std::vector<sh::qm31> last_coeffs;
-  auto cleanup = [&]() {
-    first.tree.free_all();
-    for (auto& l : inner) l.free_all();
-    cur.free_all();
-    tmp.free_all();
-  };
-#define TRY_CLEAN(expr)    \
-  do {                     \
-    rc = (expr);           \
-    if (rc != SH_OK) {     \
-      cleanup();           \
-      return rc;           \
-    }                      \
+#define TRY_RUN(expr)     \
+  do {                    \
+    rc = (expr);          \
+    if (rc != SH_OK) {    \
+      run->free_all();    \
+      delete run;         \
+      return rc;          \
+    }                     \
   } while (0)
-  // The commit phase runs without waiting for the host: each layer's root is mixed into a device
-  // copy of the channel and the folding alphas are drawn there (k_fri_channel), so the folds read
-  // them from device memory.  The host replays the same channel operations afterwards, from the
-  // downloaded roots, and checks that the alphas agree.
-  const uint32_t max_layers = log_size + 1;
-  // [ch: 16 words][alphas: 16 words per layer][roots: 8 words per layer][initial ch: 16 words]
-  // [last layer: 4 x 2^last_log words][extra words]: the commit phase's one download
-  const uint64_t aux_words = 16 + 24 * max_layers + 16;
-  const uint64_t dl_words = aux_words + (4ull << last_log) + n_extra;
-  uint32_t* d_aux = nullptr;
-  TRY_CLEAN(sh_dev_alloc(ctx, 4 * dl_words, (void**)&d_aux));
-  uint32_t* d_ch = d_aux;
-  uint32_t* d_alphas = d_aux + 16;
-  uint32_t* d_roots = d_alphas + 16 * max_layers;
-  uint32_t* d_init = d_roots + 8 * max_layers;
-  auto cleanup_all = [&]() {
-    cleanup();
-    if (d_aux) sh_dev_free(ctx, d_aux);
-    d_aux = nullptr;
-  };
-#undef TRY_CLEAN
-#define TRY_CLEAN(expr)    \
-  do {                     \
-    rc = (expr);           \
-    if (rc != SH_OK) {     \
-      cleanup_all();       \
-      return rc;           \
-    }                      \
+#define CHECK_RUN(expr)                                                                         \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess)                                                                       \
+      TRY_RUN(sh_fail(ctx, SH_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(_e)));        \
   } while (0)
+  TRY_RUN(sh_dev_alloc(ctx, 4 * run->dl_words, (void**)&run->d_aux));
+  uint32_t* const d_ch = run->d_aux;
+  uint32_t* const d_alphas = run->d_aux + run->ALPHAS;
+  uint32_t* const d_roots = run->d_aux + run->ROOTS;
   if (d_ch_in) {
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ch, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_init, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
+    CHECK_RUN(hipMemcpyAsync(d_ch, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
+    CHECK_RUN(hipMemcpyAsync(run->d_aux + run->INIT, d_ch_in, 64, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
     std::vector<uint32_t> h_ch(16, 0);
     memcpy(h_ch.data(), ch->digest, 32);
     h_ch[8] = ch->n_draws;
     void* staged;
-    TRY_CLEAN(sh_stage(ctx, h_ch.data(), 64, &staged));
-    SH_CHECK_HIP(ctx, hipMemcpyAsync(d_ch, staged, 64, hipMemcpyDeviceToDevice, ctx->stream));
+    TRY_RUN(sh_stage(ctx, h_ch.data(), 64, &staged));
+    CHECK_RUN(hipMemcpyAsync(d_ch, staged, 64, hipMemcpyDeviceToDevice, ctx->stream));
   }
+  FriLayer& first = run->first;
+  FriLayer& cur = run->cur;  // the evaluation being folded (not committed yet)
+  FriLayer tmp;              // ping-pong buffer of a multi-step fold
+  auto fail_tmp = [&](int r) {  // on an error, tmp (not yet owned by the run) is released too
+    if (r != SH_OK) tmp.free_all();
+    return r;
+  };
   auto channel_step = [&](const DeviceTree& t, uint32_t layer, uint32_t k) -> int {
     k_fri_channel<<<1, 64, 0, ctx->stream>>>((const uint32_t*)t.layers[0], d_ch, k, d_alphas + 16 * layer,
                                               d_roots + 8 * layer);
@@ -782,8 +801,8 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   first.log_size = log_size;
   first.fold_step = fs;
   first.pack = log_size >= 2 && fs > 1;
-  TRY_CLEAN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree, false));
-  TRY_CLEAN(channel_step(first.tree, 0, fs));
+  TRY_RUN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree, false));
+  TRY_RUN(channel_step(first.tree, 0, fs));
   // commit_inner_layers (fri.rs:145-196): the circle-to-line fold with alpha, then fold_step - 1
   // line folds with alpha^2, alpha^4, ... (one multi-step launch)
   uint32_t line_log = log_size - 1;
@@ -801,15 +820,16 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     *leaves = (uint32_t*)l.tree.layers[lg];
     return SH_OK;
   };
-  TRY_CLEAN(alloc_eval(cur, log_size - 1));
+  TRY_RUN(alloc_eval(cur, log_size - 1));
   {
     uint32_t* leaves;
-    TRY_CLEAN(prepare_leaves(cur, log_size - 1, &leaves));
-    TRY_CLEAN(sh_fold_circle_dev(ctx, tw, column, log_size, d_alphas, out_cols(cur).data(), leaves));
+    TRY_RUN(prepare_leaves(cur, log_size - 1, &leaves));
+    TRY_RUN(sh_fold_circle_dev(ctx, tw, column, log_size, d_alphas, out_cols(cur).data(), leaves));
   }
   if (fs > 1) {
-    TRY_CLEAN(alloc_eval(tmp, line_log - (fs - 1)));
-    TRY_CLEAN(sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 4, fs - 1, out_cols(tmp).data()));
+    TRY_RUN(fail_tmp(alloc_eval(tmp, line_log - (fs - 1))));
+    TRY_RUN(fail_tmp(
+        sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 4, fs - 1, out_cols(tmp).data())));
     cur.free_all();  // stream-ordered release: the fold that reads it is already queued
     std::swap(cur, tmp);
     for (uint32_t j = 0; j < fs - 1; j++) line_init = sh::idx_reduce((uint64_t)line_init * 2);
@@ -819,8 +839,9 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   while (line_log > last_log) {
     if (fs == 1 && line_log <= kFriTailMaxLog && line_log - last_log <= kFriTailMaxLayers) {
       // every remaining layer in one launch (k_fri_tail)
-      SH_REQUIRE(ctx, sh::is_doubling_of(line_init, line_log, tw->root_initial_index, tw->root_log_size),
-                 "line domain (initial %u, log %u) is not in the twiddle tower", line_init, line_log);
+      if (!sh::is_doubling_of(line_init, line_log, tw->root_initial_index, tw->root_log_size))
+        TRY_RUN(sh_fail(ctx, SH_ERR_INVALID, "line domain (initial %u, log %u) is not in the twiddle tower", line_init,
+                        line_log));
       FriTailArgs ta = {};
       ta.n_layers = line_log - last_log;
       ta.ch = d_ch;
@@ -833,13 +854,13 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
         DeviceTree& t = cur.tree;
         t.L = lg;
         t.owner = ctx;
-        TRY_CLEAN(sh_dev_alloc(ctx, 32ull * ((2ull << lg) - 1), (void**)&t.buf));
+        TRY_RUN(sh_dev_alloc(ctx, 32ull * ((2ull << lg) - 1), (void**)&t.buf));
         t.layers.resize(lg + 1);
         for (uint32_t k = 0; k <= lg; k++) t.layers[k] = t.buf + 32ull * ((1ull << k) - 1);
-        TRY_CLEAN(alloc_eval(tmp, lg - 1));
+        TRY_RUN(fail_tmp(alloc_eval(tmp, lg - 1)));
         ta.layer[j] = {cur.buf, t.buf, tw->itw + (1ull << tw->root_log_size) - (1ull << lg), lg, n_layers + j};
         ta.next_eval[j] = tmp.buf;
-        inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
+        run->inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
         cur = FriLayer();
         std::swap(cur, tmp);
         line_init = sh::idx_reduce((uint64_t)line_init * 2);
@@ -847,7 +868,7 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
       {
         SpanGuard span(ctx, "fri_tail");
         k_fri_tail<<<1, kFriTailThreads, 0, ctx->stream>>>(ta);
-        TRY_CLEAN(sh_launch_check(ctx, "fri_tail"));
+        TRY_RUN(sh_launch_check(ctx, "fri_tail"));
       }
       n_layers += ta.n_layers;
       line_log = last_log;
@@ -859,85 +880,106 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     cur.fold_step = k;
     cur.pack = line_log >= 2 && k > 1;
     if (cur.leaves_done)
-      TRY_CLEAN(commit_columns(ctx, {cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3]}, line_log, cur.tree, false,
-                               true));
+      TRY_RUN(commit_columns(ctx, {cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3]}, line_log, cur.tree, false,
+                             true));
     else
-      TRY_CLEAN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
-    TRY_CLEAN(channel_step(cur.tree, n_layers, k));
-    TRY_CLEAN(alloc_eval(tmp, line_log - k));
+      TRY_RUN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
+    TRY_RUN(channel_step(cur.tree, n_layers, k));
+    TRY_RUN(fail_tmp(alloc_eval(tmp, line_log - k)));
     uint32_t* next_leaves = nullptr;
-    if (k == 1) TRY_CLEAN(prepare_leaves(tmp, line_log - 1, &next_leaves));
+    if (k == 1) TRY_RUN(fail_tmp(prepare_leaves(tmp, line_log - 1, &next_leaves)));
     rc = sh_fold_line_dev(ctx, tw, cur.cols, line_init, line_log, d_alphas + 16 * n_layers, k, out_cols(tmp).data(),
                           next_leaves);
-    inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
+    run->inner.push_back(cur);  // keeps this layer's evaluation and tree for the decommitment
     cur = FriLayer();
     std::swap(cur, tmp);
-    if (rc != SH_OK) {
-      cleanup_all();
-      return rc;
-    }
+    TRY_RUN(rc);
     for (uint32_t j = 0; j < k; j++) line_init = sh::idx_reduce((uint64_t)line_init * 2);
     line_log -= k;
     n_layers++;
   }
-  // one download: the device channel, the alphas and roots of every layer, the last layer
+  run->n_layers = n_layers;
+  run->last_line_init = line_init;
+  // one download: the device channel, the alphas and roots of every layer, the last layer, the
+  // extra words (packed here; the decommitment copies them out)
   {
     PackArgs pa = {};
     for (int c = 0; c < 4; c++) {
       pa.src[c] = cur.cols[c];
       pa.len[c] = 1u << last_log;
-      pa.dst_off[c] = (uint32_t)(aux_words + ((uint64_t)c << last_log));
+      pa.dst_off[c] = (uint32_t)(run->EVAL + ((uint64_t)c << last_log));
     }
     pa.n = 4;
     if (n_extra) {
       pa.src[4] = d_extra;
       pa.len[4] = (uint32_t)n_extra;
-      pa.dst_off[4] = (uint32_t)(aux_words + (4ull << last_log));
+      pa.dst_off[4] = (uint32_t)run->EXTRA;
       pa.n = 5;
     }
     const uint64_t longest = std::max<uint64_t>(1ull << last_log, n_extra);
-    k_pack<<<(unsigned)std::min<uint64_t>(256, (longest + 255) / 256), 256, 0, ctx->stream>>>(pa, d_aux);
-    TRY_CLEAN(sh_launch_check(ctx, "pack"));
+    k_pack<<<(unsigned)std::min<uint64_t>(256, (longest + 255) / 256), 256, 0, ctx->stream>>>(pa, run->d_aux);
+    TRY_RUN(sh_launch_check(ctx, "pack"));
   }
+#undef TRY_RUN
+#undef CHECK_RUN
+  *out = run;
+  return SH_OK;
+}
+
+// FriProver::decommit (crates/stwo/src/prover/fri.rs:229-284) after fri_commit_impl: waits for the
+// commit phase's download, replays the channel on the host (ch: the final state), interpolates
+// the last layer, grinds the proof of work, draws the queries and reads every layer's witness
+// values and Merkle witness nodes, and the committed trees' openings, with one gather.  Leaves the
+// run's buffers in place (a replayed graph writes them again).
+static int fri_decommit_impl(sh_ctx* ctx, sh_fri_run* run, sh_channel* ch, uint32_t* h_extra, sh_tree_opening* trees,
+                             uint32_t n_trees, uint64_t* felts, uint64_t cap, uint64_t* n_felts,
+                             uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (!ctx || !run || !ch || !n_felts || !n_query_positions || (run->n_extra && !h_extra) || (n_trees && !trees))
+    return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, run->owner == ctx && run->d_aux, "FRI run of another context, or freed");
+  const sh_fri_params* p = &run->p;
+  const uint32_t fs = run->fs, last_log = run->last_log, log_size = run->log_size, max_layers = run->max_layers;
   uint8_t* h_dl;
-  TRY_CLEAN(sh_host_xfer(ctx, 4 * dl_words, &h_dl));
-  SH_CHECK_HIP(ctx, hipMemcpyAsync(h_dl, d_aux, 4 * dl_words, hipMemcpyDeviceToHost, ctx->stream));
+  SH_TRY(sh_host_xfer(ctx, 4 * run->dl_words, &h_dl));
+  SH_CHECK_HIP(ctx, hipMemcpyAsync(h_dl, run->d_aux, 4 * run->dl_words, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<uint32_t> aux(aux_words), words(4ull << last_log);
-  memcpy(aux.data(), h_dl, 4 * aux_words);
-  memcpy(words.data(), h_dl + 4 * aux_words, 16ull << last_log);
-  if (n_extra) memcpy(h_extra, h_dl + 4 * (aux_words + (4ull << last_log)), 4 * n_extra);
-  if (d_ch_in) {
-    memcpy(ch->digest, &aux[16 + 24 * max_layers], 32);
-    ch->n_draws = aux[16 + 24 * max_layers + 8];
+  const uint32_t* aux = (const uint32_t*)h_dl;
+  if (run->n_extra) memcpy(h_extra, aux + run->EXTRA, 4 * run->n_extra);
+  if (run->from_dch) {
+    memcpy(ch->digest, aux + run->INIT, 32);
+    ch->n_draws = aux[run->INIT + 8];
+  } else {
+    *ch = run->ch_in;
   }
-  for (uint32_t l = 0; l < n_layers; l++) {
-    DeviceTree& t = l == 0 ? first.tree : inner[l - 1].tree;
-    memcpy(t.root.data(), &aux[16 + 16 * max_layers + 8 * l], 32);
-    sh_channel_mix_root(ch, t.root.data());
+  std::vector<const FriLayer*> layers_all;
+  layers_all.push_back(&run->first);
+  for (auto& l : run->inner) layers_all.push_back(&l);
+  SH_REQUIRE(ctx, layers_all.size() == run->n_layers, "FRI run holds %zu layers, expected %u", layers_all.size(),
+             run->n_layers);
+  std::vector<Hash> roots(run->n_layers);
+  for (uint32_t l = 0; l < run->n_layers; l++) {
+    memcpy(roots[l].data(), aux + run->ROOTS + 8 * l, 32);
+    sh_channel_mix_root(ch, roots[l].data());
     uint32_t a[4];
     sh_channel_draw_secure_felt(ch, a);
-    if (memcmp(a, &aux[16 + 16 * l], 16) != 0) {
-      cleanup_all();
+    if (memcmp(a, aux + run->ALPHAS + 16 * l, 16) != 0)
       return sh_fail(ctx, SH_ERR_INVALID, "device-side FRI channel diverged from the host channel at layer %u", l);
-    }
   }
   // commit_last_layer (fri.rs:208-226)
+  std::vector<sh::qm31> last_coeffs;
   {
+    const uint32_t* words = aux + run->EVAL;
     std::vector<sh::qm31> v(1ull << last_log);
     for (uint64_t i = 0; i < v.size(); i++)
       v[i] = {{words[i], words[v.size() + i]}, {words[2 * v.size() + i], words[3 * v.size() + i]}};
     bit_reverse_q(v);  // LineEvaluation::interpolate (crates/stwo/src/prover/line.rs:54-65)
-    line_ifft(v, line_init, last_log);
+    line_ifft(v, run->last_line_init, last_log);
     uint32_t inv = sh::minv((uint32_t)v.size());
     for (auto& x : v) x = sh::qmulm(x, inv);
     bit_reverse_q(v);  // into_ordered_coefficients
     const uint64_t bound = 1ull << p->log_last_layer_degree_bound;
     for (uint64_t i = bound; i < v.size(); i++)
-      if (!sh::qeq(v[i], sh::qzero())) {
-        cleanup_all();
-        return sh_fail(ctx, SH_ERR_INVALID, "invalid degree");
-      }
+      if (!sh::qeq(v[i], sh::qzero())) return sh_fail(ctx, SH_ERR_INVALID, "invalid degree");
     v.resize(bound);
     bit_reverse_q(v);  // LinePoly::from_ordered_coefficients stores bit-reversed coefficients
     last_coeffs = v;
@@ -945,9 +987,9 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     for (size_t i = 0; i < v.size(); i++) sh::qstore(v[i], &fw[4 * i]);
     sh_channel_mix_felts(ch, fw.data(), v.size());
   }
-  // decommit (fri.rs:229-251)
+  // decommit (fri.rs:229-251): the proof of work
   uint64_t nonce = 0;
-  TRY_CLEAN(sh_grind(ctx, ch->digest, p->pow_bits, &nonce));
+  SH_TRY(sh_grind(ctx, ch->digest, p->pow_bits, &nonce));
   sh_channel_mix_u64(ch, nonce);
   std::vector<uint32_t> q(p->n_queries);
   sh_channel_draw_queries(ch, log_size, p->n_queries, q.data());
@@ -955,9 +997,6 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   q.erase(std::unique(q.begin(), q.end()), q.end());
   // decommit_on_queries (fri.rs:253-284): queries fold by each layer's own step.  Every layer's
   // witness values and Merkle witness nodes are read with one gather.
-  std::vector<const FriLayer*> layers_all;
-  layers_all.push_back(&first);
-  for (auto& l : inner) layers_all.push_back(&l);
   struct Plan {
     std::vector<uint32_t> witness_pos;
     std::vector<std::pair<uint32_t, uint32_t>> nodes;
@@ -1018,16 +1057,14 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     tree_node0[t] = gn.size();
     tree_npos[t] = pos.size();
     const size_t jobs_before = jobs.size();
-    TRY_CLEAN(plan_tree_opening(ctx, T.d_cols, T.log_sizes, T.n_cols, T.d_layers, T.L, std::move(pos), gn, jobs,
-                                &tree_nodes[t]));
-    if (tree_nodes[t] > T.witness_cap || (jobs.size() > jobs_before && !T.values)) {
-      cleanup_all();
+    SH_TRY(plan_tree_opening(ctx, T.d_cols, T.log_sizes, T.n_cols, T.d_layers, T.L, std::move(pos), gn, jobs,
+                             &tree_nodes[t]));
+    if (tree_nodes[t] > T.witness_cap || (jobs.size() > jobs_before && !T.values))
       return sh_fail(ctx, SH_ERR_INVALID, "tree %u: witness of %llu hashes exceeds the buffer or no value buffer", t,
                      (unsigned long long)tree_nodes[t]);
-    }
   }
   std::vector<uint32_t> gw, gnw, gv;
-  TRY_CLEAN(gather_all(ctx, gc, gr, gn, jobs, gw, gnw, gv));
+  SH_TRY(gather_all(ctx, gc, gr, gn, jobs, gw, gnw, gv));
   {
     uint64_t vo = 0;
     for (uint32_t t = 0; t < n_trees; t++) {
@@ -1055,15 +1092,14 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
     f.u(nnodes);
     f.v.insert(f.v.end(), gnw.begin() + noff, gnw.begin() + noff + 8 * nnodes);
     noff += 8 * nnodes;
-    f.h(layers_all[li]->tree.root);
+    f.h(roots[li]);
   }
   if (layers_all.size() == 1) f.u(0);
   f.u(last_coeffs.size());
   for (auto& x : last_coeffs) f.q(x);
   f.u(p->log_last_layer_degree_bound);
   f.u(nonce);
-  cleanup_all();
-#undef TRY_CLEAN
+  (void)max_layers;
   *n_felts = f.v.size();
   *n_query_positions = q.size();
   SH_REQUIRE(ctx, f.v.size() <= cap, "FRI proof of %zu felts exceeds the buffer (%llu)", f.v.size(),
@@ -1071,6 +1107,21 @@ static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, co
   memcpy(felts, f.v.data(), 8 * f.v.size());
   if (query_positions) memcpy(query_positions, q.data(), 4 * q.size());
   return SH_OK;
+}
+
+static int fri_prove_impl(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const uint32_t* d_ch_in,
+                          const uint32_t* d_extra, uint64_t n_extra, uint32_t* h_extra, sh_tree_opening* trees,
+                          uint32_t n_trees, const sh_fri_params* p, const uint32_t* const column[4],
+                          uint32_t log_size, uint64_t* felts, uint64_t cap, uint64_t* n_felts,
+                          uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (!ctx || !ch || !n_felts || !n_query_positions || (n_extra && !h_extra)) return SH_ERR_INVALID;
+  sh_fri_run* run = nullptr;
+  SH_TRY(fri_commit_impl(ctx, tw, d_ch_in ? nullptr : ch, d_ch_in, d_extra, n_extra, p, column, log_size, &run));
+  const int rc = fri_decommit_impl(ctx, run, ch, h_extra, trees, n_trees, felts, cap, n_felts, query_positions,
+                                   n_query_positions);
+  run->free_all();
+  delete run;
+  return rc;
 }
 
 extern "C" int sh_fri_prove(sh_ctx* ctx, const sh_twiddles* tw, sh_channel* ch, const sh_fri_params* p,
@@ -1088,6 +1139,29 @@ extern "C" int sh_fri_prove_dch(sh_ctx* ctx, const sh_twiddles* tw, const uint32
   if (!d_ch) return SH_ERR_INVALID;
   return fri_prove_impl(ctx, tw, ch_out, d_ch, d_extra, n_extra, h_extra, trees, n_trees, p, column, log_size, felts,
                         cap, n_felts, query_positions, n_query_positions);
+}
+
+extern "C" int sh_fri_commit_dch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* d_ch, const uint32_t* d_extra,
+                                 uint64_t n_extra, const sh_fri_params* p, const uint32_t* const column[4],
+                                 uint32_t log_size, sh_fri_run** out) {
+  if (!d_ch) return SH_ERR_INVALID;
+  return fri_commit_impl(ctx, tw, nullptr, d_ch, d_extra, n_extra, p, column, log_size, out);
+}
+
+extern "C" int sh_fri_decommit_dch(sh_ctx* ctx, sh_fri_run* run, sh_channel* ch_out, uint32_t* h_extra,
+                                   sh_tree_opening* trees, uint32_t n_trees, uint64_t* felts, uint64_t cap,
+                                   uint64_t* n_felts, uint32_t* query_positions, uint64_t* n_query_positions) {
+  if (run && !run->from_dch) return SH_ERR_INVALID;
+  return fri_decommit_impl(ctx, run, ch_out, h_extra, trees, n_trees, felts, cap, n_felts, query_positions,
+                           n_query_positions);
+}
+
+extern "C" int sh_fri_run_free(sh_ctx* ctx, sh_fri_run* run) {
+  if (!ctx) return SH_ERR_INVALID;
+  if (!run) return SH_OK;
+  run->free_all();
+  delete run;
+  return SH_OK;
 }
 
 // ------------------------------------------------------------------ DEEP quotient driver

@@ -18,8 +18,8 @@ import weakref
 import numpy as np
 
 from . import field as F
-from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, DeviceBuffer, DeviceChannel, FriParams,
-                      HashLayerSet, SecureColumnByCoords, _logs, _ptr_array, fri_prove, fri_prove_dch,
+from .backend import (BaseColumn, Channel, ColumnRefs, ColumnSet, DeviceBuffer, DeviceChannel, FriParams, Graph,
+                      HashLayerSet, SecureColumnByCoords, _logs, _ptr_array, fri_commit_dch, fri_prove,
                       fri_quotients_arrays, fri_quotients_dpoint, tree_decommit)
 
 P = F.P
@@ -28,8 +28,11 @@ COMPOSITION_LOG_SPLIT = 1  # crates/stwo/src/core/verifier.rs:17
 # Proofs keep their channel in device memory by default (DeviceChannel): from the trace root to
 # the FRI commit phase the host never waits for the device.  False: the host channel throughout.
 DEVICE_CHANNEL = True
-# With the device channel, a trace committed again from the same buffers replays a captured
-# hipGraph of the batched commit (CommitmentSchemeProver._commit_captured).
+# With the device channel, a trace proved again from the same buffers replays a captured hipGraph
+# of the proof's whole device work (trace commit, composition and its commit, OODS proof of work
+# and samples, DEEP quotients, FRI commit phase and proof of work: _ProofGraph); a trace committed
+# through start_wide_fibonacci replays a captured graph of the batched commit
+# (CommitmentSchemeProver._commit_captured).
 GRAPH_COMMITS = True
 
 
@@ -343,22 +346,14 @@ class CommitmentSchemeProver:
         quotients.free()
         return self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker)
 
-    def prove_values_dpoint(self, dch, channel, ws, oods_proof_of_work=None, check_samples=None,
-                            device_check=None):
-        """prove_values when every committed column is sampled once, at one point that the device
-        channel dch drew into ws (a ProofState): the samples are evaluated, mixed and combined
-        into the DEEP quotient on the device, and the FRI driver's commit-phase download brings
-        back the sampled values with the proof's other device-drawn words; the host channel
-        `channel` receives the final state.  check_samples as in prove_values (called with
-        (sampled values, the downloaded ProofState)); device_check, when given, is called once the
-        sampled values are on the device to enqueue a check that writes ws's CHECK words, and the
-        proof raises ConstraintsNotSatisfied when that check's flag is set.  Returns (felts,
-        proof dict)."""
+    def prove_values_dpoint_enqueue(self, dch, ws, device_check=None):
+        """The device half of prove_values_dpoint, enqueued without waiting: the samples, the
+        device check, their mix, the DEEP coefficient, the DEEP quotient and the FRI commit phase
+        (with its proof of work).  Returns the pending state for prove_values_dpoint_finish."""
         L = self.trees[-1].lifting_log_size
         b = self.fri["log_blowup_factor"]
         ctx = self.be.ctx
         trees = [t for t in self.trees if len(t.evals)]
-        n_per_tree = [len(t.evals) for t in self.trees]
         coeff_ptrs = np.concatenate([_ptr_array(t.coeffs) for t in trees])
         coeff_logs = np.concatenate([np.asarray(_logs(t.coeffs), np.uint32) for t in trees])
         eval_ptrs = np.concatenate([_ptr_array(t.evals) for t in trees])
@@ -379,17 +374,24 @@ class CommitmentSchemeProver:
         dch.draw_secure_felts(1, ws.ptr(ws.RC))
         quotients = fri_quotients_dpoint(self.be, self.tw, ColumnRefs(ctx, eval_ptrs, eval_logs), ws.ptr(ws.Z),
                                          ws.ptr(ws.VALS), ws.ptr(ws.RC), L, b)
-        fri_felts, queries, host, openings = fri_prove_dch(
-            self.be, self.tw, dch, channel, self.fri_params(), quotients, extra=(ws.ptr(0), ws.download_words),
-            trees=[(t.evals, t.layers, i == 0) for i, t in enumerate(self.trees)])
-        quotients.free()
+        fri_run = fri_commit_dch(self.be, self.tw, dch, self.fri_params(), quotients,
+                                 extra=(ws.ptr(0), ws.download_words))
+        return {"unknown": unknown, "quotients": quotients, "fri_run": fri_run}
+
+    def prove_values_dpoint_finish(self, channel, ws, pending, oods_proof_of_work=None, check_samples=None,
+                                   device_check=False):
+        """The host half of prove_values_dpoint: waits for the FRI commit phase's download, then
+        the decommitment (one gather for the FRI witnesses and the trees' openings) and the
+        felts.  May run again on the same pending state after a graph replays the device half."""
+        fri_felts, queries, host, openings = pending["fri_run"].decommit(
+            channel, trees=[(t.evals, t.layers, i == 0) for i, t in enumerate(self.trees)])
         ws.read(host)
-        for k, i in enumerate(unknown):
+        for k, i in enumerate(pending["unknown"]):
             self.trees[i].root = host[ws.ROOTS + 8 * k:ws.ROOTS + 8 * k + 8].tobytes()
         values = ws.values
         vl = values.tolist()
         sampled_values, blocks, c = [], [], 0
-        for n in n_per_tree:
+        for n in (len(t.evals) for t in self.trees):
             sampled_values.append([[vl[c + i]] for i in range(n)])
             blocks.append(([1] * n, values[c:c + n]))
             c += n
@@ -398,9 +400,32 @@ class CommitmentSchemeProver:
             oods_proof_of_work = ws.nonce
         out = self._decommit_and_serialize(fri_felts, queries, sampled_values, blocks, oods_proof_of_work, checker,
                                            openings)
-        if device_check is not None and ws.check_failed:
+        if device_check and ws.check_failed:
             raise ConstraintsNotSatisfied("Constraints not satisfied.")
         return out
+
+    @staticmethod
+    def free_pending(pending):
+        pending["fri_run"].free()
+        pending["quotients"].free()
+
+    def prove_values_dpoint(self, dch, channel, ws, oods_proof_of_work=None, check_samples=None,
+                            device_check=None):
+        """prove_values when every committed column is sampled once, at one point that the device
+        channel dch drew into ws (a ProofState): the samples are evaluated, mixed and combined
+        into the DEEP quotient on the device, and the FRI driver's commit-phase download brings
+        back the sampled values with the proof's other device-drawn words; the host channel
+        `channel` receives the final state.  check_samples as in prove_values (called with
+        (sampled values, the downloaded ProofState)); device_check, when given, is called once the
+        sampled values are on the device to enqueue a check that writes ws's CHECK words, and the
+        proof raises ConstraintsNotSatisfied when that check's flag is set.  Returns (felts,
+        proof dict)."""
+        pending = self.prove_values_dpoint_enqueue(dch, ws, device_check)
+        try:
+            return self.prove_values_dpoint_finish(channel, ws, pending, oods_proof_of_work, check_samples,
+                                                   device_check is not None)
+        finally:
+            self.free_pending(pending)
 
     def fri_params(self):
         b = self.fri["log_blowup_factor"]
@@ -617,17 +642,58 @@ def finish_wide_fibonacci(scheme, channel, n_cols, log_n):
     return felts, proof
 
 
-def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
-    """finish_wide_fibonacci with the channel in device memory: the composition coefficient and
-    its powers, the composition root, the OODS proof of work and point, the sampled values and the
-    DEEP coefficient are drawn, mixed and used on the device; the host reads them back with the
-    FRI commit phase (prove_values_dpoint).  Same proof, byte for byte."""
+class _ProofRun:
+    """A device-channel proof after its device work was enqueued (_finish_device_enqueue): the
+    scheme with its trees, the device channel, the ProofState words and the pending DEEP/FRI
+    state.  finish() waits for the device and completes the proof on the host."""
+
+    def __init__(self, scheme, dch, own_dch, ws, pending, n_cols, log_n):
+        self.scheme, self.dch, self.own_dch, self.ws, self.pending = scheme, dch, own_dch, ws, pending
+        self.n_cols, self.log_n = n_cols, log_n
+
+    def finish(self, channel):
+        scheme, ws = self.scheme, self.ws
+        felts, proof = scheme.prove_values_dpoint_finish(channel, ws, self.pending, device_check=True)
+        proof["composition_random_coeff"] = ws.alpha
+        proof["oods_point"] = ws.oods_point
+        return felts, proof
+
+    def free(self, trees=False):
+        """Releases the proof's device buffers (with trees: the committed trees' too, except the
+        shared empty preprocessed tree)."""
+        CommitmentSchemeProver.free_pending(self.pending)
+        self.ws.free()
+        if self.own_dch:
+            self.dch.free()
+        if trees:
+            empty = getattr(self.scheme.be, "_empty_tree", (None,))[0]
+            for t in self.scheme.trees:
+                if t.layers is empty:
+                    continue
+                for part in (t.coeffs, t.evals, t.layers):
+                    if hasattr(part, "free"):
+                        part.free()
+                    else:
+                        for x in part:
+                            x.free()
+
+
+def _finish_device_enqueue(scheme, channel, n_cols, log_n, own_dch=None):
+    """The device work of finish_wide_fibonacci with the channel in device memory, enqueued only:
+    the composition coefficient and its powers, the composition polynomial and its commitment,
+    the OODS proof of work and point, the samples and the closing check, the DEEP quotient and
+    the FRI commit phase.  Returns a _ProofRun."""
     backend, twiddles, fri = scheme.be, scheme.tw, scheme.fri
     ctx = backend.ctx
     b = fri["log_blowup_factor"]
     trace_tree = scheme.trees[1]
     # the trace committed elsewhere (batched) was mixed on the host: the device channel starts here
-    dch = scheme.dch if scheme.dch is not None and scheme.dch.be is backend else DeviceChannel(backend, channel)
+    if scheme.dch is not None and scheme.dch.be is backend:
+        dch, own = scheme.dch, False
+    else:
+        dch, own = DeviceChannel(backend, channel), True
+    if own_dch is not None:
+        own = own_dch
     m = n_cols - 2
     ws = ProofState(ctx, n_cols + 8, m)
     # compute_composition_polynomial (crates/stwo/src/prover/air/component_prover.rs:103-131)
@@ -641,22 +707,108 @@ def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
         ctx.check(ctx.lib.sh_memset(ctx.h, c.ptr, 0, c.nbytes), "memset")
     backend.wide_fibonacci_composition_dev(trace_tree.evals, log_n, eval_log, ws.ptr(ws.POW), acc, log_expansion)
     backend.interpolate_columns(acc.columns, twiddles)
-    halves = [backend.split_at_mid(c) for c in acc.columns]
+    halves = [backend.split_at_mid(c) for c in acc.columns]   # each acc column becomes a left half
     scheme.commit_polys([h[0] for h in halves] + [h[1] for h in halves], dch)
     # OODS proof of work and point (crates/stwo/src/prover/mod.rs:74-81)
     dch.grind(OODS_POW_BITS, ws.ptr(ws.CTL))
     dch.draw_random_point(ws.ptr(ws.Z))
-
     # the closing sanity check runs on the device, on the sampled values where they are
-    felts, proof = scheme.prove_values_dpoint(
-        dch, channel, ws, device_check=lambda: backend.wide_fibonacci_oods_check(
-            ws.ptr(ws.VALS), n_cols, ws.ptr(ws.POW), ws.ptr(ws.Z), log_n, ws.ptr(ws.CHECK)))
-    proof["composition_random_coeff"] = ws.alpha
-    proof["oods_point"] = ws.oods_point
-    ws.free()
-    if dch is not scheme.dch:
-        dch.free()
-    return felts, proof
+    pending = scheme.prove_values_dpoint_enqueue(dch, ws, device_check=lambda: backend.wide_fibonacci_oods_check(
+        ws.ptr(ws.VALS), n_cols, ws.ptr(ws.POW), ws.ptr(ws.Z), log_n, ws.ptr(ws.CHECK)))
+    return _ProofRun(scheme, dch, own, ws, pending, n_cols, log_n)
+
+
+def _finish_wide_fibonacci_device(scheme, channel, n_cols, log_n):
+    """finish_wide_fibonacci with the channel in device memory: the composition coefficient and
+    its powers, the composition root, the OODS proof of work and point, the sampled values and the
+    DEEP coefficient are drawn, mixed and used on the device; the host reads them back with the
+    FRI commit phase.  Same proof, byte for byte."""
+    run = _finish_device_enqueue(scheme, channel, n_cols, log_n)
+    try:
+        return run.finish(channel)
+    finally:
+        run.free()
+
+
+class _ProofGraph:
+    """A whole device-channel proof of one trace (fixed buffers and shapes) as one hipGraph: the
+    batched trace commit, the composition and its commit, the OODS proof of work and samples, the
+    DEEP quotient and the FRI commit phase with its proof of work.  The buffers the graph writes
+    (trees, quotient, FRI layers, ProofState words) are the graph's own; the trace's evaluation
+    buffers are read wherever they are, with their contents at launch time.  A launch is preceded
+    by loading the host channel's state into the graph's device channel and followed by the host
+    half of the proof (_ProofRun.finish)."""
+
+    def __init__(self, backend, twiddles, trace, log_n, fri, key):
+        self.be, self.tw, self.trace, self.log_n, self.fri, self.key = backend, twiddles, trace, log_n, fri, key
+        self.graph = Graph(backend.ctx)
+        self.run = None
+
+    def _enqueue(self, channel, load):
+        b = self.fri["log_blowup_factor"]
+        scheme = CommitmentSchemeProver(self.be, self.tw, self.fri, self.log_n + b, 0)
+        scheme.device_channel, scheme.graph_commit = True, False
+        scheme.commit_evals([], channel)                   # the cached empty preprocessed tree (host mix)
+        scheme.dch = DeviceChannel(self.be, channel, load=load)
+        scheme.commit_evals(self.trace, scheme.dch)
+        return _finish_device_enqueue(scheme, channel, len(self.trace), self.log_n, own_dch=True)
+
+    def build(self, channel):
+        """Proves once eagerly on the graph's resources (the warm-up; its proof is returned), then
+        captures the same device work."""
+        if getattr(self.be, "_empty_tree", None) is None:   # shared by every proof: not the graph's
+            self.be._empty_tree = self.be.commit_lifted([], 0)
+        with self.graph.warmup():
+            run = self._enqueue(channel, load=True)
+        try:
+            out = run.finish(channel)
+        finally:
+            run.free(trees=True)   # the warm-up's blocks, free again for the capture
+        with self.graph.capture():
+            self.run = self._enqueue(Channel(self.be.lib), load=False)
+        return out
+
+    def prove(self, channel):
+        """One proof by a graph launch: the host mixes the empty preprocessed root, loads the
+        channel into the device, launches, and finishes the proof."""
+        layers, root = self.be._empty_tree
+        _mix_root(channel, layers, root)
+        self.run.dch.load(channel)
+        self.graph.launch()
+        return self.run.finish(channel)
+
+    def free(self):
+        if self.run is not None:
+            self.run.free(trees=True)
+            self.run = None
+        self.graph.free()
+
+
+def _prove_with_graph(backend, twiddles, trace, log_n, fri, channel):
+    """prove_wide_fibonacci through a captured _ProofGraph when this trace's buffers were proved
+    before (the first proof of a set of buffers returns None: the caller proves eagerly; the
+    second builds the graph, one per backend; later ones replay it)."""
+    key = (tuple(int(p) for p in _ptr_array(trace)), tuple(int(x) for x in _logs(trace)), log_n,
+           tuple(sorted(fri.items())), id(twiddles))
+    cached = getattr(backend, "_proof_graph", None)
+    if cached is not None and cached.key == key:
+        return cached.prove(channel)
+    seen = backend.__dict__.setdefault("_proof_seen", [])
+    if key not in seen:
+        seen.append(key)
+        del seen[:-4]
+        return None
+    if cached is not None:
+        backend._proof_graph = None
+        cached.free()
+    pg = _ProofGraph(backend, twiddles, trace, log_n, fri, key)
+    try:
+        out = pg.build(channel)
+    except BaseException:
+        pg.free()
+        raise
+    backend._proof_graph = pg
+    return out
 
 
 def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, device_channel=None,
@@ -665,11 +817,20 @@ def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, dev
     183-220) through prove_ex: empty preprocessed tree, the trace tree, then
     finish_wide_fibonacci.  trace: device columns (BaseColumn, bit-reversed evaluations on
     CanonicCoset(log_n)).  device_channel: keep the channel in device memory (default
-    DEVICE_CHANNEL); graph_commit: replay a captured commit of a trace proved again from the same
-    buffers (default GRAPH_COMMITS).  Returns (felts, proof dict, the channel in its final state)."""
+    DEVICE_CHANNEL); graph_commit: with the device channel and a proof of work the device search
+    covers (pow_bits <= 32), a trace proved again from the same buffers is proved by replaying a
+    captured graph of the proof's whole device work (_ProofGraph; default GRAPH_COMMITS).
+    Returns (felts, proof dict, the channel in its final state)."""
     assert fri["log_blowup_factor"] >= 1, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
+    dc = DEVICE_CHANNEL if device_channel is None else bool(device_channel)
+    gc = GRAPH_COMMITS if graph_commit is None else bool(graph_commit)
+    if dc and gc and fri["pow_bits"] <= 32:
+        channel = channel or Channel(backend.lib)
+        out = _prove_with_graph(backend, twiddles, trace, log_n, fri, channel)
+        if out is not None:
+            return out[0], out[1], channel
     scheme, channel = start_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel,
-                                           device_channel=device_channel, graph_commit=graph_commit)
+                                           device_channel=dc, graph_commit=False)
     felts, proof = finish_wide_fibonacci(scheme, channel, len(trace), log_n)
     return felts, proof, channel
 

@@ -73,3 +73,52 @@ def test_mixed_graph_replays_match_eager(ctx, backend):
     exp_ldes = [O.evaluate(tw, O.interpolate(itw, c, lg), lg + b) for c, lg in zip(host2, logs)]
     assert g.root() == O.merkle_commit_mixed(exp_ldes)[0][0].tobytes()
     g.free()
+
+
+def test_captured_entry_point_sequence_replays(ctx, backend):
+    """sh_graph_warmup/capture around a sequence of entry points that allocate (interpolate into
+    pooled copies, evaluate into a new set): the capture hands out the warm-up's blocks in the
+    same order, and every replay, with eager work and new inputs between replays, equals the
+    oracle."""
+    from proving_amd.backend import Graph
+    logs = [9] * 5 + [12] * 3
+    L = max(logs) + 1
+    tree = backend.precompute_canonic_twiddles(L)
+    tw, itw = O.canonic_twiddles(L)
+    evals = [BaseColumn.from_cpu(ctx, c) for c in splitmix64_columns(31, logs)]
+
+    def work():
+        coeffs = []
+        for e in evals:
+            c = BaseColumn(ctx, e.log_size)
+            c.copy_from(e)
+            coeffs.append(c)
+        backend.interpolate_columns(coeffs, tree)
+        tmp = BaseColumn(ctx, 11)   # freed inside the sequence: its block is reused below
+        tmp.free()
+        return coeffs, backend.evaluate_polynomials(coeffs, 1, tree)
+
+    g = Graph(ctx)
+    with g.warmup():
+        coeffs, ldes = work()
+    for c in coeffs:
+        c.free()
+    ldes.free()
+    with g.capture():
+        coeffs, ldes = work()
+    other = [BaseColumn.from_cpu(ctx, c) for c in splitmix64_columns(32, [12] * 4)]
+    for rep, seed in enumerate((33, 33, 34)):
+        host = splitmix64_columns(seed, logs)
+        for e, h in zip(evals, host):
+            e.upload(h)
+        g.launch()
+        for k in (0, 4, 5, 7):
+            ec = O.interpolate(itw, host[k], logs[k])
+            np.testing.assert_array_equal(coeffs[k].to_cpu(), ec, err_msg=f"replay {rep} column {k}")
+            np.testing.assert_array_equal(ldes[k].to_cpu(), O.evaluate(tw, ec, logs[k] + 1))
+        backend.interpolate_columns(other, tree)
+        ctx.sync()
+    for c in coeffs:
+        c.free()
+    ldes.free()
+    g.free()

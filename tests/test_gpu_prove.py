@@ -83,29 +83,68 @@ def test_wide_fib_full_size_verifies(ctx, backend):
     assert PR.wide_fib_verify(felts.tolist(), w["n_cols"], w["log_size"], w["fri"])
 
 
-def test_repeated_proofs_replay_the_captured_commit(ctx, backend):
-    """Proving a trace again from the same buffers: the first proof commits eagerly, the second
-    captures the commit as a graph, later ones replay it, and a proof started while another's
-    tree from that graph is alive commits eagerly; every proof is byte-equal to the oracle's,
-    also after the trace buffers are overwritten with another trace."""
+def test_repeated_proofs_replay_the_proof_graph(ctx, backend):
+    """Proving a trace again from the same buffers: the first proof runs eagerly, the second is
+    the warm-up of a captured graph of the proof's whole device work (then captured), later ones
+    replay it; every proof is byte-equal to the oracle's and leaves the channel where the eager
+    proof does, also after the trace buffers are overwritten with another trace and with other
+    proofs (other buffers, the host channel) between the replays."""
     n_cols, log_n, fri = 12, 9, params(6, 1, 9)
     tw, itw = O.canonic_twiddles(log_n + 1)
     tree = backend.precompute_canonic_twiddles(log_n + 1)
     traces = [wide_fib_trace(n_cols, log_n, s) for s in (31, 32)]
     exp = [PR.wide_fib_prove(t, log_n, fri, tw, itw)[0] for t in traces]
     cols = [BaseColumn.from_cpu(ctx, c) for c in traces[0]]
+    digests = set()
     for k in range(4):
-        felts, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+        felts, _, ch = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
         assert felts.tolist() == exp[0], f"proof {k}"
-    assert backend._commit_graph is not None
-    # a proof in progress holds the graph's tree: another proof of the same buffers commits eagerly
-    scheme, ch = start_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
-    felts2, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
-    felts1, _ = finish_wide_fibonacci(scheme, ch, n_cols, log_n)
-    assert felts1.tolist() == exp[0] and felts2.tolist() == exp[0]
-    del scheme
+        digests.add((ch.digest, ch.s.n_draws))
+    assert len(digests) == 1
+    assert backend._proof_graph is not None
+    other = [BaseColumn.from_cpu(ctx, c) for c in traces[1]]
+    felts, _, _ = prove_wide_fibonacci(backend, tree, other, log_n, fri, graph_commit=True)
+    assert felts.tolist() == exp[1]
+    felts, _, _ = prove_wide_fibonacci(backend, tree, other, log_n, fri, device_channel=False)
+    assert felts.tolist() == exp[1]
+    felts, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    assert felts.tolist() == exp[0]
     # new contents in the same buffers: the replayed graph reads them
     for c, h in zip(cols, traces[1]):
         c.upload(h)
     felts, _, _ = prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
     assert felts.tolist() == exp[1]
+    # a trace that breaks a constraint, proved by the replay: the device check still fires
+    bad = [np.array(h) for h in traces[1]]
+    bad[5][17] = (int(bad[5][17]) + 1) % P
+    for c, h in zip(cols, bad):
+        c.upload(h)
+    with pytest.raises(ConstraintsNotSatisfied):
+        prove_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    backend._proof_graph.free()
+    backend._proof_graph = None
+
+
+def test_start_wide_fibonacci_replays_the_captured_commit(ctx, backend):
+    """The trace commit of start_wide_fibonacci proved again from the same buffers: the first
+    commits eagerly, the second captures the batched commit as a graph, later ones replay it, and
+    a proof started while another's tree from that graph is alive commits eagerly; every proof is
+    byte-equal to the oracle's."""
+    n_cols, log_n, fri = 12, 9, params(6, 1, 9)
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    trace = wide_fib_trace(n_cols, log_n, 41)
+    exp = PR.wide_fib_prove(trace, log_n, fri, tw, itw)[0]
+    cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
+    for k in range(4):
+        scheme, ch = start_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+        felts, _ = finish_wide_fibonacci(scheme, ch, n_cols, log_n)
+        assert felts.tolist() == exp, f"proof {k}"
+        del scheme
+    assert backend._commit_graph is not None
+    # a proof in progress holds the graph's tree: another proof of the same buffers commits eagerly
+    scheme1, ch1 = start_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    scheme2, ch2 = start_wide_fibonacci(backend, tree, cols, log_n, fri, graph_commit=True)
+    felts2, _ = finish_wide_fibonacci(scheme2, ch2, n_cols, log_n)
+    felts1, _ = finish_wide_fibonacci(scheme1, ch1, n_cols, log_n)
+    assert felts1.tolist() == exp and felts2.tolist() == exp
