@@ -304,6 +304,7 @@ def bench_prove(args):
         out["oracle_verifies"] = bool(PR.wide_fib_verify(first.tolist(), N, n, w["fri"]))
         out["oracle_verify_s"] = round(time.perf_counter() - t0, 2)
     print(json.dumps(out), flush=True)
+    be.release_graphs()
     ctx.close()
 
 

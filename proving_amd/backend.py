@@ -12,6 +12,7 @@ import contextlib
 import ctypes
 import json
 import os
+import sys
 
 import numpy as np
 
@@ -419,9 +420,9 @@ class Graph:
         self.ctx.check(self.ctx.lib.sh_graph_launch(self.ctx.h, self.h), "graph_launch")
 
     def free(self):
-        if self.h:
+        if self.h and not self.ctx._closed:
             self.ctx.check(self.ctx.lib.sh_graph_destroy(self.ctx.h, self.h), "graph_destroy")
-            self.h = None
+        self.h = None
 
 
 def canonic_half_initial(log_size):
@@ -439,6 +440,16 @@ class HipBackend:
 
     def _chk(self, rc, what):
         self.ctx.check(rc, what)
+
+    def release_graphs(self):
+        """Frees the graphs the prover keeps on this backend (a proof graph, a commit graph) and
+        the buffers they own; call before closing the context."""
+        pg = self.__dict__.pop("_proof_graph", None)
+        if pg is not None:
+            pg.free()
+        cg = self.__dict__.pop("_commit_graph", None)
+        if cg is not None:
+            cg["graph"].free()
 
     # ---- PolyOps (crates/stwo/src/prover/poly/circle/ops.rs:18-165) -------------------------
     def precompute_twiddles(self, root_initial_index, root_log_size):
@@ -1017,11 +1028,14 @@ class FriRun:
         return buf[:nf.value].copy(), qp[:nq.value].tolist(), h_extra[:self.n_extra], _openings(ops, keep)
 
     def free(self):
-        if self.h:
+        if self.h and not self.be.ctx._closed:
             self.be._chk(self.be.lib.sh_fri_run_free(self.be.h, self.h), "fri_run_free")
-            self.h = None
+        self.h = None
 
     def __del__(self):
+        # at interpreter exit the context and the library may already be gone: nothing is freed
+        if sys.is_finalizing():
+            return
         try:
             self.free()
         except Exception:

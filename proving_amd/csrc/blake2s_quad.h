@@ -81,35 +81,6 @@ __device__ __forceinline__ void quad_node_hash(const uint32_t* msg, const uint32
   lo = h_lo ^ a ^ c;
   hi = h_hi ^ b ^ d;
 }
-// The same hash with the 40 message indices packed eight to a word (4 bits each, sp from
-// quad_sigma_packed), and each round's four message words read from LDS in that round: 5
-// registers of indices instead of 40, for kernels whose workgroups of 1024 lanes leave 128
-// VGPRs per lane.
-__device__ __forceinline__ void quad_node_hash(const uint32_t* msg, const uint32_t (&sp)[5], uint32_t q,
-                                               uint32_t h_lo, uint32_t h_hi, uint32_t d0, uint32_t& lo,
-                                               uint32_t& hi) {
-  uint32_t a = h_lo, b = h_hi, c = q == 0 ? SH_B2_IV0 : q == 1 ? SH_B2_IV1 : q == 2 ? SH_B2_IV2 : SH_B2_IV3, d = d0;
-#pragma unroll
-  for (int r = 0; r < 10; r++) {
-    // round r's four indices are bits 16 (r & 1) .. 16 (r & 1) + 15 of word r / 2; the empty asm
-    // keeps the compiler from unpacking every round's indices ahead of the rounds
-    uint32_t w = sp[r >> 1];
-    asm volatile("" : "+v"(w));
-    uint32_t mw[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) mw[j] = msg[(w >> (16 * (r & 1) + 4 * j)) & 15u];
-    SH_QG(mw[0], mw[1]);
-    b = quad_rot<1>(b);
-    c = quad_rot<2>(c);
-    d = quad_rot<3>(d);
-    SH_QG(mw[2], mw[3]);
-    b = quad_rot<3>(b);
-    c = quad_rot<2>(c);
-    d = quad_rot<1>(d);
-  }
-  lo = h_lo ^ a ^ c;
-  hi = h_hi ^ b ^ d;
-}
 #undef SH_QG
 
 // The initial chaining value of a one-block hash of t bytes (final block), as a quad holds it:
@@ -124,17 +95,6 @@ __device__ __forceinline__ void quad_init(uint32_t q, uint32_t t, uint32_t& h_lo
 __device__ __forceinline__ void quad_sigma(uint32_t q, uint32_t (&sidx)[40]) {
 #pragma unroll
   for (int i = 0; i < 40; i++) sidx[i] = kQuadSigma.v[q][i];
-}
-
-// This lane's 40 message indices, eight 4-bit indices per word.
-__device__ __forceinline__ void quad_sigma_packed(uint32_t q, uint32_t (&sp)[5]) {
-#pragma unroll
-  for (int w = 0; w < 5; w++) {
-    uint32_t x = 0;
-#pragma unroll
-    for (int j = 0; j < 8; j++) x |= (uint32_t)kQuadSigma.v[q][8 * w + j] << (4 * j);
-    sp[w] = x;
-  }
 }
 
 }  // namespace

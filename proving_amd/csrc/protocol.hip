@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <mutex>
 #include <set>
 #include <tuple>
 #include <vector>
@@ -28,6 +29,7 @@ int sh_fold_line_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const s
                      uint32_t* leaves = nullptr);
 int sh_fold_circle_dev(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const src[4], uint32_t log_size,
                        const uint32_t* d_alpha, uint32_t* const dst[4], uint32_t* leaves = nullptr);
+int sh_dch_mix_impl(sh_ctx* ctx, uint32_t* d_ch, const uint32_t* d_words, uint64_t n);
 
 namespace {
 
@@ -386,8 +388,8 @@ __global__ __launch_bounds__(kFriTailThreads) void k_fri_tail(FriTailArgs a) {
   __shared__ uint32_t alpha_w[4];
   const uint32_t tid = threadIdx.x, q = tid & 3, quad = tid >> 2;
   constexpr uint32_t kQuads = kFriTailThreads / 4;
-  uint32_t sidx[5];
-  quad_sigma_packed(q, sidx);  // packed: the 1024-lane workgroup leaves 128 VGPRs per lane
+  uint32_t sidx[40];
+  quad_sigma(q, sidx);
   uint32_t h_lo, h_hi, d_leaf, d_node, d_draw;
   quad_init(q, 16, h_lo, h_hi, d_leaf);  // a leaf: the 4 coordinates, 16 bytes
   quad_init(q, 64, h_lo, h_hi, d_node);  // a node, or digest || root
@@ -678,6 +680,104 @@ extern "C" int sh_tree_decommit(sh_ctx* ctx, const uint32_t* const* d_cols, cons
 }
 
 // ------------------------------------------------------------------ FRI prover
+namespace {
+// LineEvaluation::interpolate with commit_last_layer's degree check and coefficient order
+// (crates/stwo/src/prover/line.rs:54-65, crates/stwo/src/prover/fri.rs:208-226) in one workgroup:
+// e[c] holds coordinate c of the n = 2^log values in bit-reversed order; itw holds the inverse
+// x-coordinates of line_ifft's layers (crates/stwo/src/core/poly/line.rs:202-219), layer j
+// (2^(log-1-j) values) after layer j - 1.  The ordered coefficient k is the scaled transform's
+// value at bitrev(k); out[4m..4m+4] = coefficient bitrev(m, bound_log) (LinePoly keeps its
+// coefficients bit-reversed), and flag[0] = 1 when a coefficient at or above 2^bound_log is
+// nonzero (the reference's "invalid degree").
+constexpr uint32_t kLastLayerMaxLog = 10;
+__global__ __launch_bounds__(256) void k_fri_last_layer(const uint32_t* e0, const uint32_t* e1, const uint32_t* e2,
+                                                        const uint32_t* e3, const uint32_t* __restrict__ itw,
+                                                        uint32_t log, uint32_t bound_log, uint32_t inv_n,
+                                                        uint32_t* __restrict__ out, uint32_t* __restrict__ flag) {
+  __shared__ uint32_t s[4][1u << kLastLayerMaxLog];
+  __shared__ uint32_t bad;
+  const uint32_t n = 1u << log, tid = threadIdx.x;
+  if (tid == 0) bad = 0;
+  for (uint32_t i = tid; i < n; i += blockDim.x) {
+    const uint32_t j = sh::bitrev(i, log);  // natural order
+    s[0][j] = e0[i];
+    s[1][j] = e1[i];
+    s[2][j] = e2[i];
+    s[3][j] = e3[i];
+  }
+  __syncthreads();
+  uint32_t off = 0;
+  for (uint32_t l = log; l > 0; l--) {
+    const uint32_t half = 1u << (l - 1);
+    for (uint32_t t = tid; t < n / 2; t += blockDim.x) {
+      const uint32_t c = t >> (l - 1), i = t & (half - 1);
+      const uint32_t ia = (c << l) + i, ib = ia + half;
+      const sh::qm31 a = {{s[0][ia], s[1][ia]}, {s[2][ia], s[3][ia]}};
+      const sh::qm31 b = {{s[0][ib], s[1][ib]}, {s[2][ib], s[3][ib]}};
+      const sh::qm31 x = sh::qadd(a, b), y = sh::qmulm(sh::qsub(a, b), itw[off + i]);
+      s[0][ia] = x.a.a;
+      s[1][ia] = x.a.b;
+      s[2][ia] = x.b.a;
+      s[3][ia] = x.b.b;
+      s[0][ib] = y.a.a;
+      s[1][ib] = y.a.b;
+      s[2][ib] = y.b.a;
+      s[3][ib] = y.b.b;
+    }
+    off += half;
+    __syncthreads();
+  }
+  for (uint32_t i = tid; i < n; i += blockDim.x) {
+    const uint32_t k = sh::bitrev(i, log);
+    const sh::qm31 v = sh::qmulm({{s[0][i], s[1][i]}, {s[2][i], s[3][i]}}, inv_n);
+    if (k >= (1u << bound_log)) {
+      if (!sh::qeq(v, sh::qzero())) bad = 1;
+    } else {
+      sh::qstore(v, out + 4 * sh::bitrev(k, bound_log));
+    }
+  }
+  __syncthreads();
+  if (tid == 0) flag[0] = bad;
+}
+
+// Inverse twiddles of line_ifft over the line domain (init, log), layer after layer, as
+// k_fri_last_layer reads them; computed once per domain.
+const std::vector<uint32_t>& last_layer_itw(uint32_t init, uint32_t log) {
+  static std::mutex mu;
+  static std::map<std::pair<uint32_t, uint32_t>, std::vector<uint32_t>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto& t = cache[{init, log}];
+  if (t.empty()) {
+    uint32_t step = sh::subgroup_gen(log);
+    for (uint32_t l = log; l > 0; l--) {
+      for (uint64_t i = 0; i < (1ull << (l - 1)); i++)
+        t.push_back(sh::minv(sh::index_to_point_host(sh::idx_reduce((uint64_t)init + (uint64_t)step * i)).x));
+      init = sh::idx_reduce((uint64_t)init * 2);
+      step = sh::idx_reduce((uint64_t)step * 2);
+    }
+    if (t.empty()) t.push_back(0);  // log 0: no layers (one word keeps the staged table non-empty)
+  }
+  return t;
+}
+
+// verify_pow_nonce (crates/stwo/src/core/proof_of_work.rs): Blake2s(prefixed digest || LE64(nonce))
+// has pow_bits trailing zero bits, prefixed = Blake2s(LE32(0x12345678) || digest || LE32(pow_bits)).
+bool pow_ok(const uint8_t digest[32], uint32_t pow_bits, uint64_t nonce) {
+  uint8_t buf[40], pre[32], h[32];
+  const uint32_t prefix = 0x12345678u;
+  memcpy(buf, &prefix, 4);
+  memcpy(buf + 4, digest, 32);
+  memcpy(buf + 36, &pow_bits, 4);
+  sh::b2s_hash(buf, 40, pre);
+  memcpy(buf, pre, 32);
+  memcpy(buf + 32, &nonce, 8);
+  sh::b2s_hash(buf, 40, h);
+  for (uint32_t b = 0; b < pow_bits; b++)
+    if ((h[b >> 3] >> (b & 7)) & 1) return false;
+  return true;
+}
+}  // namespace
+
 // The state of one FRI proof between its commit phase (enqueued only) and its decommitment: every
 // layer's evaluation and tree, and the device words of the commit phase's one download.
 struct sh_fri_run {
@@ -686,12 +786,14 @@ struct sh_fri_run {
   uint32_t log_size = 0, fs = 0, last_log = 0, max_layers = 0, n_layers = 0;
   uint32_t last_line_init = 0;  // the last layer's line domain
   bool from_dch = false;        // the proof's channel is a device channel (copied to CH)
+  bool device_tail = false;     // the last layer's coefficients, their mix and the proof of work ran on the device
   sh_channel ch_in{};           // the host channel the commit started from (host-channel proofs)
   uint64_t n_extra = 0;
   uint32_t* d_aux = nullptr;
   // word offsets in d_aux: the channel, the alphas (16 words per layer), the roots (8 per layer),
-  // the initial channel; then the last layer's evaluation and the extra words (the download)
-  uint64_t ALPHAS = 0, ROOTS = 0, INIT = 0, EVAL = 0, EXTRA = 0, dl_words = 0;
+  // the initial channel, the device's last-layer coefficients, the degree flag, the nonce
+  // control words; then the last layer's evaluation and the extra words (the download)
+  uint64_t ALPHAS = 0, ROOTS = 0, INIT = 0, LASTC = 0, FLAG = 0, CTL = 0, EVAL = 0, EXTRA = 0, dl_words = 0;
   FriLayer first, cur;
   std::vector<FriLayer> inner;
   void free_all() {
@@ -706,8 +808,11 @@ struct sh_fri_run {
 
 // FriProver::commit (crates/stwo/src/prover/fri.rs:113-226), enqueued without waiting for the
 // device: each layer's root is mixed into a device copy of the channel and the folding alphas are
-// drawn there (k_fri_channel), so the folds read them from device memory.  The host replays the
-// same channel operations afterwards, from the downloaded roots, and checks that the alphas agree.
+// drawn there (k_fri_channel), so the folds read them from device memory.  When the proof of work
+// fits the device search (pow_bits <= 32) and the last layer has at most 2^10 values, the last
+// layer's coefficients are computed and mixed and the proof of work is ground on the device too
+// (FriProver::decommit's first steps, fri.rs:229-244).  The host replays the same channel
+// operations afterwards, from the downloaded roots, and checks that the device agrees.
 // d_ch_in: when set, the proof's channel is this device channel (16 words, as dchannel.hip keeps
 // it); else ch is staged.  d_extra: n_extra words of the caller's device memory that the commit
 // phase's download also brings back.
@@ -733,11 +838,15 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
   run->from_dch = d_ch_in != nullptr;
   if (ch) run->ch_in = *ch;
   run->n_extra = n_extra;
+  run->device_tail = p->pow_bits <= 32 && last_log <= kLastLayerMaxLog;
   const uint32_t max_layers = run->max_layers = log_size + 1;
   run->ALPHAS = 16;
   run->ROOTS = run->ALPHAS + 16ull * max_layers;
   run->INIT = run->ROOTS + 8ull * max_layers;
-  run->EVAL = run->INIT + 16;
+  run->LASTC = run->INIT + 16;
+  run->FLAG = run->LASTC + (4ull << last_log);
+  run->CTL = run->FLAG + 4;
+  run->EVAL = run->CTL + 4;
   run->EXTRA = run->EVAL + (4ull << last_log);
   run->dl_words = run->EXTRA + n_extra;
   int rc = SH_OK;
@@ -900,8 +1009,21 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
   }
   run->n_layers = n_layers;
   run->last_line_init = line_init;
-  // one download: the device channel, the alphas and roots of every layer, the last layer, the
-  // extra words (packed here; the decommitment copies them out)
+  if (run->device_tail) {
+    // commit_last_layer (fri.rs:208-226) and the proof of work (fri.rs:229-244) on the device
+    const auto& itw = last_layer_itw(line_init, last_log);
+    void* d_itw;
+    TRY_RUN(sh_stage(ctx, itw.data(), 4 * itw.size(), &d_itw));
+    k_fri_last_layer<<<1, 256, 0, ctx->stream>>>(cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3],
+                                                 (const uint32_t*)d_itw, last_log, p->log_last_layer_degree_bound,
+                                                 sh::minv(1u << last_log), run->d_aux + run->LASTC,
+                                                 run->d_aux + run->FLAG);
+    TRY_RUN(sh_launch_check(ctx, "fri_last_layer"));
+    TRY_RUN(sh_dch_mix_impl(ctx, d_ch, run->d_aux + run->LASTC, 4ull << p->log_last_layer_degree_bound));
+    TRY_RUN(sh_grind_dchannel(ctx, d_ch, p->pow_bits, run->d_aux + run->CTL));
+  }
+  // one download: the device channel, the alphas and roots of every layer, the device tail's
+  // words, the last layer, the extra words (packed here; the decommitment copies them out)
   {
     PackArgs pa = {};
     for (int c = 0; c < 4; c++) {
@@ -928,9 +1050,10 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
 
 // FriProver::decommit (crates/stwo/src/prover/fri.rs:229-284) after fri_commit_impl: waits for the
 // commit phase's download, replays the channel on the host (ch: the final state), interpolates
-// the last layer, grinds the proof of work, draws the queries and reads every layer's witness
-// values and Merkle witness nodes, and the committed trees' openings, with one gather.  Leaves the
-// run's buffers in place (a replayed graph writes them again).
+// the last layer and grinds the proof of work (or checks the device's), draws the queries and
+// reads every layer's witness values and Merkle witness nodes, and the committed trees'
+// openings, with one gather.  Leaves the run's buffers in place (a replayed graph writes them
+// again).
 static int fri_decommit_impl(sh_ctx* ctx, sh_fri_run* run, sh_channel* ch, uint32_t* h_extra, sh_tree_opening* trees,
                              uint32_t n_trees, uint64_t* felts, uint64_t cap, uint64_t* n_felts,
                              uint32_t* query_positions, uint64_t* n_query_positions) {
@@ -985,12 +1108,23 @@ static int fri_decommit_impl(sh_ctx* ctx, sh_fri_run* run, sh_channel* ch, uint3
     last_coeffs = v;
     std::vector<uint32_t> fw(4 * v.size());
     for (size_t i = 0; i < v.size(); i++) sh::qstore(v[i], &fw[4 * i]);
+    if (run->device_tail &&
+        (aux[run->FLAG] != 0 || memcmp(fw.data(), aux + run->LASTC, 4 * fw.size()) != 0))
+      return sh_fail(ctx, SH_ERR_INVALID, "device-side last FRI layer diverged from the host's");
     sh_channel_mix_felts(ch, fw.data(), v.size());
   }
   // decommit (fri.rs:229-251): the proof of work
   uint64_t nonce = 0;
-  SH_TRY(sh_grind(ctx, ch->digest, p->pow_bits, &nonce));
+  if (run->device_tail) {
+    nonce = (uint64_t)aux[run->CTL] | ((uint64_t)aux[run->CTL + 1] << 32);
+    if (!pow_ok(ch->digest, p->pow_bits, nonce))
+      return sh_fail(ctx, SH_ERR_NOT_FOUND, "device proof of work returned no valid nonce");
+  } else {
+    SH_TRY(sh_grind(ctx, ch->digest, p->pow_bits, &nonce));
+  }
   sh_channel_mix_u64(ch, nonce);
+  if (run->device_tail && memcmp(ch->digest, aux, 32) != 0)
+    return sh_fail(ctx, SH_ERR_INVALID, "device-side FRI channel diverged from the host channel after the nonce");
   std::vector<uint32_t> q(p->n_queries);
   sh_channel_draw_queries(ch, log_size, p->n_queries, q.data());
   std::sort(q.begin(), q.end());

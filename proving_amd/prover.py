@@ -775,6 +775,10 @@ class _ProofGraph:
         _mix_root(channel, layers, root)
         self.run.dch.load(channel)
         self.graph.launch()
+        # the host waits for the replay before enqueueing the decommitment's download: a
+        # device-to-host copy enqueued behind a replay still in flight stalls the replay partway
+        # (measured: ~1.2 ms of idle device, 10.2 instead of 8.8 ms per proof)
+        self.be.ctx.sync()
         return self.run.finish(channel)
 
     def free(self):
