@@ -768,18 +768,27 @@ class _ProofGraph:
             self.run = self._enqueue(Channel(self.be.lib), load=False)
         return out
 
-    def prove(self, channel):
-        """One proof by a graph launch: the host mixes the empty preprocessed root, loads the
-        channel into the device, launches, and finishes the proof."""
+    def start(self, channel):
+        """The device half of one proof: the host mixes the empty preprocessed root, loads the
+        channel into the device and launches the graph (no wait)."""
         layers, root = self.be._empty_tree
         _mix_root(channel, layers, root)
         self.run.dch.load(channel)
         self.graph.launch()
+
+    def finish(self, channel):
+        """The host half of the proof that start() launched: waits for the replay, then the
+        decommitment and the felts.  Returns (felts, proof dict)."""
         # the host waits for the replay before enqueueing the decommitment's download: a
         # device-to-host copy enqueued behind a replay still in flight stalls the replay partway
         # (measured: ~1.2 ms of idle device, 10.2 instead of 8.8 ms per proof)
         self.be.ctx.sync()
         return self.run.finish(channel)
+
+    def prove(self, channel):
+        """One proof by a graph launch (start, then finish)."""
+        self.start(channel)
+        return self.finish(channel)
 
     def free(self):
         if self.run is not None:
@@ -837,6 +846,52 @@ def prove_wide_fibonacci(backend, twiddles, trace, log_n, fri, channel=None, dev
                                            device_channel=dc, graph_commit=False)
     felts, proof = finish_wide_fibonacci(scheme, channel, len(trace), log_n)
     return felts, proof, channel
+
+
+class ProofPipeline:
+    """Many proofs of one wide-Fibonacci trace (fixed buffers), several in flight: proof i runs on
+    backends[i % len(backends)], each backend (a context of its own, with its own stream) replaying
+    its own captured _ProofGraph, and the host finishes proof i (decommitment, felts) while the
+    device work of the proofs after it runs.  The contexts share the device and their streams run
+    concurrently, so the narrow stretches of one proof (Merkle tops, FRI folds, channel hashing,
+    the proofs of work's last waves) overlap the wide passes of the next.  Each proof is byte for
+    byte the one prove_wide_fibonacci makes with the same channel (the channel objects receive
+    their final states).  Building runs one warm-up proof per backend."""
+
+    def __init__(self, backends, trace, log_n, fri):
+        assert fri["pow_bits"] <= 32, "the proof graph covers device proofs of work of up to 32 bits"
+        assert fri["log_blowup_factor"] >= 1, "the wide-Fibonacci constraints need log_blowup_factor >= 1"
+        self.graphs = []
+        try:
+            for be in backends:
+                if getattr(be, "_empty_tree", None) is None:
+                    be._empty_tree = be.commit_lifted([], 0)
+                tw = be.precompute_canonic_twiddles(log_n + fri["log_blowup_factor"])
+                pg = _ProofGraph(be, tw, trace, log_n, fri, None)
+                self.graphs.append(pg)
+                pg.build(Channel(be.lib))
+        except BaseException:
+            self.free()
+            raise
+
+    def prove(self, channels):
+        """Proves once per channel (a fresh Channel for each None), in order, with up to
+        len(backends) proofs in flight.  Returns [(felts, proof dict)], one per channel."""
+        channels = [c if c is not None else Channel(self.graphs[0].be.lib) for c in channels]
+        k = len(self.graphs)
+        out = []
+        for i, ch in enumerate(channels):
+            if i >= k:   # the slot's previous proof is finished before its graph runs again
+                out.append(self.graphs[(i - k) % k].finish(channels[i - k]))
+            self.graphs[i % k].start(ch)
+        for i in range(max(0, len(channels) - k), len(channels)):
+            out.append(self.graphs[i % k].finish(channels[i]))
+        return out
+
+    def free(self):
+        for pg in self.graphs:
+            pg.free()
+        self.graphs = []
 
 
 def commit_traces_batched(backend, twiddles, traces, log_sizes, log_blowup):

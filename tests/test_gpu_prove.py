@@ -148,3 +148,41 @@ def test_start_wide_fibonacci_replays_the_captured_commit(ctx, backend):
     felts2, _ = finish_wide_fibonacci(scheme2, ch2, n_cols, log_n)
     felts1, _ = finish_wide_fibonacci(scheme1, ch1, n_cols, log_n)
     assert felts1.tolist() == exp and felts2.tolist() == exp
+
+
+def test_proof_pipeline_two_contexts(ctx, backend):
+    """ProofPipeline: proofs of one trace with two in flight, on two contexts each replaying its
+    own proof graph.  With fresh channels every proof is the oracle's; with channels that start
+    from other states each proof and final channel state equal the host-channel prover's for the
+    same starting state."""
+    from proving_amd.backend import Channel, Context, HipBackend
+    from proving_amd.prover import ProofPipeline
+    n_cols, log_n, fri = 12, 9, params(6, 1, 9)
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    trace = wide_fib_trace(n_cols, log_n, 33)
+    exp = PR.wide_fib_prove(trace, log_n, fri, tw, itw)[0]
+    cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
+    tree = backend.precompute_canonic_twiddles(log_n + 1)
+    ctx2 = Context(0)
+    try:
+        pipe = ProofPipeline([backend, HipBackend(ctx2)], cols, log_n, fri)
+        try:
+            for felts, _ in pipe.prove([None] * 5):
+                assert felts.tolist() == exp
+            chans = []
+            for i in range(5):
+                c = Channel(backend.lib)
+                c.mix_u64(1000 + i)
+                chans.append(c)
+            got = pipe.prove(chans)
+            for i, ((felts, _), c) in enumerate(zip(got, chans)):
+                h = Channel(backend.lib)
+                h.mix_u64(1000 + i)
+                want, _, h = prove_wide_fibonacci(backend, tree, cols, log_n, fri, channel=h, device_channel=False)
+                assert np.array_equal(felts, want), f"proof {i}"
+                assert bytes(c.s.digest) == bytes(h.s.digest) and c.s.n_draws == h.s.n_draws, f"channel {i}"
+            assert len({bytes(c.s.digest) for c in chans}) == 5
+        finally:
+            pipe.free()
+    finally:
+        ctx2.close()
