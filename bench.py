@@ -63,12 +63,52 @@ def algorithmic_work(logs, log_blowup):
     lde = sum(1 << (lg + log_blowup) for lg in logs)
     L = max(logs) + log_blowup
     return {
-        "fft_ifft_pass": {"bytes": 8 * cells},
-        "fft_rfft_pass": {"bytes": 4 * cells + 4 * lde},
+        # the interpolation (8 B per cell) and the extension (4 B per coefficient + 4 B per
+        # extended word), whichever launches run their layers
+        "fft_passes": {"bytes": 12 * cells + 4 * lde},
         "merkle_leaves_group": {"bytes": 4 * lde + 32 * (1 << L),
                                 "compressions": lifted_compressions(logs, log_blowup)},
         "merkle_next_layers": {"bytes": 96 * ((1 << L) - 1), "compressions": (1 << L) - 1},
     }
+
+
+def fft_plan(top, tune):
+    """The layer ranges [lo, hi) of a transform's passes, as the library plans them (plan_passes in
+    proving_amd/csrc/fft.hip) from a tuning entry (block_log, stride_layers, r_block, r_stride)."""
+    block, stride = tune[0], tune[1]
+    rem = 0
+    if top > block:
+        rem = (top - block) % stride
+        if rem != 0 and rem <= 2 and block + rem <= 13:
+            block, rem = block + rem, 0
+    bh = min(top, block)
+    passes, lo = [(0, bh)], bh
+    while lo < top:
+        g = min(top - lo, stride)
+        if rem != 0 and rem <= 2 and stride < top - lo < 2 * stride:
+            g = (top - lo + 1) // 2
+        passes.append((lo, lo + g))
+        lo += g
+    return passes
+
+
+def fft_butterflies(logs, b, tuning):
+    """Butterflies per commit in each FFT launch class: the interpolation's passes, the fused pass
+    (the interpolation's last strided pass and the extension's first, when their plans share those
+    layers and a fused kernel has that width: 4..8 layers) and the extension's passes."""
+    out = {"fft_ifft_pass": 0, "fft_ie_pass": 0, "fft_rfft_pass": 0}
+    for m in logs:
+        pi = fft_plan(m, tuning[m]) if m > 2 else []
+        pe = fft_plan(m, tuning[m + b]) if m > 2 else []
+        fused = (len(pi) >= 2 and len(pe) >= 2 and pi[-1] == pe[-1] and pi[-1][1] == m
+                 and 4 <= pi[-1][1] - pi[-1][0] <= 8)
+        for k, (lo, hi) in enumerate(pi):
+            cls = "fft_ie_pass" if fused and k == len(pi) - 1 else "fft_ifft_pass"
+            out[cls] += (hi - lo) << (m - 1)
+        for k, (lo, hi) in enumerate(pe):
+            cls = "fft_ie_pass" if fused and k == len(pe) - 1 else "fft_rfft_pass"
+            out[cls] += (hi - lo) << (m + b - 1)
+    return out
 
 
 def profiled_traffic(kernel, tuning):
@@ -602,29 +642,39 @@ def main():
                 entry["valu_tops"] = (w["compressions"] * B2_OPS_PER_COMPRESSION
                                       / (v["ms"] / args.steps / 1e3) / 1e12)
         kernels[k] = entry
-    # dominant HBM-bound kernel class (the FFT passes) for the roofline entry
-    hbm_kernels = [k for k in ("fft_rfft_pass", "fft_ifft_pass") if k in prof]
-    dom = max(hbm_kernels, key=lambda k: prof[k]["ms"])
-    n_launch = prof[dom]["n"] / args.steps
-    bytes_per_launch = work_per[dom]["bytes"] / n_launch
-    avg_s = prof[dom]["ms"] / prof[dom]["n"] / 1e3
+    # roofline of the dominant kernel class: the FFT passes (the interpolation's, the extension's
+    # and the fused pass between them, sh_set_fft_fusion) taken together, since which layers run in
+    # which launch depends on the pass plan; algorithmic bytes: the interpolation 8 B per cell, the
+    # extension 4 B per coefficient + 4 B per extended word
+    fft_classes = [k for k in ("fft_ifft_pass", "fft_ie_pass", "fft_rfft_pass") if k in prof]
+    fft_ms = sum(prof[k]["ms"] for k in fft_classes)
+    n_launch = sum(prof[k]["n"] for k in fft_classes) / args.steps
+    bytes_per_launch = work_per["fft_passes"]["bytes"] / n_launch
+    avg_s = fft_ms / (n_launch * args.steps) / 1e3
     achieved = bytes_per_launch / avg_s / 1e9
     # the FFT passes are bound by integer VALU issue, not HBM (the counters in
     # profiles/commit_uneven512_hbm_traffic.md): an M31 butterfly is 11 VALU instructions of
     # arithmetic (multiply by a doubled twiddle 5, add 3, subtract 3), priced against the 32-bit
     # VALU peak; the HBM fraction of the same launches is reported beside it
-    bfly = {"fft_ifft_pass": sum(lg << (lg - 1) for lg in logs),
-            "fft_rfft_pass": sum(lg << (lg + b - 1) for lg in logs)}
-    valu_ops = VALU_OPS_PER_BUTTERFLY * bfly[dom] / n_launch
+    bfly = sum(lg << (lg - 1) for lg in logs) + sum(lg << (lg + b - 1) for lg in logs)
+    valu_ops = VALU_OPS_PER_BUTTERFLY * bfly / n_launch
     valu_tops = valu_ops / avg_s / 1e12
     tuning_now = {str(m): list(be.fft_tuning(m)) for m in sorted(set(logs) | {lg + b for lg in logs})}
-    traffic, same_shapes = profiled_traffic(dom, tuning_now) if args.workload == "uneven" else (None, None)
-    roofline = {"kernel": dom, "bound": "valu", "achieved": round(valu_tops, 2), "peak": VALU_PEAK_TOPS,
-                "unit": "TOP/s", "frac": round(valu_tops / VALU_PEAK_TOPS, 4), "traffic": traffic,
-                "ops_per_butterfly": VALU_OPS_PER_BUTTERFLY, "butterflies_per_launch": bfly[dom] / n_launch,
-                "avg_launch_us": round(avg_s * 1e6, 2)}
+    traffic, same_shapes = profiled_traffic("fft_passes", tuning_now) if args.workload == "uneven" else (None, None)
+    roofline = {"kernel": "fft passes (" + ", ".join(fft_classes) + ")", "bound": "valu",
+                "achieved": round(valu_tops, 2), "peak": VALU_PEAK_TOPS, "unit": "TOP/s",
+                "frac": round(valu_tops / VALU_PEAK_TOPS, 4), "traffic": traffic,
+                "ops_per_butterfly": VALU_OPS_PER_BUTTERFLY, "butterflies_per_launch": bfly / n_launch,
+                "launches_per_step": n_launch, "avg_launch_us": round(avg_s * 1e6, 2)}
     roofline["hbm"] = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": bytes_per_launch}
+    # each launch class's own VALU fraction, from the butterflies its passes run in this plan
+    per_cls = fft_butterflies(logs, b, {m: be.fft_tuning(m) for m in set(logs) | {lg + b for lg in logs}})
+    roofline["by_class"] = {k: {"ms_per_step": round(prof[k]["ms"] / args.steps, 4),
+                                "butterflies_per_step": per_cls[k],
+                                "valu_frac": round(VALU_OPS_PER_BUTTERFLY * per_cls[k]
+                                                   / (prof[k]["ms"] / args.steps / 1e3) / 1e12 / VALU_PEAK_TOPS, 4)}
+                            for k in fft_classes}
     if traffic is not None:
         roofline["traffic_source"] = ("profiles/commit_uneven512_hbm_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + "
                                       "WRITE_SIZE x 1.0 per dispatch)")

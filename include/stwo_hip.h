@@ -90,6 +90,15 @@ int sh_bit_reverse_column(sh_ctx *ctx, uint32_t *d_col, uint32_t log_size);
 int sh_set_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t block_log, uint32_t stride_layers,
                       uint32_t r_block, uint32_t r_stride);
 int sh_get_fft_tuning(sh_ctx *ctx, uint32_t log_size, uint32_t out[4]);
+/* Interpolation followed by extension of the same columns (sh_commit_trace;
+ * no counterpart in the reference, whose SIMD backend reads the coefficients
+ * back for each subdomain): mode 0 runs them as separate transforms; mode 1
+ * (the default) runs the interpolation's last strided pass and the
+ * extension's first as one pass when their pass plans give them the same
+ * layers, so the coefficients are written once and never read back; mode 2
+ * also plans the extension with the interpolation's layer split so that they
+ * always do.  Results never depend on it. */
+int sh_set_fft_fusion(sh_ctx *ctx, uint32_t mode);
 /* Autotuner (no counterpart in the reference): for the column sizes of a
  * workload, times candidate pass shapes of every interpolation and
  * evaluation size it uses, on batches of the workload's width, and installs

@@ -71,6 +71,7 @@ SIGNATURES = {
     "sh_grind": (c_int, [c_vp, c_vp, c_u32, c_u64p]),
     "sh_grind_dchannel": (c_int, [c_vp, c_vp, c_u32, c_vp]),
     "sh_set_fft_tuning": (c_int, [c_vp, c_u32, c_u32, c_u32, c_u32, c_u32]),
+    "sh_set_fft_fusion": (c_int, [c_vp, c_u32]),
     "sh_get_fft_tuning": (c_int, [c_vp, c_u32, c_vp]),
     "sh_autotune_fft": (c_int, [c_vp, c_vp, c_vp, c_u64, c_u32]),
     "sh_commit_trace_graph_create": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_u64, c_u32, c_vp, c_vp,

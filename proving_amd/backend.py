@@ -559,6 +559,12 @@ class HipBackend:
         self._chk(self.lib.sh_set_fft_tuning(self.h, log_size, block_log, stride_layers, r_block, r_stride),
                   "set_fft_tuning")
 
+    def set_fft_fusion(self, mode):
+        """Interpolation + extension in commit_trace: 0 separate transforms, 1 the interpolation's
+        last strided pass fused with the extension's first where their plans share those layers
+        (default), 2 the extension planned with the interpolation's layer split (always fused)."""
+        self._chk(self.lib.sh_set_fft_fusion(self.h, mode), "set_fft_fusion")
+
     # ---- batched trace commitment (crates/stwo/src/prover/pcs/mod.rs:325-386) -------------------
     def commit_trace(self, evals, log_blowup, twiddles, coeffs=None, ldes=None, layers=None,
                      want_root=True, mixed=False):

@@ -15,11 +15,9 @@
 
 #include "common.h"
 
-int sh_interpolate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
-                           uint32_t* const* h_dst, const uint32_t* log_sizes, uint64_t n);
-int sh_evaluate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
-                        const uint32_t* coeff_logs, const uint32_t* domain_logs, uint32_t* const* h_dst,
-                        uint64_t n);
+int sh_interpolate_extend_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
+                                  uint32_t* const* h_coef, uint32_t* const* h_ext, uint32_t m, uint32_t b,
+                                  uint64_t n);
 int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint32_t* log_sizes, uint64_t n,
                          uint32_t L, uint32_t* d_out, const hipEvent_t* group_ready);
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers);
@@ -107,11 +105,8 @@ static int commit_forked(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* con
         co.push_back(d_coeffs[i]);
         ld.push_back(d_ldes[i]);
       }
-    std::vector<uint32_t> cl(ev.size(), m), dl(ev.size(), m + log_blowup);
     StreamScope scope(ctx, fft_stream);
-    SH_TRY(sh_interpolate_grouped(ctx, tw, ev.data(), co.data(), cl.data(), ev.size()));
-    SH_TRY(sh_evaluate_grouped(ctx, tw, (const uint32_t* const*)co.data(), cl.data(), dl.data(), ld.data(),
-                               ev.size()));
+    SH_TRY(sh_interpolate_extend_grouped(ctx, tw, ev.data(), co.data(), ld.data(), m, log_blowup, ev.size()));
     ready[k] = sh_sync_event(ctx, 1 + k);
     SH_CHECK_HIP(ctx, hipEventRecord(ready[k], ctx->stream));
   }

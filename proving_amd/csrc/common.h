@@ -88,6 +88,10 @@ struct sh_ctx {
     uint8_t r_block, r_stride;  // rows bits per lane of the quad-layout passes (fft2.hip)
   } fft_tune[32];
   bool fft2 = true;  // quad-layout pass kernels where a specialisation exists
+  // interpolation + extension of the same columns: 0 separate, 1 the interpolation's last strided
+  // pass and the extension's first run as one pass where their plans share those layers, 2 the
+  // extension also takes the interpolation's layer split so that they always do
+  uint8_t fft_fuse = 1;
 };
 
 // Every staged copy is complete (the caller has synchronised the context's streams).

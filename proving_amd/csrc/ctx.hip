@@ -566,6 +566,13 @@ extern "C" int sh_set_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t block_
   return SH_OK;
 }
 
+extern "C" int sh_set_fft_fusion(sh_ctx* ctx, uint32_t mode) {
+  if (!ctx) return SH_ERR_INVALID;
+  SH_REQUIRE(ctx, mode <= 2, "FFT fusion mode %u not in [0, 2]", mode);
+  ctx->fft_fuse = (uint8_t)mode;
+  return SH_OK;
+}
+
 extern "C" int sh_get_fft_tuning(sh_ctx* ctx, uint32_t log_size, uint32_t out[4]) {
   if (!ctx || !out || log_size >= 32) return SH_ERR_INVALID;
   const auto& t = ctx->fft_tune[log_size];

@@ -417,6 +417,10 @@ int sh_fft2_strided(sh_ctx* ctx, bool inverse, uint32_t g, uint32_t r, uint32_t 
                     uint32_t* const* d_dst, uint32_t n_cols, uint32_t m, uint32_t src_mask, const uint32_t* tw,
                     uint32_t K, uint32_t Pl, uint32_t scale);
 
+int sh_fft2_strided_ie(sh_ctx* ctx, uint32_t g, uint32_t r, uint32_t lo, uint32_t* const* d_coef,
+                       uint32_t* const* d_ext, uint32_t n_cols, uint32_t m, uint32_t nsub_log, const uint32_t* itw,
+                       const uint32_t* tw, uint32_t K, uint32_t scale);
+
 // One pass with the quad-layout kernels of fft2.hip (the tuned rows-per-lane first, then the
 // others); SH_ERR_NOT_FOUND when none of them has this shape.
 static int fft2_pass(sh_ctx* ctx, bool inverse, const FftPass& p, const sh_ctx::FftTune& t,
@@ -434,6 +438,38 @@ static int fft2_pass(sh_ctx* ctx, bool inverse, const FftPass& p, const sh_ctx::
     if (rc != SH_ERR_NOT_FOUND) return rc;
   }
   return SH_ERR_NOT_FOUND;
+}
+
+// One interpolation pass (layers [p.lo, p.hi) of a column of log m, domain of log Pl).
+static int ifft_pass(sh_ctx* ctx, const sh_twiddles* tw, const FftPass& p, const sh_ctx::FftTune& tune,
+                     const uint32_t* const* s, uint32_t* const* d, uint32_t n_cols, uint32_t m, uint32_t Pl,
+                     uint32_t scale) {
+  const uint32_t G = p.hi - p.lo;
+  SpanGuard span(ctx, "fft_ifft_pass");
+  const int rc2 = fft2_pass(ctx, true, p, tune, s, d, n_cols, m, 0xffffffffu, tw->itw, tw->root_log_size, Pl, scale);
+  if (rc2 != SH_ERR_NOT_FOUND) return rc2;
+  // the generic kernel: shapes with no quad-layout specialisation (blocks below 2^9 words)
+  dim3 grid(1u << (m - G - p.tl), n_cols);
+  const size_t lds = 4u << (G + p.tl);
+  k_fft_pass<true, 4><<<grid, kThreads, lds, ctx->stream>>>(s, d, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo,
+                                                            p.hi, p.tl, scale, 4u);
+  return sh_launch_check(ctx, "ifft_pass");
+}
+
+// One evaluation pass over a column of log m (reads through src_mask: the first pass replicates the
+// coefficients).
+static int rfft_pass(sh_ctx* ctx, const sh_twiddles* tw, const FftPass& p, const sh_ctx::FftTune& tune,
+                     const uint32_t* const* s, uint32_t* const* d, uint32_t n_cols, uint32_t m, uint32_t Pl,
+                     uint32_t src_mask) {
+  const uint32_t G = p.hi - p.lo;
+  SpanGuard span(ctx, "fft_rfft_pass");
+  const int rc2 = fft2_pass(ctx, false, p, tune, s, d, n_cols, m, src_mask, tw->tw, tw->root_log_size, Pl, 0u);
+  if (rc2 != SH_ERR_NOT_FOUND) return rc2;
+  dim3 grid(1u << (m - G - p.tl), n_cols);
+  const size_t lds = 4u << (G + p.tl);
+  k_fft_pass<false, 4><<<grid, kThreads, lds, ctx->stream>>>(s, d, src_mask, tw->tw, tw->root_log_size, Pl, p.lo,
+                                                             p.hi, p.tl, 0u, 4u);
+  return sh_launch_check(ctx, "rfft_pass");
 }
 
 // Interpolation of n_cols columns of log size m, all passes; src/dst are
@@ -464,26 +500,9 @@ int sh_fft_interpolate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t*
   uint32_t scale = sh::minv(1u << m);
   const sh_ctx::FftTune tune = ctx->fft_tune[m];
   auto passes = plan_passes(m, tune);
-  for (size_t k = 0; k < passes.size(); k++) {
-    const FftPass& p = passes[k];
-    uint32_t G = p.hi - p.lo;
-    dim3 grid(1u << (m - G - p.tl), n_cols);
-    size_t lds = 4u << (G + p.tl);
-    const uint32_t* const* s = k == 0 ? d_src : (const uint32_t* const*)d_dst;
-    SpanGuard span(ctx, "fft_ifft_pass");
-    const int rc2 = fft2_pass(ctx, true, p, tune, s, d_dst, n_cols, m, 0xffffffffu, tw->itw, tw->root_log_size, Pl,
-                              k + 1 == passes.size() ? scale : 0u);
-    if (rc2 != SH_ERR_NOT_FOUND) {
-      SH_TRY(rc2);
-      continue;
-    }
-    // the generic kernel: shapes with no quad-layout specialisation (blocks below 2^9 words)
-    auto kern = k_fft_pass<true, 4>;
-    kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, 0xffffffffu, tw->itw, tw->root_log_size, Pl, p.lo, p.hi,
-                                               p.tl, k + 1 == passes.size() ? scale : 0u,
-                                               4u);
-    SH_TRY(sh_launch_check(ctx, "ifft_pass"));
-  }
+  for (size_t k = 0; k < passes.size(); k++)
+    SH_TRY(ifft_pass(ctx, tw, passes[k], tune, k == 0 ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, m, Pl,
+                     k + 1 == passes.size() ? scale : 0u));
   return SH_OK;
 }
 
@@ -510,25 +529,56 @@ int sh_fft_evaluate_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* co
   const sh_ctx::FftTune tune = ctx->fft_tune[m];
   auto passes = plan_passes(n, tune);
   for (int k = (int)passes.size() - 1; k >= 0; k--) {
-    const FftPass& p = passes[k];
-    uint32_t G = p.hi - p.lo;
-    dim3 grid(1u << (m - G - p.tl), n_cols);
-    size_t lds = 4u << (G + p.tl);
-    bool first = k == (int)passes.size() - 1;
-    const uint32_t* const* s = first ? d_src : (const uint32_t* const*)d_dst;
-    SpanGuard span(ctx, "fft_rfft_pass");
-    const int rc2 = fft2_pass(ctx, false, p, tune, s, d_dst, n_cols, m, first ? src_mask : 0xffffffffu, tw->tw,
-                              tw->root_log_size, Pl, 0u);
-    if (rc2 != SH_ERR_NOT_FOUND) {
-      SH_TRY(rc2);
-      continue;
-    }
-    auto kern = k_fft_pass<false, 4>;
-    kern<<<grid, kThreads, lds, ctx->stream>>>(s, d_dst, first ? src_mask : 0xffffffffu, tw->tw,
-                                               tw->root_log_size, Pl, p.lo, p.hi, p.tl, 0u,
-                                               4u);
-    SH_TRY(sh_launch_check(ctx, "rfft_pass"));
+    const bool first = k == (int)passes.size() - 1;
+    SH_TRY(rfft_pass(ctx, tw, passes[k], tune, first ? d_src : (const uint32_t* const*)d_dst, d_dst, n_cols, m, Pl,
+                     first ? src_mask : 0xffffffffu));
   }
+  return SH_OK;
+}
+
+// Interpolation of n_cols columns of log m (canonic domain; d_src -> d_coef, in place allowed) and
+// their extension to log m + b (d_coef -> d_ext), with the interpolation's last strided pass and the
+// extension's first fused (sh_fft2_strided_ie) when the context's fusion mode and the two pass plans
+// allow; otherwise the two transforms one after the other.  Same words either way.
+int sh_fft_interpolate_extend_batch(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* d_src,
+                                    uint32_t* const* d_coef, uint32_t* const* d_ext, uint32_t n_cols, uint32_t m,
+                                    uint32_t b) {
+  if (n_cols == 0) return SH_OK;
+  const sh_ctx::FftTune ti = ctx->fft_tune[m], te = ctx->fft_tune[m + b];
+  std::vector<FftPass> pi, pe;
+  if (m > 2 && b >= 1) {
+    pi = plan_passes(m, ti);
+    pe = ctx->fft_fuse == 2 ? pi : plan_passes(m, te);
+  }
+  const bool fuse = ctx->fft_fuse != 0 && ctx->fft2 && pi.size() >= 2 && pe.size() >= 2 && m + b <= 30 &&
+                    pi.back().lo == pe.back().lo && pi.back().hi == m && pe.back().hi == m;
+  if (!fuse) {
+    SH_TRY(sh_fft_interpolate_batch(ctx, tw, d_src, d_coef, n_cols, m, m, sh::subgroup_gen(m + 1)));
+    return sh_fft_evaluate_batch(ctx, tw, (const uint32_t* const*)d_coef, d_ext, n_cols, m, m + b, m + b,
+                                 sh::subgroup_gen(m + b + 1));
+  }
+  const uint32_t scale = sh::minv(1u << m);
+  for (size_t k = 0; k + 1 < pi.size(); k++)
+    SH_TRY(ifft_pass(ctx, tw, pi[k], ti, k == 0 ? d_src : (const uint32_t* const*)d_coef, d_coef, n_cols, m, m, 0u));
+  const FftPass& last = pi.back();
+  int rc = SH_ERR_NOT_FOUND;
+  {
+    SpanGuard span(ctx, "fft_ie_pass");
+    const uint32_t order[6] = {te.r_stride, ti.r_stride, 3, 2, 4, 1};
+    for (uint32_t k = 0; k < 6 && rc == SH_ERR_NOT_FOUND; k++)
+      rc = sh_fft2_strided_ie(ctx, last.hi - last.lo, order[k], last.lo, d_coef, d_ext, n_cols, m, b, tw->itw, tw->tw,
+                              tw->root_log_size, scale);
+  }
+  if (rc == SH_ERR_NOT_FOUND) {  // no fused kernel of this shape: the two passes apart
+    SH_TRY(ifft_pass(ctx, tw, last, ti, pi.size() == 1 ? d_src : (const uint32_t* const*)d_coef, d_coef, n_cols, m, m,
+                     scale));
+    SH_TRY(rfft_pass(ctx, tw, pe.back(), te, (const uint32_t* const*)d_coef, d_ext, n_cols, m + b, m + b,
+                     (1u << m) - 1));
+  } else {
+    SH_TRY(rc);
+  }
+  for (int k = (int)pe.size() - 2; k >= 0; k--)
+    SH_TRY(rfft_pass(ctx, tw, pe[k], te, (const uint32_t* const*)d_ext, d_ext, n_cols, m + b, m + b, 0xffffffffu));
   return SH_OK;
 }
 
@@ -591,6 +641,25 @@ int sh_evaluate_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* cons
                                  sh::subgroup_gen(c.second + 1)));
   }
   return SH_OK;
+}
+
+// One size class of a batched commit: interpolation of n columns of log m (h_src -> h_coef) and their
+// extension to log m + b (h_coef -> h_ext) through sh_fft_interpolate_extend_batch.
+int sh_interpolate_extend_grouped(sh_ctx* ctx, const sh_twiddles* tw, const uint32_t* const* h_src,
+                                  uint32_t* const* h_coef, uint32_t* const* h_ext, uint32_t m, uint32_t b,
+                                  uint64_t n) {
+  if (n == 0) return SH_OK;
+  SH_TRY(check_domain(ctx, tw, m));
+  SH_TRY(check_domain(ctx, tw, m + b));
+  std::vector<const void*> ptrs;
+  for (uint64_t i = 0; i < n; i++) ptrs.push_back(h_src[i]);
+  for (uint64_t i = 0; i < n; i++) ptrs.push_back(h_coef[i]);
+  for (uint64_t i = 0; i < n; i++) ptrs.push_back(h_ext[i]);
+  void* d_tab;
+  SH_TRY(sh_stage(ctx, ptrs.data(), ptrs.size() * sizeof(void*), &d_tab));
+  const void** t = (const void**)d_tab;
+  return sh_fft_interpolate_extend_batch(ctx, tw, (const uint32_t* const*)t, (uint32_t* const*)(t + n),
+                                         (uint32_t* const*)(t + 2 * n), (uint32_t)n, m, b);
 }
 
 extern "C" int sh_interpolate(sh_ctx* ctx, const sh_twiddles* tw, uint32_t* d_values, uint32_t log_size) {

@@ -328,6 +328,143 @@ __global__ __launch_bounds__(1 << (G + TL - 2 - R)) void k_fft_strided(const uin
                                    tw, Kt, Pl, scale);
 }
 
+// Twiddles of a strided stage's RK layers (row bits [S, S + RK)), as strided_stage loads them.
+template <int G, int RK, int S>
+__device__ __forceinline__ void stage_twiddles(uint32_t (&t)[1 << RK], const uint32_t* tw, uint32_t Kt, uint32_t Pl,
+                                               uint32_t lo, uint32_t g_hi, uint32_t j0) {
+#pragma unroll
+  for (int b = 0; b < RK; b++) {
+    const uint32_t i = lo + S + b;
+    const uint32_t* tab = layer_tab(tw, Kt, Pl, i);
+    const uint32_t hb = (g_hi << (G - S - b - 1)) + (j0 >> (S + b + 1));
+    const int off = (1 << RK) - (1 << (RK - b));
+#pragma unroll
+    for (int eh = 0; eh < (1 << (RK - 1 - b)); eh++) t[off + eh] = tab[hb + eh] << 1;
+  }
+  t[(1 << RK) - 1] = 0;
+}
+
+// Interpolation stages K .. KEND - 1 of a strided pass (processing order = stage order), the
+// first reading src from HBM, all writing the LDS.
+template <bool INV, int G, int TL, int R, int K, int KEND>
+__device__ __forceinline__ void strided_stages_upto(uint32_t lds_base, const uint32_t* src, uint32_t* dst,
+                                                    uint32_t base, uint32_t lo, const uint32_t* tw, uint32_t Kt,
+                                                    uint32_t Pl) {
+  if constexpr (K < KEND) {
+    strided_stage<INV, G, TL, R, K, K == 0, false>(lds_base, src, 0xffffffffu, dst, base, 0u, lo, tw, Kt, Pl, 0u);
+    if constexpr (K + 1 < KEND) __syncthreads();
+    strided_stages_upto<INV, G, TL, R, K + 1, KEND>(lds_base, src, dst, base, lo, tw, Kt, Pl);
+  }
+}
+
+// Evaluation stages in processing order K .. NS - 1 (stage NS - 1 - K), from the LDS; the last
+// writes dst.
+template <int G, int TL, int R, int K>
+__device__ __forceinline__ void strided_stages_from(uint32_t lds_base, uint32_t* dst, uint32_t base, uint32_t g_hi,
+                                                    uint32_t lo, const uint32_t* tw, uint32_t Kt, uint32_t Pl) {
+  constexpr int NS = (G + R - 1) / R;
+  if constexpr (K < NS) {
+    strided_stage<false, G, TL, R, NS - 1 - K, false, K == NS - 1>(lds_base, nullptr, 0xffffffffu, dst, base, g_hi,
+                                                                   lo, tw, Kt, Pl, 0u);
+    if constexpr (K + 1 < NS) __syncthreads();
+    strided_stages_from<G, TL, R, K + 1>(lds_base, dst, base, g_hi, lo, tw, Kt, Pl);
+  }
+}
+
+// ---------------------------------------------------------------- interpolation + extension
+// A column of 2^m words whose interpolation ends with a strided pass over layers [lo, m) and whose
+// extension to 2^(m + nsub_log) words starts with a strided pass over the same layers (the
+// extension's layers m.. only replicate the coefficients).  One workgroup per tile of the column:
+// the interpolation's stages run as in k_fft_strided<true>; the last one (the pivot, row bits
+// [(NS - 1) R, G)) scales its words, writes them as coefficients and keeps them in registers; the
+// extension's stages then run once per subdomain h < 2^nsub_log, the first of them (the same row
+// bits, so the same lane owns the same quads) from those registers, the last writing the
+// subdomain's words at h * 2^m.  Coefficients are read once (as the interpolation's input) and
+// written once, never read back for the extension.
+template <int G, int TL, int R>
+__global__ __launch_bounds__(1 << (G + TL - 2 - R)) void k_fft_strided_ie(
+    uint32_t* const* coef_cols, uint32_t* const* ext_cols, const uint32_t* __restrict__ itw,
+    const uint32_t* __restrict__ tw, uint32_t Kt, uint32_t m, uint32_t nsub_log, uint32_t lo, uint32_t scale) {
+  constexpr int NQ = 1 << (G + TL - 2);
+  constexpr int NS = (G + R - 1) / R;
+  constexpr int SP = (NS - 1) * R;                  // first row bit of the pivot stage
+  constexpr int RK = G - SP;                        // its row bits (1..R)
+  constexpr int QS = SP + TL - 2;
+  constexpr int NT = 1 << (G + TL - 2 - R);
+  constexpr int TASKS = 1 << (G + TL - 2 - RK);
+  constexpr int IT = TASKS / NT;
+  __shared__ v4u lds[NS > 1 ? padq(NQ) : 1];
+  const uint32_t lds_base = (uint32_t)(size_t)(lds_quad*)lds;
+  const uint32_t g_lo0 = blockIdx.x << TL;          // lo + G = m: one tile row of high bits
+  uint32_t* coef = coef_cols[blockIdx.y];
+  uint32_t* ext = ext_cols[blockIdx.y];
+  // interpolation, stages 0 .. NS - 2 (the first reads the column from HBM)
+  if constexpr (NS > 1) {
+    strided_stages_upto<true, G, TL, R, 0, NS - 1>(lds_base, coef, coef, g_lo0, lo, itw, Kt, m);
+    __syncthreads();
+  }
+  uint32_t c[IT][1 << RK][4];
+#pragma unroll
+  for (int it = 0; it < IT; it++) {
+    const uint32_t tau = threadIdx.x + it * NT;
+    const uint32_t q0 = lane_quad(tau, QS, RK);
+    const uint32_t j0 = q0 >> (TL - 2);
+    const uint32_t goff = g_lo0 + (j0 << lo) + ((q0 & ((1u << (TL - 2)) - 1)) << 2);
+    uint32_t (&v)[1 << RK][4] = c[it];
+    if constexpr (NS > 1) {
+      lds_read<RK>(v, lds_base + 16u * padq(q0), QS);
+    } else {
+#pragma unroll
+      for (int e = 0; e < (1 << RK); e++) {
+        const v4u x = *(const gquad*)((const gword*)coef + goff + ((uint32_t)e << (SP + lo)));
+        v[e][0] = x.x;
+        v[e][1] = x.y;
+        v[e][2] = x.z;
+        v[e][3] = x.w;
+      }
+    }
+    uint32_t t[1 << RK];
+    stage_twiddles<G, RK, SP>(t, itw, Kt, m, lo, 0u, j0);
+    row_layers<true, RK>(v, t);
+    scale_rows(&v[0][0], 4 << RK, scale);
+#pragma unroll
+    for (int e = 0; e < (1 << RK); e++)
+      *(gquad*)((gword*)coef + goff + ((uint32_t)e << (SP + lo))) = v4u{v[e][0], v[e][1], v[e][2], v[e][3]};
+  }
+  // extension, one subdomain at a time
+  const uint32_t Pf = m + nsub_log;
+  for (uint32_t h = 0; h < (1u << nsub_log); h++) {
+    if (h > 0 && NS > 1) __syncthreads();  // the previous subdomain's last stage has read the LDS
+    const uint32_t base = (h << m) | g_lo0;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+      const uint32_t tau = threadIdx.x + it * NT;
+      const uint32_t q0 = lane_quad(tau, QS, RK);
+      const uint32_t j0 = q0 >> (TL - 2);
+      uint32_t v[1 << RK][4];
+#pragma unroll
+      for (int e = 0; e < (1 << RK); e++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[e][q] = c[it][e][q];
+      uint32_t t[1 << RK];
+      stage_twiddles<G, RK, SP>(t, tw, Kt, Pf, lo, h, j0);
+      row_layers<false, RK>(v, t);
+      if constexpr (NS > 1) {
+        lds_write<RK>(v, lds_base + 16u * padq(q0), QS);
+      } else {
+        const uint32_t goff = base + (j0 << lo) + ((q0 & ((1u << (TL - 2)) - 1)) << 2);
+#pragma unroll
+        for (int e = 0; e < (1 << RK); e++)
+          *(gquad*)((gword*)ext + goff + ((uint32_t)e << (SP + lo))) = v4u{v[e][0], v[e][1], v[e][2], v[e][3]};
+      }
+    }
+    if constexpr (NS > 1) {
+      __syncthreads();
+      strided_stages_from<G, TL, R, 1>(lds_base, ext, base, h, lo, tw, Kt, Pf);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- dispatch
 typedef void (*block_fn)(const uint32_t* const*, uint32_t* const*, uint32_t, const uint32_t*, uint32_t, uint32_t,
                          uint32_t);
@@ -400,6 +537,37 @@ strided_fn strided_lookup(int g, int r) {
 }
 uint32_t strided_tl_host(uint32_t g) { return g >= 5 ? 13 - g : 8; }
 
+typedef void (*ie_fn)(uint32_t* const*, uint32_t* const*, const uint32_t*, const uint32_t*, uint32_t, uint32_t,
+                      uint32_t, uint32_t, uint32_t);
+template <int G, int R>
+ie_fn ie_inst() {
+  constexpr int TL = strided_tl<G>();
+  if constexpr (R <= G && G + TL - 2 - R <= 10 && G + TL - 2 - R >= 6)
+    return k_fft_strided_ie<G, TL, R>;
+  else
+    return nullptr;
+}
+template <int G>
+ie_fn ie_by_r(int r) {
+  switch (r) {
+    case 1: return ie_inst<G, 1>();
+    case 2: return ie_inst<G, 2>();
+    case 3: return ie_inst<G, 3>();
+    case 4: return ie_inst<G, 4>();
+    default: return nullptr;
+  }
+}
+ie_fn ie_lookup(int g, int r) {
+  switch (g) {
+    case 4: return ie_by_r<4>(r);
+    case 5: return ie_by_r<5>(r);
+    case 6: return ie_by_r<6>(r);
+    case 7: return ie_by_r<7>(r);
+    case 8: return ie_by_r<8>(r);
+    default: return nullptr;
+  }
+}
+
 }  // namespace
 
 // Launches one block pass (layers [0, g)) over n_cols columns of 2^m words, or returns
@@ -425,4 +593,20 @@ int sh_fft2_strided(sh_ctx* ctx, bool inverse, uint32_t g, uint32_t r, uint32_t 
   hipLaunchKernelGGL(f, grid, dim3(1u << (g + tl - 2 - r)), 0, ctx->stream, d_src, d_dst, src_mask, tw, K, Pl, lo,
                      scale);
   return sh_launch_check(ctx, "fft_strided");
+}
+
+// The interpolation's last strided pass (layers [lo, m), scaled by `scale`) fused with the first
+// strided pass of the extension of the same columns to 2^(m + nsub_log) words: d_coef holds the
+// interpolation's intermediate and receives the coefficients, d_ext receives the extension's
+// intermediate.  SH_ERR_NOT_FOUND (nothing launched) when no kernel has that shape.
+int sh_fft2_strided_ie(sh_ctx* ctx, uint32_t g, uint32_t r, uint32_t lo, uint32_t* const* d_coef,
+                       uint32_t* const* d_ext, uint32_t n_cols, uint32_t m, uint32_t nsub_log, const uint32_t* itw,
+                       const uint32_t* tw, uint32_t K, uint32_t scale) {
+  ie_fn f = ie_lookup((int)g, (int)r);
+  const uint32_t tl = strided_tl_host(g);
+  if (!f || lo < tl || lo + g != m || nsub_log == 0 || m + nsub_log > 31) return SH_ERR_NOT_FOUND;
+  dim3 grid(1u << (m - g - tl), n_cols);
+  hipLaunchKernelGGL(f, grid, dim3(1u << (g + tl - 2 - r)), 0, ctx->stream, d_coef, d_ext, itw, tw, K, m, nsub_log,
+                     lo, scale);
+  return sh_launch_check(ctx, "fft_strided_ie");
 }

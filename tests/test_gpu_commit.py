@@ -88,6 +88,47 @@ def test_autotune_keeps_results(ctx, backend):
     assert r1 == root and r2 == root
 
 
+@pytest.mark.parametrize("m,b,tune", [
+    (13, 1, (9, 4, 2, 1)),    # last strided pass: 4 layers, 2 rows per lane
+    (14, 1, (9, 5, 2, 2)),    # 5 layers, 4 rows
+    (16, 2, (10, 6, 2, 3)),   # 6 layers, 8 rows, extension onto 4 subdomains
+    (17, 1, (10, 7, 2, 4)),   # 7 layers, 16 rows
+    (20, 1, (12, 8, 2, 3)),   # the 256 x 2^20 trace's shape
+    (20, 1, (12, 8, 2, 2)),
+    (19, 1, (11, 4, 2, 4)),   # two strided passes; the last one a single register stage
+])
+def test_fused_interpolate_extend(ctx, backend, m, b, tune):
+    """The interpolation's last strided pass fused with the extension's first (sh_set_fft_fusion):
+    every coefficient and extension word, and the root, equal the separate transforms' and the
+    oracle's, for each fused pass shape (layers 4..8, 2..16 rows per lane, one register stage)."""
+    saved = {k: backend.fft_tuning(k) for k in (m, m + b)}
+    n = 3
+    host = O.splitmix64_columns(40 + m, [m] * n)
+    tree = backend.precompute_canonic_twiddles(m + b)
+    evals = [BaseColumn.from_cpu(ctx, c) for c in host]
+    try:
+        for k in (m, m + b):
+            backend.set_fft_tuning(k, *tune)
+        out = {}
+        for mode in (0, 1, 2):
+            backend.set_fft_fusion(mode)
+            co, ld, _, root = backend.commit_trace(evals, b, tree)
+            out[mode] = ([c.to_cpu() for c in co], [x.to_cpu() for x in ld], root)
+        tw, itw = O.canonic_twiddles(m + b)
+        co_exp = O.interpolate(itw, host[1], m)
+        np.testing.assert_array_equal(out[1][0][1], co_exp)
+        np.testing.assert_array_equal(out[1][1][1], O.evaluate(tw, co_exp, m + b))
+        for mode in (0, 2):
+            for i in range(n):
+                np.testing.assert_array_equal(out[mode][0][i], out[1][0][i], err_msg=f"coeffs mode {mode} col {i}")
+                np.testing.assert_array_equal(out[mode][1][i], out[1][1][i], err_msg=f"lde mode {mode} col {i}")
+            assert out[mode][2] == out[1][2]
+    finally:
+        backend.set_fft_fusion(1)
+        for k, t in saved.items():
+            backend.set_fft_tuning(k, *t)
+
+
 def test_pool_reuses_freed_buffers(ctx, backend):
     """sh_free returns a block to the context pool; the next allocation of that size gets it back
     (no device sync), and work through reused blocks is unchanged."""
