@@ -182,6 +182,11 @@ def _tune(args, be, logs, log_blowup, tree):
     """FFT pass shapes: the committed table (proving_amd/fft_tuning_gfx950.json, written by
     python -m proving_amd.autotune) unless --autotune asks for a live autotune or there is no
     table; --no-autotune keeps the built-in shapes.  Returns whether shapes were set."""
+    if args.fft_tuning:
+        for m, t in json.loads(args.fft_tuning).items():
+            be.set_fft_tuning(int(m), *[int(x) for x in t])
+        log("[bench] FFT pass shapes set from --fft-tuning")
+        return True
     if args.no_autotune:
         return False
     if not args.autotune and be.load_fft_tuning():
@@ -344,8 +349,8 @@ def bench_prove(args):
     # host finishes one proof while the next one's device work runs)
     ctx2 = Context(0)
     be2 = HipBackend(ctx2)
-    if not args.no_autotune:
-        be2.load_fft_tuning()
+    for m in range(32):   # the same pass shapes as the first context's
+        be2.set_fft_tuning(m, *be.fft_tuning(m))
     pipe = ProofPipeline([be, be2], trace, n, w["fri"])
     for f, _ in pipe.prove([None] * 2):   # warm-up
         assert np.array_equal(f, first), "pipelined and single proofs differ"
@@ -477,8 +482,8 @@ def main():
                          "of autotuning (the 'fft_tuning' field of an earlier run), so that a profiled run "
                          "launches exactly the measured kernels")
     ap.add_argument("--no-plain", action="store_true",
-                    help="skip the plain per-column and host-to-device variants (their smaller FFT "
-                         "launches would mix into a profiler's per-kernel averages)")
+                    help="skip the plain per-column, unfused-FFT and host-to-device variants (their "
+                         "FFT launches would mix into a profiler's per-kernel averages)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device-to-device once (counter calibration run) and exit")
     args = ap.parse_args()
@@ -570,6 +575,20 @@ def main():
 
     variants = {"graph_batched_ms_per_step": round(ms_per_step, 4),
                 "eager_batched_ms_per_step": round(sec_eager * 1e3 / args.steps, 4)}
+
+    # (2a) the same graph with the interpolation and the extension as separate transforms (no fused
+    # pass): what the fusion is worth on this box
+    if not args.no_plain:
+        be.set_fft_fusion(0)
+        ugraph = be.commit_trace_graph(evals, b, tree, coeffs, ldes, layers)
+        for _ in range(args.warmup):
+            ugraph.launch()
+        sec_u = timed(ugraph.launch, args.steps)
+        assert ugraph.root() == root0, "unfused commit differs"
+        ugraph.free()
+        be.set_fft_fusion(1)
+        variants["unfused_graph_ms_per_step"] = round(sec_u * 1e3 / args.steps, 4)
+        log(f"[bench] graph batched, interpolation and extension unfused: {sec_u * 1e3 / args.steps:.3f} ms/step")
 
     # (2b) the same pipeline with the mixed-degree tree that injects the smaller columns at their
     # matching layer (MerkleOps::commit_on_layer), also captured as one graph
