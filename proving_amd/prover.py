@@ -880,12 +880,22 @@ class ProofPipeline:
         channels = [c if c is not None else Channel(self.graphs[0].be.lib) for c in channels]
         k = len(self.graphs)
         out = []
-        for i, ch in enumerate(channels):
-            if i >= k:   # the slot's previous proof is finished before its graph runs again
-                out.append(self.graphs[(i - k) % k].finish(channels[i - k]))
-            self.graphs[i % k].start(ch)
-        for i in range(max(0, len(channels) - k), len(channels)):
-            out.append(self.graphs[i % k].finish(channels[i]))
+        try:
+            for i, ch in enumerate(channels):
+                if i >= k:   # the slot's previous proof is finished before its graph runs again
+                    out.append(self.graphs[(i - k) % k].finish(channels[i - k]))
+                self.graphs[i % k].start(ch)
+            for i in range(max(0, len(channels) - k), len(channels)):
+                out.append(self.graphs[i % k].finish(channels[i]))
+        except BaseException:
+            # a proof that failed (ConstraintsNotSatisfied, a device error) leaves the others of
+            # this call unfinished: wait for them so that the pipeline is idle again
+            for pg in self.graphs:
+                try:
+                    pg.be.ctx.sync()
+                except Exception:
+                    pass
+            raise
         return out
 
     def free(self):
