@@ -150,8 +150,9 @@ def test_start_wide_fibonacci_replays_the_captured_commit(ctx, backend):
     assert felts1.tolist() == exp and felts2.tolist() == exp
 
 
-def test_proof_pipeline_two_contexts(ctx, backend):
-    """ProofPipeline: proofs of one trace with two in flight, on two contexts each replaying its
+@pytest.mark.parametrize("n_ctx", [2, 3])
+def test_proof_pipeline_contexts(ctx, backend, n_ctx):
+    """ProofPipeline: proofs of one trace with n_ctx in flight, on n_ctx contexts each replaying its
     own proof graph.  With fresh channels every proof is the oracle's; with channels that start
     from other states each proof and final channel state equal the host-channel prover's for the
     same starting state."""
@@ -163,9 +164,9 @@ def test_proof_pipeline_two_contexts(ctx, backend):
     exp = PR.wide_fib_prove(trace, log_n, fri, tw, itw)[0]
     cols = [BaseColumn.from_cpu(ctx, c) for c in trace]
     tree = backend.precompute_canonic_twiddles(log_n + 1)
-    ctx2 = Context(0)
+    extra = [Context(0) for _ in range(n_ctx - 1)]
     try:
-        pipe = ProofPipeline([backend, HipBackend(ctx2)], cols, log_n, fri)
+        pipe = ProofPipeline([backend] + [HipBackend(c) for c in extra], cols, log_n, fri)
         try:
             for felts, _ in pipe.prove([None] * 5):
                 assert felts.tolist() == exp
@@ -185,4 +186,5 @@ def test_proof_pipeline_two_contexts(ctx, backend):
         finally:
             pipe.free()
     finally:
-        ctx2.close()
+        for c in extra:
+            c.close()
