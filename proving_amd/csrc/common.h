@@ -94,6 +94,19 @@ struct sh_ctx {
   uint8_t fft_fuse = 1;
 };
 
+// The FRI channel step run by the launch that computes a tree's root (merkle.hip): mix the root
+// into the device channel ch (digest, n_draws) and draw alpha, writing its k squared powers to
+// alphas (4 words each) and a copy of the root to root_out.  ch == nullptr: no step.
+struct sh_root_step {
+  uint32_t* ch;
+  uint32_t* alphas;
+  uint32_t* root_out;
+  uint32_t k;
+};
+// Layers L-1 .. 0 of a tree from layer L; with st, the launch that computes the root also runs
+// the channel step (a tree of one leaf, L == 0, has no such launch: the caller steps itself).
+int sh_build_layers_step_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers, const sh_root_step* st);
+
 // Every staged copy is complete (the caller has synchronised the context's streams).
 void sh_stage_all_complete(sh_ctx* ctx);
 

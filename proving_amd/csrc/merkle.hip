@@ -219,10 +219,13 @@ constexpr uint64_t kQuadMaxNodes = 1ull << kQuadMaxLog;
 // Up to 7 levels below an input layer of 2^in_log hashes, four lanes per node: a workgroup of
 // 256 lanes (64 quads) reduces 128 input hashes to 64 nodes, then its own 64 to 1 through LDS;
 // level l (2^(in_log-1-l) nodes) goes to outs.p[l].  For fewer than 64 nodes one workgroup
-// takes them all.
+// takes them all.  When st.ch is set (only for the one-workgroup launch that computes the root),
+// the workgroup then runs the FRI channel step on the root: Blake2sChannel::mix_root and
+// draw_secure_felt (crates/stwo/src/core/channel/blake2s.rs:33-117) with four lanes per hash,
+// and the k squared powers of alpha (crates/stwo/src/prover/fri.rs:79-87).
 constexpr uint32_t kQuadGroupNodes = 64;
 __global__ __launch_bounds__(256) void k_next_layers_quad(const uint32_t* __restrict__ prev, uint32_t in_log,
-                                                          uint32_t levels, LayerOuts outs) {
+                                                          uint32_t levels, LayerOuts outs, sh_root_step st) {
   __shared__ uint32_t buf_a[2 * kQuadGroupNodes * 8];
   __shared__ uint32_t buf_b[kQuadGroupNodes * 8];
   const uint32_t tid = threadIdx.x, q = tid & 3, quad = tid >> 2;
@@ -253,6 +256,42 @@ __global__ __launch_bounds__(256) void k_next_layers_quad(const uint32_t* __rest
     uint32_t* t = cur;
     cur = nxt;
     nxt = t;
+  }
+  if (st.ch) {  // uniform: the barriers below are reached by the whole workgroup
+    __shared__ uint32_t msg[16], digest[8], draw[8];
+    uint32_t dh_lo, dh_hi, d_draw;
+    quad_init(q, 37, dh_lo, dh_hi, d_draw);  // digest || LE32(n_draws) || 0x00
+    if (tid < 16) msg[tid] = tid < 8 ? st.ch[tid] : cur[tid - 8];
+    if (tid < 8) st.root_out[tid] = cur[tid];
+    __syncthreads();
+    if (tid < 4) {  // digest = Blake2s(digest || root)
+      uint32_t lo, hi;
+      quad_node_hash(msg, sidx, q, h_lo, h_hi, d0, lo, hi);
+      digest[q] = lo;
+      digest[4 + q] = hi;
+    }
+    __syncthreads();
+    if (tid < 16) msg[tid] = tid < 8 ? digest[tid] : 0u;  // n_draws = 0 after the mix
+    __syncthreads();
+    if (tid < 4) {  // the draw: Blake2s(digest || LE32(0) || 0x00)
+      uint32_t lo, hi;
+      quad_node_hash(msg, sidx, q, dh_lo, dh_hi, d_draw, lo, hi);
+      draw[q] = lo;
+      draw[4 + q] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t f[8], w[4];
+      for (int i = 0; i < 8; i++) f[i] = sh::reduce_once((draw[i] & sh::P) + (draw[i] >> 31));  // M31::reduce(u32)
+      for (int i = 0; i < 4; i++) w[i] = sh::madd(sh::madd(f[2 * i], f[2 * i + 1]), f[2 * i + 1]);
+      sh::qm31 a = sh::qload(w);
+      for (uint32_t j = 0; j < st.k; j++) {
+        sh::qstore(a, st.alphas + 4 * j);
+        a = sh::qmul(a, a);
+      }
+      for (int i = 0; i < 8; i++) st.ch[i] = digest[i];
+      st.ch[8] = 1;
+    }
   }
 }
 
@@ -466,6 +505,11 @@ int sh_build_leaves_impl(sh_ctx* ctx, const uint32_t* const* h_cols, const uint3
 
 // Layers below the leaves: layer k-1 from layer k for k = L..1; layers[k] has 2^k hashes.
 int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers) {
+  return sh_build_layers_step_impl(ctx, L, layers, nullptr);
+}
+
+int sh_build_layers_step_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers, const sh_root_step* st) {
+  const sh_root_step none = {nullptr, nullptr, nullptr, 0};
   uint32_t k = L;
   while (k > 0) {
     const uint64_t n0 = 1ull << (k - 1);  // nodes of the first level of this launch
@@ -485,7 +529,9 @@ int sh_build_layers_impl(sh_ctx* ctx, uint32_t L, uint32_t* const* layers) {
       const uint32_t levels = std::min<uint32_t>(7, k);
       for (uint32_t l = 0; l < levels; l++) outs.p[l] = layers[k - 1 - l];
       const unsigned grid = (unsigned)((n0 + kQuadGroupNodes - 1) / kQuadGroupNodes);
-      k_next_layers_quad<<<grid, 256, 0, ctx->stream>>>(layers[k], k, levels, outs);
+      // the launch that computes the root (levels == k) is a single workgroup: it runs the step
+      k_next_layers_quad<<<grid, 256, 0, ctx->stream>>>(layers[k], k, levels, outs,
+                                                         st && levels == k ? *st : none);
       SH_TRY(sh_launch_check(ctx, "next_layers_quad"));
       k -= levels;
     }

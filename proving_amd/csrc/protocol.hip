@@ -269,15 +269,22 @@ int alloc_tree(sh_ctx* ctx, uint32_t log_size, DeviceTree& t) {
 
 // leaves_ready: t is allocated already and its leaf layer written (by the fold that made the
 // columns); only the layers above it are built.
+// st: the FRI channel step, run by the launch that computes the root (k_fri_channel for a tree of
+// one leaf).
+__global__ void k_fri_channel(const uint32_t* root, uint32_t* ch, uint32_t k, uint32_t* alphas, uint32_t* root_out);
 int commit_columns(sh_ctx* ctx, const std::vector<const uint32_t*>& cols, uint32_t log_size, DeviceTree& t,
-                   bool fetch_root = true, bool leaves_ready = false) {
+                   bool fetch_root = true, bool leaves_ready = false, const sh_root_step* st = nullptr) {
   if (!leaves_ready) {
     SH_TRY(alloc_tree(ctx, log_size, t));
     std::vector<uint32_t> logs(cols.size(), log_size);
     SH_TRY(sh_build_leaves_impl(ctx, cols.data(), logs.data(), cols.size(), log_size,
                                 (uint32_t*)t.layers[log_size], nullptr));
   }
-  SH_TRY(sh_build_layers_impl(ctx, log_size, (uint32_t* const*)t.layers.data()));
+  SH_TRY(sh_build_layers_step_impl(ctx, log_size, (uint32_t* const*)t.layers.data(), st));
+  if (st && log_size == 0) {
+    k_fri_channel<<<1, 64, 0, ctx->stream>>>((const uint32_t*)t.layers[0], st->ch, st->k, st->alphas, st->root_out);
+    SH_TRY(sh_launch_check(ctx, "fri_channel"));
+  }
   if (!fetch_root) return SH_OK;
   SH_CHECK_HIP(ctx, hipMemcpyAsync(t.root.data(), t.layers[0], 32, hipMemcpyDeviceToHost, ctx->stream));
   SH_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -301,8 +308,8 @@ __global__ void k_pack_leaves(const uint32_t* c0, const uint32_t* c1, const uint
 // 357-368): a tree over the 4 coordinate columns, one leaf per value, or with packed leaves one
 // leaf per 4 consecutive values (16 words: one Blake2s block) and a tree 2 levels shorter.
 int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_size, bool pack, DeviceTree& t,
-                     bool fetch_root = true) {
-  if (!pack) return commit_columns(ctx, {cols[0], cols[1], cols[2], cols[3]}, log_size, t, fetch_root);
+                     bool fetch_root = true, const sh_root_step* st = nullptr) {
+  if (!pack) return commit_columns(ctx, {cols[0], cols[1], cols[2], cols[3]}, log_size, t, fetch_root, false, st);
   const uint32_t n_out = 1u << (log_size - 2);
   uint32_t* packed;
   SH_TRY(sh_dev_alloc(ctx, 64ull * n_out, (void**)&packed));
@@ -311,7 +318,7 @@ int commit_fri_layer(sh_ctx* ctx, const uint32_t* const cols[4], uint32_t log_si
   if (rc == SH_OK) {
     std::vector<const uint32_t*> pc(16);
     for (int k = 0; k < 16; k++) pc[k] = packed + (uint64_t)k * n_out;
-    rc = commit_columns(ctx, pc, log_size - 2, t, fetch_root);
+    rc = commit_columns(ctx, pc, log_size - 2, t, fetch_root, false, st);
   }
   sh_dev_free(ctx, packed);  // stream-ordered: a later allocation reuses it after the leaves read it
   return rc;
@@ -808,7 +815,8 @@ struct sh_fri_run {
 
 // FriProver::commit (crates/stwo/src/prover/fri.rs:113-226), enqueued without waiting for the
 // device: each layer's root is mixed into a device copy of the channel and the folding alphas are
-// drawn there (k_fri_channel), so the folds read them from device memory.  When the proof of work
+// drawn there, by the launch that computes the root (k_next_layers_quad's channel step), so the
+// folds read them from device memory.  When the proof of work
 // fits the device search (pow_bits <= 32) and the last layer has at most 2^10 values, the last
 // layer's coefficients are computed and mixed and the proof of work is ground on the device too
 // (FriProver::decommit's first steps, fri.rs:229-244).  The host replays the same channel
@@ -887,10 +895,10 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
     if (r != SH_OK) tmp.free_all();
     return r;
   };
-  auto channel_step = [&](const DeviceTree& t, uint32_t layer, uint32_t k) -> int {
-    k_fri_channel<<<1, 64, 0, ctx->stream>>>((const uint32_t*)t.layers[0], d_ch, k, d_alphas + 16 * layer,
-                                              d_roots + 8 * layer);
-    return sh_launch_check(ctx, "fri_channel");
+  // the channel step of layer `layer` (mix its root, draw alpha, k squared powers), run by the
+  // launch that computes the layer's root
+  auto channel_step = [&](uint32_t layer, uint32_t k) {
+    return sh_root_step{d_ch, d_alphas + 16 * layer, d_roots + 8 * layer, k};
   };
   auto alloc_eval = [&](FriLayer& l, uint32_t lg) -> int {
     l.log_size = lg;
@@ -910,8 +918,10 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
   first.log_size = log_size;
   first.fold_step = fs;
   first.pack = log_size >= 2 && fs > 1;
-  TRY_RUN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree, false));
-  TRY_RUN(channel_step(first.tree, 0, fs));
+  {
+    const sh_root_step st = channel_step(0, fs);
+    TRY_RUN(commit_fri_layer(ctx, column, log_size, first.pack, first.tree, false, &st));
+  }
   // commit_inner_layers (fri.rs:145-196): the circle-to-line fold with alpha, then fold_step - 1
   // line folds with alpha^2, alpha^4, ... (one multi-step launch)
   uint32_t line_log = log_size - 1;
@@ -988,12 +998,12 @@ static int fri_commit_impl(sh_ctx* ctx, const sh_twiddles* tw, const sh_channel*
     const uint32_t k = line_log > last_log + fs ? fs : line_log - last_log;
     cur.fold_step = k;
     cur.pack = line_log >= 2 && k > 1;
+    const sh_root_step st = channel_step(n_layers, k);
     if (cur.leaves_done)
       TRY_RUN(commit_columns(ctx, {cur.cols[0], cur.cols[1], cur.cols[2], cur.cols[3]}, line_log, cur.tree, false,
-                             true));
+                             true, &st));
     else
-      TRY_RUN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false));
-    TRY_RUN(channel_step(cur.tree, n_layers, k));
+      TRY_RUN(commit_fri_layer(ctx, cur.cols, line_log, cur.pack, cur.tree, false, &st));
     TRY_RUN(fail_tmp(alloc_eval(tmp, line_log - k)));
     uint32_t* next_leaves = nullptr;
     if (k == 1) TRY_RUN(fail_tmp(prepare_leaves(tmp, line_log - 1, &next_leaves)));

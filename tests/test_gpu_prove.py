@@ -188,3 +188,34 @@ def test_proof_pipeline_contexts(ctx, backend, n_ctx):
     finally:
         for c in extra:
             c.close()
+
+
+def test_proof_pipeline_bad_trace_then_recovers(ctx, backend):
+    """A trace that breaks a constraint, proved through ProofPipeline: the proof raises
+    ConstraintsNotSatisfied, the pipeline is left idle, and once the buffers hold a good trace
+    again its proofs are the oracle's."""
+    from proving_amd.backend import Context, HipBackend
+    from proving_amd.prover import ProofPipeline
+    n_cols, log_n, fri = 12, 9, params(6, 1, 9)
+    tw, itw = O.canonic_twiddles(log_n + 1)
+    good = wide_fib_trace(n_cols, log_n, 35)
+    exp = PR.wide_fib_prove(good, log_n, fri, tw, itw)[0]
+    bad = [np.array(c) for c in good]
+    bad[4][33] = (int(bad[4][33]) + 1) % P
+    cols = [BaseColumn.from_cpu(ctx, c) for c in good]
+    ctx2 = Context(0)
+    try:
+        pipe = ProofPipeline([backend, HipBackend(ctx2)], cols, log_n, fri)
+        try:
+            for c, h in zip(cols, bad):
+                c.upload(h)
+            with pytest.raises(ConstraintsNotSatisfied):
+                pipe.prove([None] * 3)
+            for c, h in zip(cols, good):
+                c.upload(h)
+            for felts, _ in pipe.prove([None] * 3):
+                assert felts.tolist() == exp
+        finally:
+            pipe.free()
+    finally:
+        ctx2.close()
