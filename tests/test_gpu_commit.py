@@ -96,6 +96,7 @@ def test_autotune_keeps_results(ctx, backend):
     (20, 1, (12, 8, 2, 3)),   # the 256 x 2^20 trace's shape
     (20, 1, (12, 8, 2, 2)),
     (19, 1, (11, 4, 2, 4)),   # two strided passes; the last one a single register stage
+    (14, 3, (9, 5, 2, 2)),    # extension onto 8 subdomains
 ])
 def test_fused_interpolate_extend(ctx, backend, m, b, tune):
     """The interpolation's last strided pass fused with the extension's first (sh_set_fft_fusion):
