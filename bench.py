@@ -610,6 +610,32 @@ def main():
     variants["mixed_root"] = mroot0.hex()
     log(f"[bench] graph batched, mixed-degree tree: {sec_m * 1e3 / args.steps:.3f} ms/step")
     if not args.no_plain:
+        # (2c) two commits of the same columns in flight, each on a context of its own (its own
+        # graph and output buffers): how much of the device one commit leaves idle
+        ctx2 = Context(0)
+        be2 = HipBackend(ctx2)
+        for m in range(32):
+            be2.set_fft_tuning(m, *be.fft_tuning(m))
+        tree2 = be2.precompute_canonic_twiddles(L)
+        out2 = (ColumnSet(ctx2, logs), ColumnSet(ctx2, [lg + b for lg in logs]), HashLayerSet(ctx2, L))
+        graph2 = be2.commit_trace_graph(evals, b, tree2, *out2)
+        for _ in range(args.warmup):
+            graph.launch()
+            graph2.launch()
+        ctx.sync()
+        ctx2.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.launch()
+            graph2.launch()
+        ctx.sync()
+        ctx2.sync()
+        sec_2 = time.perf_counter() - t0
+        assert graph2.root() == root0 and graph.root() == root0, "concurrent commits differ"
+        graph2.free()
+        ctx2.close()
+        variants["two_in_flight_ms_per_commit"] = round(sec_2 * 1e3 / (2 * args.steps), 4)
+        log(f"[bench] two commits in flight (two contexts): {sec_2 * 1e3 / (2 * args.steps):.3f} ms per commit")
         # (3) host-inclusive: the columns start in (pageable) host memory and cross PCIe first
         h2d_steps = max(1, args.steps // 4)
         def h2d_step():
