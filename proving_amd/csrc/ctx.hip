@@ -329,7 +329,8 @@ int sh_dev_alloc(sh_ctx* ctx, uint64_t bytes, void** out) {
         return sh_fail(ctx, SH_ERR_INVALID,
                        "graph capture: allocation %zu (%llu bytes) differs from the warm-up's", a->cursor,
                        (unsigned long long)cls);
-      fl.erase(std::find(fl.begin(), fl.end(), p));
+      const auto f = std::find(fl.begin(), fl.end(), p);  // a block not in use is on its free list
+      if (f != fl.end()) fl.erase(f);
       b->second.second = true;
       a->cursor++;
       *out = p;
