@@ -720,10 +720,11 @@ class _Ops:
                   "wide_fibonacci_composition")
         return accum
 
-    def wide_fibonacci_trace(self, c0, c1, n_cols):
+    def wide_fibonacci_trace(self, c0, c1, n_cols, out=None):
         """generate_trace (crates/examples/src/wide_fibonacci/mod.rs:27-48) on the device from the
-        two input columns; returns n_cols BaseColumns."""
-        cols = [BaseColumn(self.ctx, c0.log_size) for _ in range(n_cols)]
+        two input columns; returns n_cols BaseColumns (out: existing columns to write)."""
+        cols = out if out is not None else [BaseColumn(self.ctx, c0.log_size) for _ in range(n_cols)]
+        assert len(cols) == n_cols and all(c.log_size == c0.log_size for c in cols)
         self._chk(self.lib.sh_wide_fibonacci_trace(self.h, c0.ptr, c1.ptr, c0.log_size, n_cols, _ptrs(cols)),
                   "wide_fibonacci_trace")
         return cols

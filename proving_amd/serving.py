@@ -12,6 +12,7 @@ Worker contexts are independent (own stream and buffer pool); in the batched sch
 committed on the scheduler's context are handed to a worker after the scheduler's stream is
 synchronised, and are released only after the worker has synchronised on its last use.
 """
+import collections
 import queue
 import threading
 import time
@@ -20,7 +21,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .backend import BaseColumn, Channel, Context, HipBackend
-from .prover import commit_traces_batched, finish_wide_fibonacci, prove_wide_fibonacci, start_wide_fibonacci
+from .prover import (_ProofGraph, commit_traces_batched, finish_wide_fibonacci, prove_wide_fibonacci,
+                     start_wide_fibonacci)
 
 
 @dataclass
@@ -50,6 +52,66 @@ def device_bytes(n_cols, log_n, log_blowup=1):
     return 4 * cells + 4 * (cells << log_blowup) + tree + comp + fri + (8 << log_n)
 
 
+class ShapeGraphs:
+    """One worker's proof graphs, one per request shape (columns, log rows) it has proved, each with
+    its own input and trace buffers: a request of a shape seen before has its two input columns
+    uploaded and its trace generated into that shape's buffers and is proved by a replay of the
+    shape's captured proof (_ProofGraph); the first request of a shape is proved eagerly, the
+    second builds the graph.  Shapes beyond `cap` bytes (device_bytes estimates) are dropped least
+    recently used first; a shape larger than the cap is not cached (prove returns None).  The
+    proofs are the same bytes as prove_wide_fibonacci's."""
+
+    def __init__(self, backend, twiddles, fri, cap):
+        self.be, self.tw, self.fri, self.cap = backend, twiddles, dict(fri), cap
+        self.entries = collections.OrderedDict()
+        self.bytes = 0
+
+    def _drop(self, key):
+        e = self.entries.pop(key)
+        self.bytes -= e["bytes"]
+        if e["pg"] is not None:
+            e["pg"].free()
+        for c in e["trace"] + [e["c0"], e["c1"]]:
+            c.free()
+
+    def prove(self, req):
+        key = (req.n_cols, req.log_n)
+        e = self.entries.get(key)
+        if e is None:
+            cost = device_bytes(req.n_cols, req.log_n, self.fri["log_blowup_factor"])
+            if cost > self.cap:
+                return None
+            while self.entries and self.bytes + cost > self.cap:
+                self._drop(next(iter(self.entries)))
+            ctx = self.be.ctx
+            e = {"c0": BaseColumn(ctx, req.log_n), "c1": BaseColumn(ctx, req.log_n),
+                 "trace": [BaseColumn(ctx, req.log_n) for _ in range(req.n_cols)], "pg": None, "seen": False,
+                 "bytes": cost}
+            self.entries[key] = e
+            self.bytes += cost
+        self.entries.move_to_end(key)
+        e["c0"].upload(req.c0)
+        e["c1"].upload(req.c1)
+        self.be.wide_fibonacci_trace(e["c0"], e["c1"], req.n_cols, out=e["trace"])
+        if e["pg"] is not None:
+            return e["pg"].prove(Channel(self.be.lib))[0]
+        if not e["seen"]:
+            e["seen"] = True
+            return prove_wide_fibonacci(self.be, self.tw, e["trace"], req.log_n, self.fri, graph_commit=False)[0]
+        pg = _ProofGraph(self.be, self.tw, e["trace"], req.log_n, self.fri, key)
+        try:
+            felts = pg.build(Channel(self.be.lib))[0]
+        except BaseException:
+            pg.free()
+            raise
+        e["pg"] = pg
+        return felts
+
+    def free(self):
+        while self.entries:
+            self._drop(next(iter(self.entries)))
+
+
 class ProofServer:
     """The request loop.  Two scheduling modes, both admitting by the HBM budget and keeping the
     twiddle tree resident:
@@ -69,8 +131,11 @@ class ProofServer:
     whole_proofs=True selects "lanes", False "batched" (the names the tests use)."""
 
     def __init__(self, twiddle_log, fri, n_workers=2, hbm_budget=64 << 30, max_in_flight=None, max_wait=0.05,
-                 whole_proofs=True):
+                 whole_proofs=True, graph_cache_bytes=24 << 30):
         self.fri = dict(fri)
+        # lanes: each worker keeps a proof graph per request shape (ShapeGraphs), within this many
+        # bytes of device memory beside the budget (0: every proof eager)
+        self.graph_cache_bytes = graph_cache_bytes
         self.mode = "lanes" if whole_proofs else "batched"
         self.whole_proofs = whole_proofs
         self.twiddle_log = twiddle_log
@@ -105,22 +170,29 @@ class ProofServer:
         be = HipBackend(ctx)
         be.load_fft_tuning()
         tw = be.precompute_canonic_twiddles(self.twiddle_log) if self.mode == "lanes" else None
+        graphs = ShapeGraphs(be, tw, self.fri, self.graph_cache_bytes) \
+            if self.mode == "lanes" and self.graph_cache_bytes > 0 else None
         ctx.sync()
         self._ready.release()
         while True:
             item = self._queues[k].get()
             if item is None:
+                if graphs is not None:
+                    graphs.free()
                 if tw is not None:
                     tw.free()
+                be.release_graphs()
                 ctx.close()
                 return
             req, scheme, channel, cost = item
             try:
                 if scheme is None:              # the whole proof on this worker's context
-                    c0, c1 = BaseColumn.from_cpu(ctx, req.c0), BaseColumn.from_cpu(ctx, req.c1)
-                    trace = be.wide_fibonacci_trace(c0, c1, req.n_cols)
-                    # every request's trace is new: no commit graph to replay
-                    req.felts, _, _ = prove_wide_fibonacci(be, tw, trace, req.log_n, self.fri, graph_commit=False)
+                    felts = graphs.prove(req) if graphs is not None else None
+                    if felts is None:
+                        c0, c1 = BaseColumn.from_cpu(ctx, req.c0), BaseColumn.from_cpu(ctx, req.c1)
+                        trace = be.wide_fibonacci_trace(c0, c1, req.n_cols)
+                        felts = prove_wide_fibonacci(be, tw, trace, req.log_n, self.fri, graph_commit=False)[0]
+                    req.felts = felts
                 else:
                     scheme.be = be              # continue the proof on this worker's context
                     req.felts, _ = finish_wide_fibonacci(scheme, channel, req.n_cols, req.log_n)

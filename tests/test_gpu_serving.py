@@ -35,6 +35,32 @@ def test_each_proof_is_the_same_alone_and_in_the_mix(whole_proofs):
     ctx.close()
 
 
+@pytest.mark.parametrize("cap_shapes", [64, 1], ids=["every_shape_cached", "one_shape_cached"])
+def test_repeated_shapes_replay_their_proof_graphs(cap_shapes):
+    """Requests of a few shapes, each seen several times: a worker proves the first request of a
+    shape eagerly, builds the shape's proof graph on the second and replays it afterwards (with a
+    cache of one shape, shapes are dropped and rebuilt as they alternate).  Every proof equals the
+    same request proved alone and verifies."""
+    reqs = make_requests(12, 41, col_choices=(4, 16), log_range=(6, 7))
+    shapes = {(r.n_cols, r.log_n) for r in reqs}
+    assert len(shapes) < len(reqs)
+    cap = cap_shapes * max(device_bytes(c, lg) for c, lg in shapes)
+    server = ProofServer(12, FRI, n_workers=2, hbm_budget=1 << 34, graph_cache_bytes=cap)
+    try:
+        mixed = server.serve(reqs, realtime=False)
+    finally:
+        server.close()
+    alone = make_requests(12, 41, col_choices=(4, 16), log_range=(6, 7))
+    ctx = Context(0)
+    be = HipBackend(ctx)
+    tw = be.precompute_canonic_twiddles(12)
+    prove_sequential(be, tw, alone, FRI, realtime=False)
+    for m, a in zip(mixed, alone):
+        assert np.array_equal(m.felts, a.felts), f"request {m.index} differs"
+        assert PR.wide_fib_verify(m.felts.tolist(), m.n_cols, m.log_n, FRI)
+    ctx.close()
+
+
 # The serving workload itself: column counts from {32, 64, 128, 256}, 2^14..2^20 rows, seed 4,
 # production FriParams::new(26, 0, 1, 70, 1) (crates/prover/src/prover.rs:385-396).
 C5_FRI = {"pow_bits": 26, "log_blowup_factor": 1, "log_last_layer_degree_bound": 0, "n_queries": 70,
