@@ -450,7 +450,8 @@ def bench_serve(args):
                       "scheduling": ("shortest job first with 50 ms aging, HBM-budget admission, batched trace commits"
                                      if args.serve_batched else
                                      "HBM-budget admission; whole proofs per worker context, each idle worker taking "
-                                     "the oldest waiting request")},
+                                     "the oldest waiting request and replaying a captured proof graph per request "
+                                     "shape")},
            "loads": loads, "capacity_plain_back_to_back": round(capacity, 2),
            "server_p99_le_plain": {k: v["server"]["p99_ms"] <= v["plain_sequential"]["p99_ms"] for k, v in loads.items()},
            "server_throughput_ge_plain": {k: v["server"]["proofs_per_s"] >= v["plain_sequential"]["proofs_per_s"]
